@@ -1,0 +1,253 @@
+"""Benchmark: all-pairs AccJerk on a Plummer sphere of N = 65536 on one GPU.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+A step is one all-pairs AccJerk evaluation (N x N interactions, self pairs
+included as the reference loops over them) on device-resident inputs.  The
+headline value is the split-FP32 path in Gint/s (1e9 pair interactions per
+second); the FP64 path is timed the same way beside it.  Kernel time comes
+from HIP events recorded by libtupan_hip.so around the all-pairs kernel on
+the stream it runs on; the step time is bracketed by device synchronisation.
+
+Also reported:
+  roofline      algorithmic FLOP/s of the dominant kernel (42 flops per pair,
+                the reference's count, acc_jerk_kernel_common.h:56) against
+                the vector peak that bounds it
+  energy        |dE/E| of a short ahermite4 run in FP64 and in split-FP32 from
+                the same initial state
+  sakura        Kepler solves/s of one Sakura pair-drift call (flag -1,
+                dt 1/64) on a Plummer sphere of N = 16384 with eps = 0
+  cpu_baseline  the CPU oracle (serial C99 restatement of the reference C
+                backend, same operation order, gcc -O3 -std=c99) on a bounded
+                sample of i-rows against all N j-particles, scaled per pair
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+FLOPS_PER_PAIR = 42          # acc_jerk_kernel_common.h:56
+PEAK_FP64 = 78.6e12          # MI355X vector FP64
+PEAK_FP32 = 157.3e12         # MI355X vector FP32 (counts packed FP32)
+HBM_PEAK = 8.0e12
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=65536)
+    ap.add_argument("--energy-steps", type=int, default=20)
+    ap.add_argument("--sakura-n", type=int, default=16384)
+    ap.add_argument("--cpu-rows", type=int, default=0,
+                    help="i-rows of the CPU baseline sample (0 = auto)")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def time_accjerk(ps, mode, steps, warmup):
+    import torch
+    from tupan_amd import _native
+    from tupan_amd.lib import extensions
+    ext = extensions.AccJerk("HIP", "float64", mode=mode)
+    for _ in range(warmup):
+        ext.calc(ps, ps)
+    torch.cuda.synchronize()
+    _native.timing(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ext.calc(ps, ps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kms, cnt = _native.timing_read()
+    _native.timing(False)
+    return wall / steps, kms / 1e3 / max(cnt, 1)
+
+
+def energy_run(ps0, mode, steps):
+    """ahermite4 (eta 0.01) for `steps` steps from a copy of ps0; returns
+    |dE/E| with E from FP64 Phi and exactly rounded sums."""
+    from tupan_amd.diagnostics import energy
+    from tupan_amd.integrator import Integrator
+    from tupan_amd.lib import extensions
+    from tupan_amd.lib.utils import ctype
+    ps = ps0.copy()
+    type(ps).t_curr = 0.0
+    e0 = energy(ps)
+    old = ctype.force_mode
+    ctype.force_mode = mode
+    extensions.install("HIP", "float64")
+    try:
+        it = Integrator(0.01, 0.0, ps, method="ahermite4")
+        for _ in range(steps):
+            it.evolve_step(1.0)
+        ps = it.particle_system
+        t_end = type(ps).t_curr
+    finally:
+        ctype.force_mode = old
+        extensions.install("HIP", "float64")
+    e1 = energy(ps)
+    return abs((e1 - e0) / e0), t_end
+
+
+def sakura_rate(n):
+    import torch
+    from tupan_amd import _native
+    from tupan_amd.ics.plummer import make_plummer
+    from tupan_amd.lib import extensions
+    ps = make_plummer(n, 0.0, ("equalmass",), seed=1, device="cuda:0")
+    ext = extensions.Sakura("HIP", "float64")
+    ext.calc(ps, ps, 1.0 / 64, -1)
+    torch.cuda.synchronize()
+    ext.kernel.counters = torch.zeros(4, dtype=torch.int64, device="cuda:0")
+    _native.timing(True)
+    t0 = time.perf_counter()
+    ext.calc(ps, ps, 1.0 / 64, -1)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kms, _ = _native.timing_read()
+    _native.timing(False)
+    solves, iters, retries, leap = [int(x) for x in
+                                    ext.kernel.counters.cpu().tolist()]
+    return {"n": n, "pairs": n * n, "kepler_solves": solves,
+            "laguerre_iterations": iters, "subdivisions": retries,
+            "leapfrog_pairs": leap, "kernel_s": kms / 1e3, "call_s": wall,
+            "solves_per_s": solves / (kms / 1e3),
+            "pairs_per_s": n * n / (kms / 1e3)}
+
+
+def cpu_baseline(ps, rows):
+    """Serial oracle on `rows` i-rows against all j; pairs/s -> Gint/s."""
+    import numpy as np
+    from oracle import Oracle
+    host = ps.to_numpy()
+    n = len(host["mass"])
+    sub = {k: v[:rows] for k, v in host.items()}
+    orc = Oracle("float64")
+    best = None
+    for _ in range(3):
+        t0 = time.perf_counter()
+        orc.acc_jerk(sub, host)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    rate = rows * n / best
+    omp = Oracle("float64", omp=True)
+    threads = omp.threads()
+    rows_omp = min(n, rows * threads)
+    subo = {k: v[:rows_omp] for k, v in host.items()}
+    best_o = None
+    for _ in range(2):
+        t0 = time.perf_counter()
+        omp.acc_jerk(subo, host)
+        dt = time.perf_counter() - t0
+        best_o = dt if best_o is None else min(best_o, dt)
+    rate_o = rows_omp * n / best_o
+    del np
+    return ({"value": rate / 1e9, "unit": "Gint/s", "cores": 1,
+             "kind": "port",
+             "sample": "%d i-rows x %d j of the same Plummer state, FP64, "
+                       "best of 3 (%.2f s)" % (rows, n, best)},
+            {"value": rate_o / 1e9, "unit": "Gint/s", "cores": threads,
+             "kind": "port",
+             "sample": "%d i-rows x %d j, OpenMP over i, best of 2 (%.2f s)"
+                       % (rows_omp, n, best_o)})
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(--gpus %d requested)" % args.gpus, file=sys.stderr)
+        return 2
+    import torch
+    from tupan_amd import _native
+    from tupan_amd.ics.plummer import make_plummer
+    assert torch.cuda.is_available(), "no HIP device"
+    cu, wg = _native.device_info()
+    n = args.n
+    log("bench: building Plummer N=%d eps=1e-4 seed=1" % n)
+    ps = make_plummer(n, 1e-4, ("equalmass",), seed=1, device="cuda:0")
+    pairs = float(n) * n
+
+    log("bench: split-FP32 AccJerk, %d steps" % args.steps)
+    t_split, k_split = time_accjerk(ps, "split32", args.steps, args.warmup)
+    log("bench: FP64 AccJerk, %d steps" % args.steps)
+    t_fp64, k_fp64 = time_accjerk(ps, "fp64", args.steps, args.warmup)
+    log("bench: FP64 (reference operation order) AccJerk")
+    t_ieee, k_ieee = time_accjerk(ps, "ieee64", max(2, args.steps // 4), 1)
+
+    out = {
+        "metric": "Gint/s AccJrk N=64k split-FP32 vs FP64 at |dE/E|<=1e-10; "
+                  "Sakura solves/s",
+        "value": pairs / t_split / 1e9,
+        "unit": "Gint/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_split * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-split (f64 sums)",
+        "data": "synthetic",
+        "config": {"workload": "plummer_accjerk", "n": n, "eps": 1e-4,
+                   "imf": "equalmass", "seed": 1,
+                   "step": "one all-pairs AccJerk evaluation"},
+        "fp64": {"gint_s": pairs / t_fp64 / 1e9, "ms_per_step": t_fp64 * 1e3,
+                 "kernel_ms": k_fp64 * 1e3},
+        "ieee64": {"gint_s": pairs / t_ieee / 1e9,
+                   "ms_per_step": t_ieee * 1e3, "kernel_ms": k_ieee * 1e3},
+        "split32": {"gint_s": pairs / t_split / 1e9,
+                    "ms_per_step": t_split * 1e3, "kernel_ms": k_split * 1e3},
+        "split_over_fp64": t_fp64 / t_split,
+        "device": {"cu": cu, "target_workgroups": wg},
+    }
+    flops = FLOPS_PER_PAIR * pairs
+    out["roofline"] = {
+        "bound": "valu",
+        "kernel": "pair_kernel<OpSplitAccJerk>",
+        "achieved": flops / k_split / 1e12,
+        "peak": PEAK_FP32 / 1e12,
+        "unit": "TFLOP/s",
+        "frac": flops / k_split / PEAK_FP32,
+        "traffic": None,
+        "fp64_kernel": {"kernel": "pair_kernel<OpAccJerk<Fast64>>",
+                        "achieved": flops / k_fp64 / 1e12,
+                        "peak": PEAK_FP64 / 1e12,
+                        "frac": flops / k_fp64 / PEAK_FP64},
+    }
+
+    if args.energy_steps > 0:
+        log("bench: energy check, %d ahermite4 steps per mode"
+            % args.energy_steps)
+        e64, t64 = energy_run(ps, "fp64", args.energy_steps)
+        e32, t32 = energy_run(ps, "split32", args.energy_steps)
+        out["energy"] = {"method": "ahermite4", "eta": 0.01,
+                         "steps": args.energy_steps, "t_end": t64,
+                         "dE_E_fp64": e64, "dE_E_split32": e32}
+
+    if args.sakura_n > 0:
+        log("bench: Sakura N=%d" % args.sakura_n)
+        out["sakura"] = sakura_rate(args.sakura_n)
+
+    if not args.no_cpu:
+        rows = args.cpu_rows or 2048
+        log("bench: CPU baseline, %d rows" % rows)
+        one, allc = cpu_baseline(ps, rows)
+        out["cpu_baseline"] = one
+        out["cpu_baseline_all_cores"] = allc
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,285 @@
+/*
+ * tupan_hip.h -- C ABI of libtupan_hip.so, the gfx950 (MI355X) implementation
+ * of tupan's all-pairs gravity kernels and Sakura two-body drift.
+ *
+ * Two layers:
+ *
+ * 1. Host-pointer layer: the exact names, argument orders and types of the
+ *    reference's FP64 C backend (tupan/lib/src/libtupan.h:2-246 built with
+ *    CONFIG_USE_DOUBLE: UINT = unsigned long, INT = long, REAL = double).
+ *    Arrays are host memory; each call copies its inputs to the GPU, runs
+ *    the FP64 kernel and copies the outputs back before returning, exactly
+ *    like the reference's synchronous cffi calls.  libtupan_hip_f32.so
+ *    exports the same names for the float build (UINT = unsigned int,
+ *    INT = int, REAL = float), as the reference builds one library per
+ *    precision (tupan/lib/cffi_backend.py:84-87).  These functions return
+ *    void like the reference; on failure they fill every output with NaN,
+ *    print the reason to stderr and leave it in tupan_last_error().
+ *
+ * 2. Device-resident layer (tupan_*): the same kernels on DEVICE pointers,
+ *    enqueued on a caller-supplied HIP stream (NULL = the null stream), so
+ *    particle arrays stay in HBM across steps.  Each returns TUPAN_OK or a
+ *    negative status; tupan_last_error() gives the message.  `mode` selects
+ *    the arithmetic of the FP64 entry points (TUPAN_MODE_*).
+ *
+ * Output arrays are overwritten, never accumulated into.  Kernels keep no
+ * pointer past the call.  The device layer uses a per-device scratch
+ * workspace for j-split partial sums, so calls on one device must not
+ * overlap in time on different streams.
+ */
+#ifndef TUPAN_HIP_H
+#define TUPAN_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define TUPAN_OK 0
+#define TUPAN_ERR_ARG (-1)     /* bad argument (null pointer, size, flag) */
+#define TUPAN_ERR_HIP (-2)     /* HIP runtime error (launch, copy, device) */
+#define TUPAN_ERR_ALLOC (-3)   /* device allocation failed */
+#define TUPAN_ERR_SOLVER (-4)  /* Kepler solver hit its iteration bound */
+
+/* Arithmetic modes of the FP64 device entry points. */
+#define TUPAN_MODE_FP64 0     /* FP64, refined hardware rsqrt, fused ops      */
+#define TUPAN_MODE_SPLIT32 1  /* split-FP32 pair arithmetic, FP64 sums (Acc,
+                                 AccJerk; other kernels fall back to FP64)    */
+#define TUPAN_MODE_IEEE64 2   /* FP64 in the reference's operation order with
+                                 IEEE division and square root               */
+
+const char *tupan_last_error(void);
+int tupan_version(void);
+/* Number of compute units of the current device, and the workgroup count the
+ * launch planner aims for; both -1 if no device. */
+int tupan_device_info(int *cu_count, int *target_workgroups);
+
+/* Define TUPAN_HIP_NO_FP64_REFERENCE_ABI before including this header to
+ * hide the FP64 reference-ABI names (the float build re-declares them with
+ * float types). */
+#ifndef TUPAN_HIP_NO_FP64_REFERENCE_ABI
+/* Kernel timing (no counterpart in the reference; used by the benchmark).
+ * tupan_timing(1) clears the record and brackets the main all-pairs kernel
+ * of every later call with HIP events on the stream it is launched on;
+ * tupan_timing_read waits for the recorded launches and returns the summed
+ * kernel time in milliseconds and the number of launches.  Timing state is
+ * process-wide and not thread-safe. */
+int tupan_timing(int enable);
+int tupan_timing_read(double *total_ms, int *count);
+
+/* ---------------------------------------------------------------------------
+ * Host-pointer layer (reference ABI, FP64).  libtupan.h line numbers cite
+ * the declaration each function replaces.
+ * ------------------------------------------------------------------------- */
+
+/* libtupan.h:2-15 */
+void phi_kernel(const unsigned long ni,
+                const double *_im, const double *_irx, const double *_iry,
+                const double *_irz, const double *_ie2,
+                const unsigned long nj,
+                const double *_jm, const double *_jrx, const double *_jry,
+                const double *_jrz, const double *_je2,
+                double *_iphi);
+
+/* libtupan.h:17-32 */
+void acc_kernel(const unsigned long ni,
+                const double *_im, const double *_irx, const double *_iry,
+                const double *_irz, const double *_ie2,
+                const unsigned long nj,
+                const double *_jm, const double *_jrx, const double *_jry,
+                const double *_jrz, const double *_je2,
+                double *_iax, double *_iay, double *_iaz);
+
+/* libtupan.h:34-58 */
+void acc_jerk_kernel(const unsigned long ni,
+                     const double *_im, const double *_irx, const double *_iry,
+                     const double *_irz, const double *_ie2,
+                     const double *_ivx, const double *_ivy, const double *_ivz,
+                     const unsigned long nj,
+                     const double *_jm, const double *_jrx, const double *_jry,
+                     const double *_jrz, const double *_je2,
+                     const double *_jvx, const double *_jvy, const double *_jvz,
+                     double *_iax, double *_iay, double *_iaz,
+                     double *_ijx, double *_ijy, double *_ijz);
+
+/* libtupan.h:60-96 */
+void snap_crackle_kernel(const unsigned long ni,
+                         const double *_im, const double *_irx,
+                         const double *_iry, const double *_irz,
+                         const double *_ie2, const double *_ivx,
+                         const double *_ivy, const double *_ivz,
+                         const double *_iax, const double *_iay,
+                         const double *_iaz, const double *_ijx,
+                         const double *_ijy, const double *_ijz,
+                         const unsigned long nj,
+                         const double *_jm, const double *_jrx,
+                         const double *_jry, const double *_jrz,
+                         const double *_je2, const double *_jvx,
+                         const double *_jvy, const double *_jvz,
+                         const double *_jax, const double *_jay,
+                         const double *_jaz, const double *_jjx,
+                         const double *_jjy, const double *_jjz,
+                         double *_isx, double *_isy, double *_isz,
+                         double *_icx, double *_icy, double *_icz);
+
+/* libtupan.h:98-119 */
+void tstep_kernel(const unsigned long ni,
+                  const double *_im, const double *_irx, const double *_iry,
+                  const double *_irz, const double *_ie2,
+                  const double *_ivx, const double *_ivy, const double *_ivz,
+                  const unsigned long nj,
+                  const double *_jm, const double *_jrx, const double *_jry,
+                  const double *_jrz, const double *_je2,
+                  const double *_jvx, const double *_jvy, const double *_jvz,
+                  const double eta,
+                  double *_idt_a, double *_idt_b);
+
+/* libtupan.h:203-229 */
+void sakura_kernel(const unsigned long ni,
+                   const double *_im, const double *_irx, const double *_iry,
+                   const double *_irz, const double *_ie2,
+                   const double *_ivx, const double *_ivy, const double *_ivz,
+                   const unsigned long nj,
+                   const double *_jm, const double *_jrx, const double *_jry,
+                   const double *_jrz, const double *_je2,
+                   const double *_jvx, const double *_jvy, const double *_jvz,
+                   const double dt, const long flag,
+                   double *_idrx, double *_idry, double *_idrz,
+                   double *_idvx, double *_idvy, double *_idvz);
+
+/* libtupan.h:231-246.  Two particles (index 0 and 1); outputs may alias the
+ * inputs, as the reference extension layer passes them (extensions.py:
+ * 642-646). */
+void kepler_solver_kernel(const double *_im, const double *_irx,
+                          const double *_iry, const double *_irz,
+                          const double *_ie2, const double *_ivx,
+                          const double *_ivy, const double *_ivz,
+                          const double dt,
+                          double *_ir1x, double *_ir1y, double *_ir1z,
+                          double *_iv1x, double *_iv1y, double *_iv1z);
+#endif /* TUPAN_HIP_NO_FP64_REFERENCE_ABI */
+
+/* ---------------------------------------------------------------------------
+ * Device-resident layer (FP64 arrays).  Same argument order as above, with
+ * size_t counts, a mode and a trailing stream.  Replaces the same reference
+ * functions (libtupan.h lines as above) for callers that keep particles in
+ * HBM.
+ * ------------------------------------------------------------------------- */
+int tupan_phi(int mode, size_t ni,
+              const double *im, const double *irx, const double *iry,
+              const double *irz, const double *ie2,
+              size_t nj,
+              const double *jm, const double *jrx, const double *jry,
+              const double *jrz, const double *je2,
+              double *iphi, void *stream);
+
+int tupan_acc(int mode, size_t ni,
+              const double *im, const double *irx, const double *iry,
+              const double *irz, const double *ie2,
+              size_t nj,
+              const double *jm, const double *jrx, const double *jry,
+              const double *jrz, const double *je2,
+              double *iax, double *iay, double *iaz, void *stream);
+
+int tupan_acc_jerk(int mode, size_t ni,
+                   const double *im, const double *irx, const double *iry,
+                   const double *irz, const double *ie2,
+                   const double *ivx, const double *ivy, const double *ivz,
+                   size_t nj,
+                   const double *jm, const double *jrx, const double *jry,
+                   const double *jrz, const double *je2,
+                   const double *jvx, const double *jvy, const double *jvz,
+                   double *iax, double *iay, double *iaz,
+                   double *ijx, double *ijy, double *ijz, void *stream);
+
+int tupan_snap_crackle(int mode, size_t ni,
+                       const double *const *ifields /* 14: m rx ry rz e2 vx
+                                                      vy vz ax ay az jx jy jz */,
+                       size_t nj, const double *const *jfields /* same 14 */,
+                       double *isx, double *isy, double *isz,
+                       double *icx, double *icy, double *icz, void *stream);
+
+int tupan_tstep(int mode, size_t ni,
+                const double *im, const double *irx, const double *iry,
+                const double *irz, const double *ie2,
+                const double *ivx, const double *ivy, const double *ivz,
+                size_t nj,
+                const double *jm, const double *jrx, const double *jry,
+                const double *jrz, const double *je2,
+                const double *jvx, const double *jvy, const double *jvz,
+                double eta, double *idt_a, double *idt_b, void *stream);
+
+/* Generic form of the five pairwise kernels above: `kernel` is one of
+ * TUPAN_K_*; ifields/jfields hold the reference's input arrays in its order
+ * (Phi, Acc: m rx ry rz e2; AccJerk, Tstep: + vx vy vz; SnapCrackle: + ax ay
+ * az jx jy jz); out holds the outputs in the reference's order; eta is read
+ * by Tstep only. */
+#define TUPAN_K_PHI 0
+#define TUPAN_K_ACC 1
+#define TUPAN_K_ACC_JERK 2
+#define TUPAN_K_SNAP_CRACKLE 3
+#define TUPAN_K_TSTEP 4
+int tupan_pairwise(int kernel, int mode, size_t ni,
+                   const double *const *ifields, size_t nj,
+                   const double *const *jfields, double *const *out,
+                   double eta, void *stream);
+/* FP32 arrays (the reference's float build); mode is ignored. */
+int tupan_pairwise_f32(int kernel, int mode, size_t ni,
+                       const float *const *ifields, size_t nj,
+                       const float *const *jfields, float *const *out,
+                       float eta, void *stream);
+
+/* Sakura pair drift (sakura_kernel.c:5-63, libtupan.h:203-229) on device
+ * arrays.  flag: -1, +1, -2, +2 (any other value leaves every pair
+ * unchanged, as in the reference).  `counters`, if not NULL, is a device
+ * array of 4 unsigned 64-bit integers that receives (added to, not
+ * overwritten) the numbers of Kepler solves, Laguerre iterations, step
+ * subdivisions and leapfrog pair steps; `status` if not NULL is a device int
+ * set to a nonzero value if any pair exhausted the solver's bounded retries.
+ */
+int tupan_sakura(size_t ni,
+                 const double *im, const double *irx, const double *iry,
+                 const double *irz, const double *ie2,
+                 const double *ivx, const double *ivy, const double *ivz,
+                 size_t nj,
+                 const double *jm, const double *jrx, const double *jry,
+                 const double *jrz, const double *je2,
+                 const double *jvx, const double *jvy, const double *jvz,
+                 double dt, long flag,
+                 double *idrx, double *idry, double *idrz,
+                 double *idvx, double *idvy, double *idvz,
+                 unsigned long long *counters, int *status, void *stream);
+
+/* Kepler two-body advance of many independent pairs (the SIA n = 2 leaves,
+ * kepler_solver_kernel.c:5-49) in one launch: pair p uses particles
+ * idx[2p] and idx[2p+1] of the device SoA arrays and advances them in place
+ * by dt.  npairs may be 0.  status as for tupan_sakura. */
+int tupan_kepler_pairs(size_t npairs, const int *idx,
+                       const double *m, double *rx, double *ry, double *rz,
+                       const double *e2, double *vx, double *vy, double *vz,
+                       double dt, int *status, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Host layer of the float build.  libtupan_hip_f32.so implements the
+ * reference's float ABI (phi_kernel, ... with unsigned int / int / float)
+ * on top of these; ifields/jfields/out as for tupan_pairwise.  Sakura and
+ * Kepler of the float build compute in FP64 and round to float.  Return
+ * TUPAN_OK or a negative status.
+ * ------------------------------------------------------------------------- */
+int tupan_host_pairwise_f32(int kernel, size_t ni, const float *const *ifields,
+                            size_t nj, const float *const *jfields,
+                            float *const *out, float eta);
+int tupan_host_sakura_f32(size_t ni, const float *const *ifields, size_t nj,
+                          const float *const *jfields, float dt, int flag,
+                          float *const *out);
+/* in8: m rx ry rz e2 vx vy vz of two particles; out6: rx ry rz vx vy vz */
+int tupan_host_kepler_f32(const float *const *in8, float dt,
+                          float *const *out6);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TUPAN_HIP_H */

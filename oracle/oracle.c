@@ -1,0 +1,716 @@
+/*
+ * CPU ORACLE -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of tupan's pairwise gravity kernels and of its
+ * universal-variable Kepler solver.  It exists to CHECK the HIP kernels of
+ * tupan_amd: only the test suite, the smoke check and the CPU-baseline leg of
+ * the benchmark may load it.  No product code path calls it.
+ *
+ * Every floating-point operation is written in the same order as the
+ * reference C backend (tupan/lib/src/*.c and *_common.h), so that a build with
+ * `-O3 -std=c99 -ffp-contract=off` reproduces the reference outputs bit for
+ * bit.  The pinning of each function against the compiled reference and
+ * against closed forms lives in tests/test_oracle.py.
+ *
+ * Build: REAL is double by default; -DORACLE_SP gives the float build that
+ * matches the reference's `--use_sp` library (cffi_backend.py:35-55).
+ *
+ * Deviations from the reference, all outside the arithmetic:
+ *   - the reference's two unbounded retry loops (universal_kepler_solver.h:
+ *     486-499 and 580-606) are capped at ORACLE_MAX_DOUBLINGS halvings of the
+ *     step; reaching the cap increments a failure counter instead of looping
+ *     forever.  No golden case comes near the cap.
+ *   - optional counters (solves, Laguerre iterations, substeps, leapfrogs)
+ *     instrument the Sakura work for the benchmark's work accounting.
+ */
+#include <math.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef ORACLE_SP
+typedef float real;
+typedef int sint;
+#define ORACLE_TOL ((real)(1.52587890625e-5))          /* 2^-16 */
+#define R_SQRT sqrtf
+#define R_FABS fabsf
+#define R_FMAX fmaxf
+#define R_COS cosf
+#define R_SIN sinf
+#define R_COSH coshf
+#define R_SINH sinhf
+#define R_LOG logf
+#define SYM(name) oracle_##name##_f32
+#else
+typedef double real;
+typedef long sint;
+#define ORACLE_TOL ((real)(2.2737367544323205948e-13))  /* 2^-42 */
+#define R_SQRT sqrt
+#define R_FABS fabs
+#define R_FMAX fmax
+#define R_COS cos
+#define R_SIN sin
+#define R_COSH cosh
+#define R_SINH sinh
+#define R_LOG log
+#define SYM(name) oracle_##name
+#endif
+
+#define ORACLE_MAXITER 64
+#define ORACLE_MAX_DOUBLINGS 40
+#define ORACLE_PI ((real)(3.141592653589793))
+
+static int sgn(real x) { return (x > 0) - (x < 0); }
+
+/* ---------------------------------------------------------------------------
+ * Work counters (serial use only).
+ * ------------------------------------------------------------------------- */
+#ifdef _OPENMP
+#define ORACLE_TLS __thread   /* per-thread counters in the OpenMP build */
+#else
+#define ORACLE_TLS
+#endif
+static ORACLE_TLS uint64_t cnt_solves, cnt_iters, cnt_retries, cnt_leapfrog,
+    cnt_fail;
+
+void SYM(counters_reset)(void)
+{
+    cnt_solves = cnt_iters = cnt_retries = cnt_leapfrog = cnt_fail = 0;
+}
+
+void SYM(counters_get)(uint64_t *out5)
+{
+    out5[0] = cnt_solves;
+    out5[1] = cnt_iters;
+    out5[2] = cnt_retries;
+    out5[3] = cnt_leapfrog;
+    out5[4] = cnt_fail;
+}
+
+/* ---------------------------------------------------------------------------
+ * Plummer-softened inverse powers, smoothing.h:10-186.  The pair mask is
+ * r2 > 0 (not i != j); a masked pair gets 0 for every inverse power.
+ * ------------------------------------------------------------------------- */
+static inline real soft_inv_r2(real r2, real e2, int mask)
+{
+    real q = 1 / (r2 + e2);
+    return mask ? q : (real)0;
+}
+
+/* ---------------------------------------------------------------------------
+ * Phi: phi_kernel.c:5-34, phi_kernel_common.h:7-32.
+ * ------------------------------------------------------------------------- */
+void SYM(phi)(size_t ni, const real *im, const real *irx, const real *iry,
+              const real *irz, const real *ie2,
+              size_t nj, const real *jm, const real *jrx, const real *jry,
+              const real *jrz, const real *je2, real *iphi)
+{
+    (void)im;
+    for (size_t i = 0; i < ni; ++i) {
+        real xi = irx[i], yi = iry[i], zi = irz[i], e2i = ie2[i];
+        real acc = 0;
+        for (size_t j = 0; j < nj; ++j) {
+            real dx = xi - jrx[j];
+            real dy = yi - jry[j];
+            real dz = zi - jrz[j];
+            real e2 = e2i + je2[j];
+            real r2 = dx * dx + dy * dy + dz * dz;
+            real q = soft_inv_r2(r2, e2, r2 > 0);
+            real inv_r = R_SQRT(q);
+            acc -= jm[j] * inv_r;
+        }
+        iphi[i] = acc;
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * Acc: acc_kernel.c:5-40, acc_kernel_common.h:7-38.
+ * ------------------------------------------------------------------------- */
+void SYM(acc)(size_t ni, const real *im, const real *irx, const real *iry,
+              const real *irz, const real *ie2,
+              size_t nj, const real *jm, const real *jrx, const real *jry,
+              const real *jrz, const real *je2,
+              real *iax, real *iay, real *iaz)
+{
+    (void)im;
+    for (size_t i = 0; i < ni; ++i) {
+        real xi = irx[i], yi = iry[i], zi = irz[i], e2i = ie2[i];
+        real ax = 0, ay = 0, az = 0;
+        for (size_t j = 0; j < nj; ++j) {
+            real dx = xi - jrx[j];
+            real dy = yi - jry[j];
+            real dz = zi - jrz[j];
+            real e2 = e2i + je2[j];
+            real r2 = dx * dx + dy * dy + dz * dz;
+            real q = soft_inv_r2(r2, e2, r2 > 0);
+            real w = q * R_SQRT(q);
+            w *= jm[j];
+            ax -= w * dx;
+            ay -= w * dy;
+            az -= w * dz;
+        }
+        iax[i] = ax;
+        iay[i] = ay;
+        iaz[i] = az;
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * AccJerk: acc_jerk_kernel.c:5-60, acc_jerk_kernel_common.h:7-56.
+ * ------------------------------------------------------------------------- */
+void SYM(acc_jerk)(size_t ni, const real *im, const real *irx,
+                   const real *iry, const real *irz, const real *ie2,
+                   const real *ivx, const real *ivy, const real *ivz,
+                   size_t nj, const real *jm, const real *jrx,
+                   const real *jry, const real *jrz, const real *je2,
+                   const real *jvx, const real *jvy, const real *jvz,
+                   real *iax, real *iay, real *iaz,
+                   real *ijx, real *ijy, real *ijz)
+{
+    (void)im;
+    for (size_t i = 0; i < ni; ++i) {
+        real xi = irx[i], yi = iry[i], zi = irz[i], e2i = ie2[i];
+        real uxi = ivx[i], uyi = ivy[i], uzi = ivz[i];
+        real ax = 0, ay = 0, az = 0, jx = 0, jy = 0, jz = 0;
+        for (size_t j = 0; j < nj; ++j) {
+            real dx = xi - jrx[j];
+            real dy = yi - jry[j];
+            real dz = zi - jrz[j];
+            real e2 = e2i + je2[j];
+            real ux = uxi - jvx[j];
+            real uy = uyi - jvy[j];
+            real uz = uzi - jvz[j];
+            real r2 = dx * dx + dy * dy + dz * dz;
+            real rv = dx * ux + dy * uy + dz * uz;
+            real q2 = soft_inv_r2(r2, e2, r2 > 0);
+            real q3 = q2 * R_SQRT(q2);
+            real alpha = 3 * rv * q2;
+            ux -= alpha * dx;
+            uy -= alpha * dy;
+            uz -= alpha * dz;
+            q3 *= jm[j];
+            ax -= q3 * dx;
+            ay -= q3 * dy;
+            az -= q3 * dz;
+            jx -= q3 * ux;
+            jy -= q3 * uy;
+            jz -= q3 * uz;
+        }
+        iax[i] = ax;
+        iay[i] = ay;
+        iaz[i] = az;
+        ijx[i] = jx;
+        ijy[i] = jy;
+        ijz[i] = jz;
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * SnapCrackle: snap_crackle_kernel.c:5-82, snap_crackle_kernel_common.h:7-95.
+ * ------------------------------------------------------------------------- */
+void SYM(snap_crackle)(size_t ni, const real *im, const real *irx,
+                       const real *iry, const real *irz, const real *ie2,
+                       const real *ivx, const real *ivy, const real *ivz,
+                       const real *iax, const real *iay, const real *iaz,
+                       const real *ijx, const real *ijy, const real *ijz,
+                       size_t nj, const real *jm, const real *jrx,
+                       const real *jry, const real *jrz, const real *je2,
+                       const real *jvx, const real *jvy, const real *jvz,
+                       const real *jax, const real *jay, const real *jaz,
+                       const real *jjx, const real *jjy, const real *jjz,
+                       real *isx, real *isy, real *isz,
+                       real *icx, real *icy, real *icz)
+{
+    (void)im;
+    for (size_t i = 0; i < ni; ++i) {
+        real sx = 0, sy = 0, sz = 0, cx = 0, cy = 0, cz = 0;
+        for (size_t j = 0; j < nj; ++j) {
+            real dx = irx[i] - jrx[j];
+            real dy = iry[i] - jry[j];
+            real dz = irz[i] - jrz[j];
+            real e2 = ie2[i] + je2[j];
+            real ux = ivx[i] - jvx[j];
+            real uy = ivy[i] - jvy[j];
+            real uz = ivz[i] - jvz[j];
+            real Ax = iax[i] - jax[j];
+            real Ay = iay[i] - jay[j];
+            real Az = iaz[i] - jaz[j];
+            real Jx = ijx[i] - jjx[j];
+            real Jy = ijy[i] - jjy[j];
+            real Jz = ijz[i] - jjz[j];
+            real r2 = dx * dx + dy * dy + dz * dz;
+            real rv = dx * ux + dy * uy + dz * uz;
+            real v2 = ux * ux + uy * uy + uz * uz;
+            real rJ = dx * Jx + dy * Jy + dz * Jz;
+            real rA = dx * Ax + dy * Ay + dz * Az;
+            real vA = ux * Ax + uy * Ay + uz * Az;
+            real q2 = soft_inv_r2(r2, e2, r2 > 0);
+            real q3 = q2 * R_SQRT(q2);
+
+            real a1 = rv * q2;
+            real a1sq = a1 * a1;
+            real b1 = 3 * ((v2 + rA) * q2 + a1sq);
+            real c1 = (3 * vA + rJ) * q2 + a1 * (b1 - 4 * a1sq);
+            a1 *= 3;
+            c1 *= 3;
+            ux -= a1 * dx;
+            uy -= a1 * dy;
+            uz -= a1 * dz;
+            real two_a = 2 * a1;
+            Ax -= two_a * ux + b1 * dx;
+            Ay -= two_a * uy + b1 * dy;
+            Az -= two_a * uz + b1 * dz;
+            a1 *= 3;
+            b1 *= 3;
+            Jx -= a1 * Ax + b1 * ux + c1 * dx;
+            Jy -= a1 * Ay + b1 * uy + c1 * dy;
+            Jz -= a1 * Az + b1 * uz + c1 * dz;
+            q3 *= jm[j];
+            sx -= q3 * Ax;
+            sy -= q3 * Ay;
+            sz -= q3 * Az;
+            cx -= q3 * Jx;
+            cy -= q3 * Jy;
+            cz -= q3 * Jz;
+        }
+        isx[i] = sx;
+        isy[i] = sy;
+        isz[i] = sz;
+        icx[i] = cx;
+        icy[i] = cy;
+        icz[i] = cz;
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * Tstep: tstep_kernel.c:5-49, tstep_kernel_common.h:7-60.  A masked pair
+ * produces NaN inside (0 * eta/sqrt(0)) and is then replaced by 0; the max
+ * starts at 0.
+ * ------------------------------------------------------------------------- */
+void SYM(tstep)(size_t ni, const real *im, const real *irx, const real *iry,
+                const real *irz, const real *ie2,
+                const real *ivx, const real *ivy, const real *ivz,
+                size_t nj, const real *jm, const real *jrx, const real *jry,
+                const real *jrz, const real *je2,
+                const real *jvx, const real *jvy, const real *jvz,
+                real eta, real *idt_a, real *idt_b)
+{
+    for (size_t i = 0; i < ni; ++i) {
+        real sum_w2 = 0, max_w2 = 0;
+        for (size_t j = 0; j < nj; ++j) {
+            real dx = irx[i] - jrx[j];
+            real dy = iry[i] - jry[j];
+            real dz = irz[i] - jrz[j];
+            real e2 = ie2[i] + je2[j];
+            real ux = ivx[i] - jvx[j];
+            real uy = ivy[i] - jvy[j];
+            real uz = ivz[i] - jvz[j];
+            real mtot = im[i] + jm[j];
+            real r2 = dx * dx + dy * dy + dz * dz;
+            real rv = dx * ux + dy * uy + dz * uz;
+            real v2 = ux * ux + uy * uy + uz * uz;
+            int mask = r2 > 0;
+            real q2 = soft_inv_r2(r2, e2, mask);
+            real q1 = R_SQRT(q2);
+            real a = (real)2;
+            real b = (1 + a / 2) * q2;
+            real w2 = q2;
+            real g = q2;
+            real phi = mtot * q1;
+            w2 *= (v2 + a * phi);
+            g *= (w2 + b * phi);
+            g *= (eta / R_SQRT(w2));
+            w2 -= g * rv;
+            w2 = mask ? w2 : (real)0;
+            sum_w2 += w2;
+            max_w2 = R_FMAX(w2, max_w2);
+        }
+        idt_a[i] = eta / R_SQRT(1 + sum_w2);
+        idt_b[i] = eta / R_SQRT(1 + max_w2);
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * Universal-variable Kepler solver: universal_kepler_solver.h:17-614.
+ * ------------------------------------------------------------------------- */
+static inline real stumpff_c0(real z)
+{
+    if (z < 0) return R_COS(R_SQRT(-z));
+    if (z > 0) return R_COSH(R_SQRT(z));
+    return (real)1;
+}
+
+static inline real stumpff_c1(real z)
+{
+    if (z < 0) { real t = R_SQRT(-z); return R_SIN(t) / t; }
+    if (z > 0) { real t = R_SQRT(z); return R_SINH(t) / t; }
+    return (real)1;
+}
+
+static inline real stumpff_c2(real z)
+{
+    if (z < 0) { real t = R_SQRT(-z); return (R_COS(t) - 1) / z; }
+    if (z > 0) { real t = R_SQRT(z); return (R_COSH(t) - 1) / z; }
+    return 1 / ((real)2);
+}
+
+static inline real stumpff_c3(real z)
+{
+    if (z < 0) { real t = R_SQRT(-z); return (R_SIN(t) / t - 1) / z; }
+    if (z > 0) { real t = R_SQRT(z); return (R_SINH(t) / t - 1) / z; }
+    return 1 / ((real)6);
+}
+
+/* S_k(s, alpha) = s^k c_k(alpha s^2), :82-120. */
+static inline real uS0(real s, real al) { real s2 = s * s; return stumpff_c0(al * s2); }
+static inline real uS1(real s, real al) { real s2 = s * s; return s * stumpff_c1(al * s2); }
+static inline real uS2(real s, real al) { real s2 = s * s; return s2 * stumpff_c2(al * s2); }
+static inline real uS3(real s, real al)
+{
+    real s2 = s * s;
+    real s3 = s * s2;
+    return s3 * stumpff_c3(al * s2);
+}
+
+/* Kepler's equation in s and its first two derivatives, :166-222.
+ * k[0]=dt, k[1]=r0, k[2]=r0.v0, k[3]=m, k[4]=alpha. */
+static inline real kep_f(real s, const real *k)
+{
+    return (k[1] * s + k[2] * uS2(s, k[4]) + (k[3] + k[4] * k[1]) * uS3(s, k[4])) - k[0];
+}
+
+static inline real kep_fp(real s, const real *k)
+{
+    return k[1] + k[2] * uS1(s, k[4]) + (k[3] + k[4] * k[1]) * uS2(s, k[4]);
+}
+
+static inline real kep_fpp(real s, const real *k)
+{
+    return k[2] * uS0(s, k[4]) + (k[3] + k[4] * k[1]) * uS1(s, k[4]);
+}
+
+/* Laguerre iteration of order 5, :225-254.  Returns 0, or -1 (h == 0),
+ * -2 (more than MAXITER steps), -3 (sign of s changed). */
+static sint laguerre5(real x0, real *x, const real *k)
+{
+    sint it = 0;
+    real delta;
+    *x = x0;
+    do {
+        real fv = kep_f(*x, k);
+        real dfv = kep_fp(*x, k);
+        real ddfv = kep_fpp(*x, k);
+        real a = dfv;
+        real a2 = a * a;
+        real b = a2 - fv * ddfv;
+        real g = 5 * fv;
+        real h = a + sgn(a) * R_SQRT(R_FABS(4 * (5 * b - a2)));
+        cnt_iters++;
+        if (h == 0) return -1;
+        delta = -g / h;
+        *x += delta;
+        it += 1;
+        if (it > ORACLE_MAXITER) return -2;
+    } while (R_FABS(delta) > ORACLE_TOL);
+    if (sgn(*x) != sgn(x0)) return -3;
+    return 0;
+}
+
+/* Lagrange f, g and derivatives applied to the state, :312-357. */
+static void lagrange_update(real dt, real s, real r0, real m, real e2,
+                            real alpha, real *p, real *v)
+{
+    real x0 = p[0], y0 = p[1], z0 = p[2];
+    real u0 = v[0], w0 = v[1], t0 = v[2];
+    real lf = 1 - m * uS2(s, alpha) / r0;
+    real lg = dt - m * uS3(s, alpha);
+    real x1 = x0 * lf + u0 * lg;
+    real y1 = y0 * lf + w0 * lg;
+    real z1 = z0 * lf + t0 * lg;
+    real r1sq = x1 * x1 + y1 * y1 + z1 * z1;
+    real r1 = R_SQRT(r1sq + e2);
+    real ldf = -m * uS1(s, alpha) / (r0 * r1);
+    real ldg = (1 + lg * ldf) / lf;
+    p[0] = x1;
+    p[1] = y1;
+    p[2] = z1;
+    v[0] = x0 * ldf + u0 * ldg;
+    v[1] = y0 * ldf + w0 * ldg;
+    v[2] = z0 * ldf + t0 * ldg;
+}
+
+/* One solve over dt0, :360-463.  p, v in/out. */
+static sint kepler_once(real dt0, real m, real e2, real *p, real *v)
+{
+    real r2 = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+    if (!(r2 > 0)) return 0;
+    cnt_solves++;
+    r2 += e2;
+    real r = R_SQRT(r2);
+    real v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    real rv = p[0] * v[0] + p[1] * v[1] + p[2] * v[2];
+    real beta = 2 - (e2 / r2);
+    real alpha = v2 - beta * m / r;
+
+    real dt = dt0;
+    if (alpha < 0) {
+        real a = m / R_FABS(alpha);
+        real T = 2 * ORACLE_PI * a * R_SQRT(a / m);
+        real ratio = dt0 / T;
+        dt = (ratio - (sint)(ratio)) * T;
+    }
+
+    real s0 = dt / r;
+    if (alpha > 0) {
+        real sal = R_SQRT(alpha);
+        real ss = R_FABS(2 * alpha * dt / (rv + (m + alpha * r) / sal));
+        if (ss > 1) s0 = sgn(dt) * R_LOG(ss) / sal;
+    }
+
+    real k[5] = {dt, r, rv, m, alpha};
+    real s;
+    sint err = laguerre5(s0, &s, k);
+    if (err == 0) lagrange_update(dt, s, r, m, e2, alpha, p, v);
+    return err;
+}
+
+/* Retry with n = 2, 4, ... equal substeps on any solver error, :466-507. */
+static void kepler_subdivided(real dt, real m, real e2, const real *p0,
+                              const real *v0, real *p, real *v)
+{
+    for (int lvl = 0; lvl <= ORACLE_MAX_DOUBLINGS; ++lvl) {
+        long n = 1L << lvl;
+        int ok = 1;
+        p[0] = p0[0]; p[1] = p0[1]; p[2] = p0[2];
+        v[0] = v0[0]; v[1] = v0[1]; v[2] = v0[2];
+        for (long i = 0; i < n; ++i) {
+            if (kepler_once(dt / n, m, e2, p, v) != 0) { ok = 0; break; }
+        }
+        if (ok) return;
+        cnt_retries++;
+    }
+    cnt_fail++;
+}
+
+/* Softened-energy check with step doubling, :510-614. */
+static void kepler_solve(real dt, real m, real e2, const real *p0,
+                         const real *v0, real *p1, real *v1)
+{
+    real p[3], v[3];
+    kepler_subdivided(dt, m, e2, p0, v0, p, v);
+    if (e2 == 0) goto done;
+    {
+        real r2 = p0[0] * p0[0] + p0[1] * p0[1] + p0[2] * p0[2];
+        if (!(r2 > 0)) {
+            p[0] = p0[0]; p[1] = p0[1]; p[2] = p0[2];
+            v[0] = v0[0]; v[1] = v0[1]; v[2] = v0[2];
+            goto done;
+        }
+        r2 += e2;
+        real r = R_SQRT(r2);
+        real v2 = v0[0] * v0[0] + v0[1] * v0[1] + v0[2] * v0[2];
+        real u0 = 2 * m / r;
+        real en0 = v2 - u0;
+
+        r2 = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+        r2 += e2;
+        r = R_SQRT(r2);
+        v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+        real u1 = 2 * m / r;
+        real en1 = v2 - u1;
+
+        real tol = 64 * ORACLE_TOL;
+        real err = R_FABS(en1 - en0);
+        if (2 * err < tol * (u1 + u0)) goto done;
+
+        long n = 1;
+        int lvl;
+        for (lvl = 1; lvl <= ORACLE_MAX_DOUBLINGS; ++lvl) {
+            n *= 2;
+            int ok = 1;
+            p[0] = p0[0]; p[1] = p0[1]; p[2] = p0[2];
+            v[0] = v0[0]; v[1] = v0[1]; v[2] = v0[2];
+            for (long i = 0; i < n; ++i) {
+                real q[3] = {p[0], p[1], p[2]}, w[3] = {v[0], v[1], v[2]};
+                kepler_subdivided(dt / n, m, e2, q, w, p, v);
+                r2 = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+                r2 += e2;
+                r = R_SQRT(r2);
+                v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+                u1 = 2 * m / r;
+                en1 = v2 - u1;
+                err = R_FABS(en1 - en0);
+                if (2 * err > tol * (u1 + u0)) { ok = 0; break; }
+            }
+            if (ok) break;
+            cnt_retries++;
+        }
+        if (lvl > ORACLE_MAX_DOUBLINGS) cnt_fail++;
+    }
+done:
+    p1[0] = p[0]; p1[1] = p[1]; p1[2] = p[2];
+    v1[0] = v[0]; v1[1] = v[1]; v1[2] = v[2];
+}
+
+/* ---------------------------------------------------------------------------
+ * Sakura pair drift: sakura_kernel_common.h:45-243, sakura_kernel.c:5-63.
+ * ------------------------------------------------------------------------- */
+static void lf_kdk(real dt, real m, real e2, const real *p0, const real *v0,
+                   real *p1, real *v1)
+{
+    /* leapfrog DKD with one softened kick, :45-91 */
+    real h = dt / 2;
+    real x = p0[0], y = p0[1], z = p0[2];
+    real u = v0[0], w = v0[1], t = v0[2];
+    x += u * h;
+    y += w * h;
+    z += t * h;
+    real r2 = x * x + y * y + z * z;
+    real q2 = soft_inv_r2(r2, e2, r2 > 0);
+    real q3 = q2 * R_SQRT(q2);
+    real mq = m * q3;
+    u += (-mq * x) * dt;
+    w += (-mq * y) * dt;
+    t += (-mq * z) * dt;
+    x += u * h;
+    y += w * h;
+    z += t * h;
+    p1[0] = x; p1[1] = y; p1[2] = z;
+    v1[0] = u; v1[1] = w; v1[2] = t;
+    cnt_leapfrog++;
+}
+
+static void twobody(real dt, real m, real e2, real *p, real *v)
+{
+    /* far pairs (r > 64 m / v^2) take one leapfrog step, :94-123 */
+    real r2 = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+    real v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    real R = 64 * (m / v2);
+    real p0[3] = {p[0], p[1], p[2]}, v0[3] = {v[0], v[1], v[2]};
+    if (r2 > R * R)
+        lf_kdk(dt, m, e2, p0, v0, p, v);
+    else
+        kepler_solve(dt, m, e2, p0, v0, p, v);
+}
+
+static void drift_twobody(real dt, sint flag, real m, real e2, real *p, real *v)
+{
+    /* :126-191 */
+    if (flag == -1) {
+        p[0] -= v[0] * dt; p[1] -= v[1] * dt; p[2] -= v[2] * dt;
+        twobody(dt, m, e2, p, v);
+    }
+    if (flag == 1) {
+        twobody(dt, m, e2, p, v);
+        p[0] -= v[0] * dt; p[1] -= v[1] * dt; p[2] -= v[2] * dt;
+    }
+    if (flag == -2) {
+        p[0] -= v[0] * dt / 2; p[1] -= v[1] * dt / 2; p[2] -= v[2] * dt / 2;
+        twobody(dt, m, e2, p, v);
+        p[0] -= v[0] * dt / 2; p[1] -= v[1] * dt / 2; p[2] -= v[2] * dt / 2;
+    }
+    if (flag == 2) {
+        twobody(dt / 2, m, e2, p, v);
+        p[0] -= v[0] * dt; p[1] -= v[1] * dt; p[2] -= v[2] * dt;
+        twobody(dt / 2, m, e2, p, v);
+    }
+}
+
+void SYM(sakura)(size_t ni, const real *im, const real *irx, const real *iry,
+                 const real *irz, const real *ie2,
+                 const real *ivx, const real *ivy, const real *ivz,
+                 size_t nj, const real *jm, const real *jrx, const real *jry,
+                 const real *jrz, const real *je2,
+                 const real *jvx, const real *jvy, const real *jvz,
+                 real dt, sint flag,
+                 real *idrx, real *idry, real *idrz,
+                 real *idvx, real *idvy, real *idvz)
+{
+    for (size_t i = 0; i < ni; ++i) {
+        real sx = 0, sy = 0, sz = 0, tx = 0, ty = 0, tz = 0;
+        for (size_t j = 0; j < nj; ++j) {
+            real p0[3] = {irx[i] - jrx[j], iry[i] - jry[j], irz[i] - jrz[j]};
+            real e2 = ie2[i] + je2[j];
+            real v0[3] = {ivx[i] - jvx[j], ivy[i] - jvy[j], ivz[i] - jvz[j]};
+            real m = im[i] + jm[j];
+            real p[3] = {p0[0], p0[1], p0[2]}, v[3] = {v0[0], v0[1], v0[2]};
+            drift_twobody(dt, flag, m, e2, p, v);
+            real mu = jm[j] / m;
+            sx += mu * (p[0] - p0[0]);
+            sy += mu * (p[1] - p0[1]);
+            sz += mu * (p[2] - p0[2]);
+            tx += mu * (v[0] - v0[0]);
+            ty += mu * (v[1] - v0[1]);
+            tz += mu * (v[2] - v0[2]);
+        }
+        idrx[i] = sx;
+        idry[i] = sy;
+        idrz[i] = sz;
+        idvx[i] = tx;
+        idvy[i] = ty;
+        idvz[i] = tz;
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * Exact two-body advance in the centre-of-mass frame (SIA n=2 leaves):
+ * kepler_solver_kernel.c:5-49, kepler_solver_kernel_common.h:8-81.
+ * Reads particles 0 and 1; outputs may alias the inputs.
+ * ------------------------------------------------------------------------- */
+void SYM(kepler)(const real *m, const real *rx, const real *ry,
+                 const real *rz, const real *e2, const real *vx,
+                 const real *vy, const real *vz, real dt,
+                 real *orx, real *ory, real *orz,
+                 real *ovx, real *ovy, real *ovz)
+{
+    real p0[3] = {rx[0] - rx[1], ry[0] - ry[1], rz[0] - rz[1]};
+    real ee = e2[0] + e2[1];
+    real v0[3] = {vx[0] - vx[1], vy[0] - vy[1], vz[0] - vz[1]};
+    real mt = m[0] + m[1];
+    real mu0 = m[0] / mt;
+    real mu1 = m[1] / mt;
+    real cx = mu0 * rx[0] + mu1 * rx[1];
+    real cy = mu0 * ry[0] + mu1 * ry[1];
+    real cz = mu0 * rz[0] + mu1 * rz[1];
+    real wx = mu0 * vx[0] + mu1 * vx[1];
+    real wy = mu0 * vy[0] + mu1 * vy[1];
+    real wz = mu0 * vz[0] + mu1 * vz[1];
+    cx += wx * dt;
+    cy += wy * dt;
+    cz += wz * dt;
+    real p[3], v[3];
+    kepler_solve(dt, mt, ee, p0, v0, p, v);
+    orx[0] = cx + mu1 * p[0];
+    ory[0] = cy + mu1 * p[1];
+    orz[0] = cz + mu1 * p[2];
+    ovx[0] = wx + mu1 * v[0];
+    ovy[0] = wy + mu1 * v[1];
+    ovz[0] = wz + mu1 * v[2];
+    orx[1] = cx - mu0 * p[0];
+    ory[1] = cy - mu0 * p[1];
+    orz[1] = cz - mu0 * p[2];
+    ovx[1] = wx - mu0 * v[0];
+    ovy[1] = wy - mu0 * v[1];
+    ovz[1] = wz - mu0 * v[2];
+}
+
+/* ---------------------------------------------------------------------------
+ * Single relative-orbit solve (test hook): advances (p, v) by dt with the
+ * full solver used by Sakura and Kepler, returning the state.
+ * ------------------------------------------------------------------------- */
+void SYM(kepler_rel)(real dt, real m, real e2, const real *p0, const real *v0,
+                     real *p1, real *v1)
+{
+    kepler_solve(dt, m, e2, p0, v0, p1, v1);
+}
+
+/* Laguerre solve from a supplied initial guess (test hook for the learned
+ * initial guess): returns the error code and the iteration count. */
+sint SYM(laguerre_from)(real s0, real dt, real r, real rv, real m, real alpha,
+                        real *s_out, sint *iters)
+{
+    uint64_t before = cnt_iters;
+    real k[5] = {dt, r, rv, m, alpha};
+    sint err = laguerre5(s0, s_out, k);
+    *iters = (sint)(cnt_iters - before);
+    return err;
+}

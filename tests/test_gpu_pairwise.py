@@ -1,0 +1,262 @@
+"""GPU parity of the pairwise kernels against the reference golden vectors
+and the CPU oracle.
+
+Bars (per particle, vector outputs normalised by the particle's vector norm,
+scalar outputs relative):
+  FP64 (mode fp64):     <= 1e-13
+  FP64 (mode ieee64):   bit-exact with the reference C backend when the
+                        launch uses one j split (all golden cases), since the
+                        operation order, IEEE division and square root are
+                        the reference's
+  split-FP32:           Acc, AccJerk <= 2e-6 on the golden sets (close
+                        random pairs; the Plummer error budget is in
+                        test_gpu_precision.py)
+  FP32 (float build):   <= 2e-5 against the reference's float build
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import vec_rel_err
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+KERNELS = {
+    # name: (extension class, scalars, output names, vector groups)
+    "phi": ("Phi", (), ("phi",), [(0,)]),
+    "acc": ("Acc", (), ("ax", "ay", "az"), [(0, 1, 2)]),
+    "acc_jerk": ("AccJerk", (), ("ax", "ay", "az", "jx", "jy", "jz"),
+                 [(0, 1, 2), (3, 4, 5)]),
+    "snap_crackle": ("SnapCrackle", (), ("sx", "sy", "sz", "cx", "cy", "cz"),
+                     [(0, 1, 2), (3, 4, 5)]),
+    "tstep": ("Tstep", (1.0 / 64,), ("tstep", "tstepij"), [(0,), (1,)]),
+}
+FIELDS = ("mass", "rx", "ry", "rz", "eps2", "vx", "vy", "vz",
+          "ax", "ay", "az", "jx", "jy", "jz")
+PREFIXES = (1, 2, 3, 7, 16, 33, 61)  # as written by golden/make_golden.py
+
+
+def device_ps(d, dtype=torch.float64):
+    from tupan_amd.particles import ParticleSystem
+    from tupan_amd.lib.utils import ctype  # noqa: F401
+    ps = ParticleSystem.from_numpy({k: v for k, v in d.items()},
+                                   device="cuda:0")
+    if dtype == torch.float32:
+        for k, v in list(ps.arrays().items()):
+            if v.dtype == torch.float64:
+                ps.__dict__[k] = v.float()
+    return ps
+
+
+def split_set(g, tag, which, upto=None):
+    d = {}
+    for k in FIELDS:
+        v = g["%s_%s_%s" % (tag, which, k)]
+        d[k] = v if upto is None else v[:upto]
+    return d
+
+
+def run_ext(name, prec, mode, ips, jps, scalars):
+    from tupan_amd.lib import extensions
+    cls = getattr(extensions, KERNELS[name][0])
+    ext = cls("HIP", prec, mode=mode)
+    res = ext.calc(ips, jps, *scalars)
+    torch.cuda.synchronize()
+    return [r.detach().cpu().numpy() for r in res]
+
+
+def assert_close(name, got, g, key, bar):
+    outs = KERNELS[name][2]
+    ref = [g["%s_%s" % (key, o)] for o in outs]
+    for grp in KERNELS[name][3]:
+        e = vec_rel_err([got[k] for k in grp], [ref[k] for k in grp])
+        assert e <= bar, "%s %s: rel err %.3e > %.1e" % (name, key, e, bar)
+
+
+@pytest.mark.parametrize("soft", [0, 1])
+@pytest.mark.parametrize("name", sorted(KERNELS))
+def test_fp64_golden(golden, name, soft):
+    g = golden("pairwise_f64.npz")
+    tag = "s%d" % soft
+    A = device_ps(split_set(g, tag, "A"))
+    B = device_ps(split_set(g, tag, "B"))
+    sc = KERNELS[name][1]
+    for lab, ii, jj in (("AB", A, B), ("BA", B, A), ("AA", A, A)):
+        got = run_ext(name, "float64", "fp64", ii, jj, sc)
+        assert_close(name, got, g, "%s_%s_%s" % (tag, name, lab), 1e-13)
+
+
+@pytest.mark.parametrize("soft", [0, 1])
+@pytest.mark.parametrize("name", sorted(KERNELS))
+def test_ieee64_bitexact(golden, name, soft):
+    g = golden("pairwise_f64.npz")
+    tag = "s%d" % soft
+    A = device_ps(split_set(g, tag, "A"))
+    B = device_ps(split_set(g, tag, "B"))
+    sc = KERNELS[name][1]
+    for lab, ii, jj in (("AB", A, B), ("BA", B, A), ("AA", A, A)):
+        got = run_ext(name, "float64", "ieee64", ii, jj, sc)
+        for o, r in zip(KERNELS[name][2], got):
+            ref = g["%s_%s_%s_%s" % (tag, name, lab, o)]
+            assert np.array_equal(r, ref), (name, lab, o,
+                                            np.max(np.abs(r - ref)))
+
+
+@pytest.mark.parametrize("name", sorted(KERNELS))
+def test_prefix_sweep(golden, name):
+    """A (97) against every prefix B[:k], as the reference's TestCase1
+    sweeps j-prefixes (tupan/tests/test_extensions.py:56-83)."""
+    g = golden("pairwise_f64.npz")
+    sc = KERNELS[name][1]
+    for soft in (0, 1):
+        tag = "s%d" % soft
+        A = device_ps(split_set(g, tag, "A"))
+        for k in PREFIXES:
+            B = device_ps(split_set(g, tag, "B", k))
+            got = run_ext(name, "float64", "fp64", A, B, sc)
+            assert_close(name, got, g, "%s_%s_pre%d" % (tag, name, k), 1e-13)
+
+
+@pytest.mark.parametrize("name", ["acc", "acc_jerk"])
+def test_split32_golden(golden, name):
+    g = golden("pairwise_f64.npz")
+    for soft in (0, 1):
+        tag = "s%d" % soft
+        A = device_ps(split_set(g, tag, "A"))
+        B = device_ps(split_set(g, tag, "B"))
+        for lab, ii, jj in (("AB", A, B), ("BA", B, A), ("AA", A, A)):
+            got = run_ext(name, "float64", "split32", ii, jj, ())
+            assert_close(name, got, g, "%s_%s_%s" % (tag, name, lab), 2e-6)
+
+
+@pytest.mark.parametrize("name", sorted(KERNELS))
+def test_fp32_golden(golden, name):
+    g = golden("pairwise_f32.npz")
+    sc = KERNELS[name][1]
+    for soft in (0, 1):
+        tag = "s%d" % soft
+        A = device_ps(split_set(g, tag, "A"), torch.float32)
+        B = device_ps(split_set(g, tag, "B"), torch.float32)
+        for lab, ii, jj in (("AB", A, B), ("BA", B, A), ("AA", A, A)):
+            got = run_ext(name, "float32", "fp64", ii, jj, sc)
+            assert_close(name, got, g, "%s_%s_%s" % (tag, name, lab), 2e-5)
+
+
+def test_host_pointer_abi(golden):
+    """The reference-named host-pointer entry points (libtupan.h ABI) on
+    numpy arrays, called through ctypes as cffi would call the reference."""
+    from tupan_amd import _native
+    lib = ctypes.CDLL(_native.LIB_PATH)
+    g = golden("pairwise_f64.npz")
+    A = split_set(g, "s1", "A")
+    B = split_set(g, "s1", "B")
+    P = ctypes.c_void_p
+
+    def p(a):
+        return np.ascontiguousarray(a).ctypes.data_as(P)
+    ni, nj = len(A["mass"]), len(B["mass"])
+    out = [np.zeros(ni) for _ in range(6)]
+    lib.acc_jerk_kernel.restype = None
+    lib.acc_jerk_kernel(ctypes.c_ulong(ni), *[p(A[k]) for k in FIELDS[:8]],
+                        ctypes.c_ulong(nj), *[p(B[k]) for k in FIELDS[:8]],
+                        *[p(o) for o in out])
+    assert_close("acc_jerk", out, g, "s1_acc_jerk_AB", 1e-13)
+    out2 = [np.zeros(ni) for _ in range(2)]
+    lib.tstep_kernel.restype = None
+    lib.tstep_kernel(ctypes.c_ulong(ni), *[p(A[k]) for k in FIELDS[:8]],
+                     ctypes.c_ulong(nj), *[p(B[k]) for k in FIELDS[:8]],
+                     ctypes.c_double(1.0 / 64), *[p(o) for o in out2])
+    assert_close("tstep", out2, g, "s1_tstep_AB", 1e-13)
+
+
+def test_host_pointer_abi_f32(golden):
+    from tupan_amd import _native
+    lib = ctypes.CDLL(_native.LIB32_PATH)
+    g = golden("pairwise_f32.npz")
+    A = split_set(g, "s0", "A")
+    B = split_set(g, "s0", "B")
+    P = ctypes.c_void_p
+
+    def p(a):
+        return np.ascontiguousarray(a, dtype=np.float32).ctypes.data_as(P)
+    ni, nj = len(A["mass"]), len(B["mass"])
+    out = [np.zeros(ni, np.float32) for _ in range(3)]
+    lib.acc_kernel.restype = None
+    lib.acc_kernel(ctypes.c_uint(ni), *[p(A[k]) for k in FIELDS[:5]],
+                   ctypes.c_uint(nj), *[p(B[k]) for k in FIELDS[:5]],
+                   *[p(o) for o in out])
+    assert_close("acc", out, g, "s0_acc_AB", 2e-5)
+
+
+def test_numpy_staging_through_extension(golden):
+    """Host numpy arrays through the HIP backend (staged copies, outputs
+    copied back by map_buffers), as the reference's OpenCL backend does."""
+    from tupan_amd.lib import extensions
+
+    class HostPS(object):
+        def __init__(self, d):
+            self.__dict__.update({k: np.array(v) for k, v in d.items()})
+            self.n = len(d["mass"])
+
+        def register_auxiliary_attribute(self, name, sctype):
+            self.__dict__[name] = np.zeros(self.n)
+    g = golden("pairwise_f64.npz")
+    A = HostPS(split_set(g, "s1", "A"))
+    B = HostPS(split_set(g, "s1", "B"))
+    res = extensions.AccJerk("HIP", "float64").calc(A, B)
+    assert all(isinstance(r, np.ndarray) for r in res)
+    assert res[0] is A.ax
+    assert_close("acc_jerk", list(res), g, "s1_acc_jerk_AB", 1e-13)
+
+
+def test_empty_and_degenerate():
+    from tupan_amd.lib import extensions
+    rng = np.random.default_rng(3)
+    d = {k: rng.random(5) for k in FIELDS}
+    d["eps2"] = np.zeros(5)
+    ps = device_ps(d)
+    empty = device_ps({k: v[:0] for k, v in d.items()})
+    # nj == 0: sums are zero, Tstep gives eta
+    r = extensions.AccJerk("HIP", "float64").calc(ps, empty)
+    for x in r:
+        assert torch.all(x == 0)
+    r = extensions.Tstep("HIP", "float64").calc(ps, empty, 0.25)
+    assert torch.all(r[0] == 0.25) and torch.all(r[1] == 0.25)
+    # ni == 0: nothing to do
+    r = extensions.Acc("HIP", "float64").calc(empty, ps)
+    assert all(x.numel() == 0 for x in r)
+    # a lone particle with itself: masked, exactly zero
+    one = device_ps({k: v[:1] for k, v in d.items()})
+    r = extensions.AccJerk("HIP", "float64").calc(one, one)
+    for x in r:
+        assert float(x[0]) == 0.0
+    r = extensions.Phi("HIP", "float64").calc(one, one)
+    assert float(r[0][0]) == 0.0
+
+
+@pytest.mark.parametrize("mode", ["fp64", "split32"])
+def test_large_random_rows_vs_oracle(mode):
+    """N = 65536 uniform cube: a sample of rows against the oracle, plus
+    momentum conservation over all rows (sum m_i a_i = 0 up to rounding)."""
+    from oracle import Oracle
+    from tupan_amd.lib import extensions
+    rng = np.random.default_rng(11)
+    n = 65536
+    d = {k: rng.random(n) for k in ("rx", "ry", "rz", "vx", "vy", "vz")}
+    d["mass"] = np.full(n, 1.0 / n)
+    d["eps2"] = np.full(n, 0.5e-8)
+    ps = device_ps(d)
+    res = extensions.AccJerk("HIP", "float64", mode=mode).calc(ps, ps)
+    got = [r.detach().cpu().numpy() for r in res]
+    rows = rng.choice(n, 64, replace=False)
+    sub = {k: v[rows] for k, v in d.items()}
+    ref = Oracle("float64").acc_jerk(sub, d)
+    bar = 1e-13 if mode == "fp64" else 1e-6
+    e = vec_rel_err([got[k][rows] for k in range(3)], ref[:3])
+    assert e <= bar, e
+    fx = float(np.sum(d["mass"] * got[0]))
+    amag = float(np.sum(d["mass"] * np.abs(got[0])))
+    assert abs(fx) <= (1e-12 if mode == "fp64" else 1e-6) * amag

@@ -1,0 +1,119 @@
+"""ctypes binding of libtupan_hip.so (include/tupan_hip.h).
+
+The library is built in-tree by ``python -m tupan_amd.csrc.build`` into
+``tupan_amd/_lib``.  There is no fallback: if the library is missing or no
+HIP device is present, calls raise.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_lib", "libtupan_hip.so")
+LIB32_PATH = os.path.join(HERE, "_lib", "libtupan_hip_f32.so")
+
+# Status codes and modes (include/tupan_hip.h).
+OK = 0
+MODE_FP64 = 0
+MODE_SPLIT32 = 1
+MODE_IEEE64 = 2
+MODES = {"fp64": MODE_FP64, "split32": MODE_SPLIT32, "ieee64": MODE_IEEE64}
+
+K_PHI, K_ACC, K_ACC_JERK, K_SNAP_CRACKLE, K_TSTEP = range(5)
+
+_lib = None
+
+_P = ctypes.c_void_p
+_S = ctypes.c_size_t
+_PP = ctypes.POINTER(ctypes.c_void_p)
+
+
+class TupanHipError(RuntimeError):
+    pass
+
+
+def _declare(lib):
+    lib.tupan_last_error.restype = ctypes.c_char_p
+    lib.tupan_last_error.argtypes = []
+    lib.tupan_version.restype = ctypes.c_int
+    lib.tupan_device_info.restype = ctypes.c_int
+    lib.tupan_device_info.argtypes = [ctypes.POINTER(ctypes.c_int)] * 2
+    lib.tupan_timing.restype = ctypes.c_int
+    lib.tupan_timing.argtypes = [ctypes.c_int]
+    lib.tupan_timing_read.restype = ctypes.c_int
+    lib.tupan_timing_read.argtypes = [ctypes.POINTER(ctypes.c_double),
+                                      ctypes.POINTER(ctypes.c_int)]
+    lib.tupan_pairwise.restype = ctypes.c_int
+    lib.tupan_pairwise.argtypes = [ctypes.c_int, ctypes.c_int, _S, _PP, _S, _PP,
+                                   _PP, ctypes.c_double, _P]
+    lib.tupan_pairwise_f32.restype = ctypes.c_int
+    lib.tupan_pairwise_f32.argtypes = [ctypes.c_int, ctypes.c_int, _S, _PP, _S,
+                                       _PP, _PP, ctypes.c_float, _P]
+    lib.tupan_sakura.restype = ctypes.c_int
+    lib.tupan_sakura.argtypes = ([_S] + [_P] * 8 + [_S] + [_P] * 8
+                                 + [ctypes.c_double, ctypes.c_long]
+                                 + [_P] * 6 + [_P, _P, _P])
+    lib.tupan_kepler_pairs.restype = ctypes.c_int
+    lib.tupan_kepler_pairs.argtypes = ([_S, _P] + [_P] * 8
+                                       + [ctypes.c_double, _P, _P])
+    return lib
+
+
+def lib():
+    """The loaded library (loads on first use; raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise TupanHipError(
+                "libtupan_hip.so is not built at %s; run "
+                "`python -m tupan_amd.csrc.build`" % LIB_PATH)
+        _lib = _declare(ctypes.CDLL(LIB_PATH))
+    return _lib
+
+
+def last_error():
+    return lib().tupan_last_error().decode(errors="replace")
+
+
+def check(rc, what):
+    if rc != OK:
+        raise TupanHipError("%s failed (%d): %s" % (what, rc, last_error()))
+
+
+def ptr_array(ptrs):
+    arr = (ctypes.c_void_p * max(1, len(ptrs)))()
+    for k, p in enumerate(ptrs):
+        arr[k] = p
+    return arr
+
+
+def timing(enable):
+    check(lib().tupan_timing(1 if enable else 0), "tupan_timing")
+
+
+def timing_read():
+    """(summed kernel milliseconds, launches) since timing(True)."""
+    ms = ctypes.c_double()
+    cnt = ctypes.c_int()
+    check(lib().tupan_timing_read(ctypes.byref(ms), ctypes.byref(cnt)),
+          "tupan_timing_read")
+    return ms.value, cnt.value
+
+
+def device_info():
+    cu = ctypes.c_int()
+    wg = ctypes.c_int()
+    rc = lib().tupan_device_info(ctypes.byref(cu), ctypes.byref(wg))
+    check(rc, "tupan_device_info")
+    return cu.value, wg.value
+
+
+# Names exported by the shared libraries (used by tests that check the ABI).
+REFERENCE_ABI = ("phi_kernel", "acc_kernel", "acc_jerk_kernel",
+                 "snap_crackle_kernel", "tstep_kernel", "sakura_kernel",
+                 "kepler_solver_kernel")
+DEVICE_ABI = ("tupan_last_error", "tupan_version", "tupan_device_info",
+              "tupan_timing", "tupan_timing_read",
+              "tupan_phi", "tupan_acc", "tupan_acc_jerk", "tupan_snap_crackle",
+              "tupan_tstep", "tupan_pairwise", "tupan_pairwise_f32",
+              "tupan_sakura", "tupan_kepler_pairs", "tupan_host_pairwise_f32",
+              "tupan_host_sakura_f32", "tupan_host_kepler_f32")

@@ -1,0 +1,578 @@
+// C ABI of libtupan_hip.so: device state, scratch workspace, error state,
+// the device-resident entry points and the reference-compatible host-pointer
+// layer (tupan/lib/src/libtupan.h:2-246).
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "tupan_internal.h"
+
+namespace tupan {
+
+namespace {
+
+thread_local std::string g_error;
+std::mutex g_dev_mutex;
+std::vector<Device> g_devices;
+std::vector<void*> g_stage;       // host-layer staging buffers, per device
+std::vector<size_t> g_stage_bytes;
+
+Device& device_slot()
+{
+    int id = 0;
+    if (hipGetDevice(&id) != hipSuccess) id = 0;
+    std::lock_guard<std::mutex> lock(g_dev_mutex);
+    if (static_cast<int>(g_devices.size()) <= id) {
+        g_devices.resize(id + 1);
+        g_stage.resize(id + 1, nullptr);
+        g_stage_bytes.resize(id + 1, 0);
+    }
+    Device& d = g_devices[id];
+    if (d.id < 0) {
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, id) == hipSuccess) {
+            d.cu_count = p.multiProcessorCount;
+        } else {
+            d.cu_count = 256;
+        }
+        // four 256-thread workgroups per CU keeps 16 waves per CU in flight
+        d.target_wg = 4 * d.cu_count;
+        d.id = id;
+    }
+    return d;
+}
+
+}  // namespace
+
+const Device& current_device() { return device_slot(); }
+
+void* workspace(size_t bytes, hipStream_t)
+{
+    Device& d = device_slot();
+    if (d.ws_bytes >= bytes) return d.ws;
+    size_t want = bytes + bytes / 4;
+    if (d.ws) {
+        // earlier launches may still read the old buffer
+        (void)hipDeviceSynchronize();
+        (void)hipFree(d.ws);
+        d.ws = nullptr;
+        d.ws_bytes = 0;
+    }
+    if (hipMalloc(&d.ws, want) != hipSuccess) {
+        (void)hipGetLastError();
+        d.ws = nullptr;
+        return nullptr;
+    }
+    d.ws_bytes = want;
+    return d.ws;
+}
+
+static void* staging(size_t bytes)
+{
+    Device& d = device_slot();
+    const int id = d.id;
+    if (g_stage_bytes[id] >= bytes) return g_stage[id];
+    if (g_stage[id]) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(g_stage[id]);
+        g_stage[id] = nullptr;
+        g_stage_bytes[id] = 0;
+    }
+    size_t want = bytes + bytes / 4 + 4096;
+    if (hipMalloc(&g_stage[id], want) != hipSuccess) {
+        (void)hipGetLastError();
+        g_stage[id] = nullptr;
+        return nullptr;
+    }
+    g_stage_bytes[id] = want;
+    return g_stage[id];
+}
+
+// ---------------------------------------------------------------------------
+// Kernel timing: pairs of HIP events around each main kernel launch.
+// ---------------------------------------------------------------------------
+namespace {
+bool g_timing = false;
+std::vector<std::pair<hipEvent_t, hipEvent_t>> g_events;
+size_t g_events_used = 0;
+}  // namespace
+
+void timing_begin(hipStream_t stream)
+{
+    if (!g_timing) return;
+    if (g_events_used == g_events.size()) {
+        hipEvent_t a, b;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+            g_timing = false;
+            return;
+        }
+        g_events.emplace_back(a, b);
+    }
+    (void)hipEventRecord(g_events[g_events_used].first, stream);
+}
+
+void timing_end(hipStream_t stream)
+{
+    if (!g_timing || g_events_used >= g_events.size()) return;
+    (void)hipEventRecord(g_events[g_events_used].second, stream);
+    ++g_events_used;
+}
+
+int set_error(int code, const char* msg)
+{
+    g_error = msg;
+    return code;
+}
+
+int check_hip(hipError_t e, const char* what)
+{
+    if (e == hipSuccess) return TUPAN_OK;
+    g_error = std::string(what) + ": " + hipGetErrorString(e);
+    return TUPAN_ERR_HIP;
+}
+
+int check_launch(const char* what) { return check_hip(hipGetLastError(), what); }
+
+// ---------------------------------------------------------------------------
+// Host-pointer layer: stage to the device, run, copy back, synchronously.
+// ---------------------------------------------------------------------------
+template <class T>
+static void fail_outputs(const char* fn, T* const* out, const size_t* lens, int nout)
+{
+    fprintf(stderr, "tupan_hip: %s failed: %s\n", fn, g_error.c_str());
+    for (int k = 0; k < nout; ++k)
+        for (size_t i = 0; i < lens[k]; ++i) out[k][i] = static_cast<T>(NAN);
+}
+
+template <class T>
+static void host_pairwise(const char* fn, int kernel, size_t ni, const T* const* ih, size_t nj,
+                          const T* const* jh, T* const* oh, T eta)
+{
+    const int nin = kernel_in_fields(kernel);
+    const int nout = kernel_out_fields(kernel);
+    size_t lens[6];
+    for (int k = 0; k < nout; ++k) lens[k] = ni;
+    if (ni == 0) return;
+    const size_t bytes = sizeof(T) * (nin * (ni + nj) + nout * ni);
+    T* base = static_cast<T*>(staging(bytes));
+    if (!base) {
+        set_error(TUPAN_ERR_ALLOC, "staging allocation failed");
+        fail_outputs(fn, oh, lens, nout);
+        return;
+    }
+    const T* ip[14];
+    const T* jp[14];
+    T* op[6];
+    int rc = TUPAN_OK;
+    T* cur = base;
+    for (int f = 0; f < nin && rc == TUPAN_OK; ++f) {
+        rc = check_hip(hipMemcpy(cur, ih[f], sizeof(T) * ni, hipMemcpyHostToDevice), "copy in");
+        ip[f] = cur;
+        cur += ni;
+    }
+    for (int f = 0; f < nin && rc == TUPAN_OK; ++f) {
+        if (nj) rc = check_hip(hipMemcpy(cur, jh[f], sizeof(T) * nj, hipMemcpyHostToDevice), "copy in");
+        jp[f] = cur;
+        cur += nj;
+    }
+    for (int k = 0; k < nout; ++k) {
+        op[k] = cur;
+        cur += ni;
+    }
+    if (rc == TUPAN_OK) rc = pairwise_call<T>(kernel, TUPAN_MODE_FP64, ni, ip, nj, jp, op, eta, nullptr);
+    for (int k = 0; k < nout && rc == TUPAN_OK; ++k)
+        rc = check_hip(hipMemcpy(oh[k], op[k], sizeof(T) * ni, hipMemcpyDeviceToHost), "copy out");
+    if (rc != TUPAN_OK) fail_outputs(fn, oh, lens, nout);
+}
+
+static void host_sakura(size_t ni, const double* const* ih, size_t nj, const double* const* jh,
+                        double dt, long flag, double* const* oh)
+{
+    size_t lens[6] = {ni, ni, ni, ni, ni, ni};
+    if (ni == 0) return;
+    const size_t bytes = sizeof(double) * (8 * (ni + nj) + 6 * ni) + 64;
+    double* base = static_cast<double*>(staging(bytes));
+    if (!base) {
+        set_error(TUPAN_ERR_ALLOC, "staging allocation failed");
+        fail_outputs("sakura_kernel", oh, lens, 6);
+        return;
+    }
+    const double* ip[8];
+    const double* jp[8];
+    double* op[6];
+    int rc = TUPAN_OK;
+    double* cur = base;
+    int* status = reinterpret_cast<int*>(cur);
+    cur += 8;
+    for (int f = 0; f < 8 && rc == TUPAN_OK; ++f) {
+        rc = check_hip(hipMemcpy(cur, ih[f], 8 * ni, hipMemcpyHostToDevice), "copy in");
+        ip[f] = cur;
+        cur += ni;
+    }
+    for (int f = 0; f < 8 && rc == TUPAN_OK; ++f) {
+        if (nj) rc = check_hip(hipMemcpy(cur, jh[f], 8 * nj, hipMemcpyHostToDevice), "copy in");
+        jp[f] = cur;
+        cur += nj;
+    }
+    for (int k = 0; k < 6; ++k) {
+        op[k] = cur;
+        cur += ni;
+    }
+    if (rc == TUPAN_OK) rc = check_hip(hipMemset(status, 0, sizeof(int)), "memset");
+    if (rc == TUPAN_OK) rc = sakura_call(ni, ip, nj, jp, dt, flag, op, nullptr, status, nullptr);
+    int st = 0;
+    if (rc == TUPAN_OK)
+        rc = check_hip(hipMemcpy(&st, status, sizeof(int), hipMemcpyDeviceToHost), "copy out");
+    if (rc == TUPAN_OK && st != 0)
+        rc = set_error(TUPAN_ERR_SOLVER, "Kepler solver exhausted its bounded step subdivision");
+    for (int k = 0; k < 6 && rc == TUPAN_OK; ++k)
+        rc = check_hip(hipMemcpy(oh[k], op[k], 8 * ni, hipMemcpyDeviceToHost), "copy out");
+    if (rc != TUPAN_OK) fail_outputs("sakura_kernel", oh, lens, 6);
+}
+
+}  // namespace tupan
+
+using namespace tupan;
+
+extern "C" {
+
+const char* tupan_last_error(void) { return g_error.c_str(); }
+
+int tupan_version(void) { return 100; }
+
+int tupan_timing(int enable)
+{
+    g_timing = enable != 0;
+    g_events_used = 0;
+    return TUPAN_OK;
+}
+
+int tupan_timing_read(double* total_ms, int* count)
+{
+    double tot = 0.0;
+    for (size_t k = 0; k < g_events_used; ++k) {
+        int rc = check_hip(hipEventSynchronize(g_events[k].second), "event sync");
+        if (rc != TUPAN_OK) return rc;
+        float ms = 0.0f;
+        rc = check_hip(hipEventElapsedTime(&ms, g_events[k].first, g_events[k].second),
+                       "event time");
+        if (rc != TUPAN_OK) return rc;
+        tot += ms;
+    }
+    if (total_ms) *total_ms = tot;
+    if (count) *count = static_cast<int>(g_events_used);
+    return TUPAN_OK;
+}
+
+int tupan_device_info(int* cu_count, int* target_workgroups)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        (void)hipGetLastError();
+        if (cu_count) *cu_count = -1;
+        if (target_workgroups) *target_workgroups = -1;
+        return set_error(TUPAN_ERR_HIP, "no HIP device");
+    }
+    const Device& d = current_device();
+    if (cu_count) *cu_count = d.cu_count;
+    if (target_workgroups) *target_workgroups = d.target_wg;
+    return TUPAN_OK;
+}
+
+// ---- host-pointer layer (reference ABI, FP64) ------------------------------
+
+void phi_kernel(const unsigned long ni, const double* _im, const double* _irx,
+                const double* _iry, const double* _irz, const double* _ie2,
+                const unsigned long nj, const double* _jm, const double* _jrx,
+                const double* _jry, const double* _jrz, const double* _je2, double* _iphi)
+{
+    const double* ih[5] = {_im, _irx, _iry, _irz, _ie2};
+    const double* jh[5] = {_jm, _jrx, _jry, _jrz, _je2};
+    double* oh[1] = {_iphi};
+    host_pairwise<double>("phi_kernel", K_PHI, ni, ih, nj, jh, oh, 0.0);
+}
+
+void acc_kernel(const unsigned long ni, const double* _im, const double* _irx,
+                const double* _iry, const double* _irz, const double* _ie2,
+                const unsigned long nj, const double* _jm, const double* _jrx,
+                const double* _jry, const double* _jrz, const double* _je2, double* _iax,
+                double* _iay, double* _iaz)
+{
+    const double* ih[5] = {_im, _irx, _iry, _irz, _ie2};
+    const double* jh[5] = {_jm, _jrx, _jry, _jrz, _je2};
+    double* oh[3] = {_iax, _iay, _iaz};
+    host_pairwise<double>("acc_kernel", K_ACC, ni, ih, nj, jh, oh, 0.0);
+}
+
+void acc_jerk_kernel(const unsigned long ni, const double* _im, const double* _irx,
+                     const double* _iry, const double* _irz, const double* _ie2,
+                     const double* _ivx, const double* _ivy, const double* _ivz,
+                     const unsigned long nj, const double* _jm, const double* _jrx,
+                     const double* _jry, const double* _jrz, const double* _je2,
+                     const double* _jvx, const double* _jvy, const double* _jvz, double* _iax,
+                     double* _iay, double* _iaz, double* _ijx, double* _ijy, double* _ijz)
+{
+    const double* ih[8] = {_im, _irx, _iry, _irz, _ie2, _ivx, _ivy, _ivz};
+    const double* jh[8] = {_jm, _jrx, _jry, _jrz, _je2, _jvx, _jvy, _jvz};
+    double* oh[6] = {_iax, _iay, _iaz, _ijx, _ijy, _ijz};
+    host_pairwise<double>("acc_jerk_kernel", K_ACC_JERK, ni, ih, nj, jh, oh, 0.0);
+}
+
+void snap_crackle_kernel(const unsigned long ni, const double* _im, const double* _irx,
+                         const double* _iry, const double* _irz, const double* _ie2,
+                         const double* _ivx, const double* _ivy, const double* _ivz,
+                         const double* _iax, const double* _iay, const double* _iaz,
+                         const double* _ijx, const double* _ijy, const double* _ijz,
+                         const unsigned long nj, const double* _jm, const double* _jrx,
+                         const double* _jry, const double* _jrz, const double* _je2,
+                         const double* _jvx, const double* _jvy, const double* _jvz,
+                         const double* _jax, const double* _jay, const double* _jaz,
+                         const double* _jjx, const double* _jjy, const double* _jjz,
+                         double* _isx, double* _isy, double* _isz, double* _icx, double* _icy,
+                         double* _icz)
+{
+    const double* ih[14] = {_im, _irx, _iry, _irz, _ie2, _ivx, _ivy,
+                            _ivz, _iax, _iay, _iaz, _ijx, _ijy, _ijz};
+    const double* jh[14] = {_jm, _jrx, _jry, _jrz, _je2, _jvx, _jvy,
+                            _jvz, _jax, _jay, _jaz, _jjx, _jjy, _jjz};
+    double* oh[6] = {_isx, _isy, _isz, _icx, _icy, _icz};
+    host_pairwise<double>("snap_crackle_kernel", K_SNAP_CRACKLE, ni, ih, nj, jh, oh, 0.0);
+}
+
+void tstep_kernel(const unsigned long ni, const double* _im, const double* _irx,
+                  const double* _iry, const double* _irz, const double* _ie2,
+                  const double* _ivx, const double* _ivy, const double* _ivz,
+                  const unsigned long nj, const double* _jm, const double* _jrx,
+                  const double* _jry, const double* _jrz, const double* _je2,
+                  const double* _jvx, const double* _jvy, const double* _jvz, const double eta,
+                  double* _idt_a, double* _idt_b)
+{
+    const double* ih[8] = {_im, _irx, _iry, _irz, _ie2, _ivx, _ivy, _ivz};
+    const double* jh[8] = {_jm, _jrx, _jry, _jrz, _je2, _jvx, _jvy, _jvz};
+    double* oh[2] = {_idt_a, _idt_b};
+    host_pairwise<double>("tstep_kernel", K_TSTEP, ni, ih, nj, jh, oh, eta);
+}
+
+void sakura_kernel(const unsigned long ni, const double* _im, const double* _irx,
+                   const double* _iry, const double* _irz, const double* _ie2,
+                   const double* _ivx, const double* _ivy, const double* _ivz,
+                   const unsigned long nj, const double* _jm, const double* _jrx,
+                   const double* _jry, const double* _jrz, const double* _je2,
+                   const double* _jvx, const double* _jvy, const double* _jvz, const double dt,
+                   const long flag, double* _idrx, double* _idry, double* _idrz,
+                   double* _idvx, double* _idvy, double* _idvz)
+{
+    const double* ih[8] = {_im, _irx, _iry, _irz, _ie2, _ivx, _ivy, _ivz};
+    const double* jh[8] = {_jm, _jrx, _jry, _jrz, _je2, _jvx, _jvy, _jvz};
+    double* oh[6] = {_idrx, _idry, _idrz, _idvx, _idvy, _idvz};
+    host_sakura(ni, ih, nj, jh, dt, flag, oh);
+}
+
+void kepler_solver_kernel(const double* _im, const double* _irx, const double* _iry,
+                          const double* _irz, const double* _ie2, const double* _ivx,
+                          const double* _ivy, const double* _ivz, const double dt,
+                          double* _ir1x, double* _ir1y, double* _ir1z, double* _iv1x,
+                          double* _iv1y, double* _iv1z)
+{
+    // stage: m rx ry rz e2 vx vy vz (2 each), pair index, status
+    double* outs[6] = {_ir1x, _ir1y, _ir1z, _iv1x, _iv1y, _iv1z};
+    size_t lens[6] = {2, 2, 2, 2, 2, 2};
+    double* base = static_cast<double*>(staging(sizeof(double) * 16 + 64));
+    if (!base) {
+        set_error(TUPAN_ERR_ALLOC, "staging allocation failed");
+        fail_outputs("kepler_solver_kernel", outs, lens, 6);
+        return;
+    }
+    const double* in[8] = {_im, _irx, _iry, _irz, _ie2, _ivx, _ivy, _ivz};
+    double host[16];
+    for (int f = 0; f < 8; ++f) {
+        host[2 * f] = in[f][0];
+        host[2 * f + 1] = in[f][1];
+    }
+    int* meta = reinterpret_cast<int*>(base + 16);
+    int hmeta[3] = {0, 1, 0};
+    int rc = check_hip(hipMemcpy(base, host, sizeof(host), hipMemcpyHostToDevice), "copy in");
+    if (rc == TUPAN_OK)
+        rc = check_hip(hipMemcpy(meta, hmeta, sizeof(hmeta), hipMemcpyHostToDevice), "copy in");
+    if (rc == TUPAN_OK)
+        rc = kepler_pairs_call(1, meta, base + 0, base + 2, base + 4, base + 6, base + 8,
+                               base + 10, base + 12, base + 14, dt, meta + 2, nullptr);
+    if (rc == TUPAN_OK)
+        rc = check_hip(hipMemcpy(host, base, sizeof(host), hipMemcpyDeviceToHost), "copy out");
+    if (rc == TUPAN_OK)
+        rc = check_hip(hipMemcpy(hmeta, meta, sizeof(hmeta), hipMemcpyDeviceToHost), "copy out");
+    if (rc == TUPAN_OK && hmeta[2] != 0)
+        rc = set_error(TUPAN_ERR_SOLVER, "Kepler solver exhausted its bounded step subdivision");
+    if (rc != TUPAN_OK) {
+        fail_outputs("kepler_solver_kernel", outs, lens, 6);
+        return;
+    }
+    const int map[6] = {1, 2, 3, 5, 6, 7};  // rx ry rz vx vy vz
+    for (int k = 0; k < 6; ++k) {
+        outs[k][0] = host[2 * map[k]];
+        outs[k][1] = host[2 * map[k] + 1];
+    }
+}
+
+// ---- host layer of the float build (used by libtupan_hip_f32.so) ----------
+
+int tupan_host_pairwise_f32(int kernel, size_t ni, const float* const* ih, size_t nj,
+                            const float* const* jh, float* const* oh, float eta)
+{
+    static const char* names[5] = {"phi_kernel", "acc_kernel", "acc_jerk_kernel",
+                                   "snap_crackle_kernel", "tstep_kernel"};
+    if (kernel < 0 || kernel > 4) return set_error(TUPAN_ERR_ARG, "unknown pairwise kernel");
+    g_error.clear();
+    host_pairwise<float>(names[kernel], kernel, ni, ih, nj, jh, oh, eta);
+    return g_error.empty() ? TUPAN_OK : TUPAN_ERR_HIP;
+}
+
+// Sakura and Kepler of the float build run in FP64 (the device solver is
+// FP64 only) and round the results to float.
+int tupan_host_sakura_f32(size_t ni, const float* const* ih, size_t nj, const float* const* jh,
+                          float dt, int flag, float* const* oh)
+{
+    std::vector<double> buf(8 * (ni + nj) + 6 * ni);
+    const double* ip[8];
+    const double* jp[8];
+    double* op[6];
+    double* cur = buf.data();
+    for (int f = 0; f < 8; ++f) {
+        for (size_t i = 0; i < ni; ++i) cur[i] = ih[f][i];
+        ip[f] = cur;
+        cur += ni;
+    }
+    for (int f = 0; f < 8; ++f) {
+        for (size_t j = 0; j < nj; ++j) cur[j] = jh[f][j];
+        jp[f] = cur;
+        cur += nj;
+    }
+    for (int k = 0; k < 6; ++k) {
+        op[k] = cur;
+        cur += ni;
+    }
+    g_error.clear();
+    host_sakura(ni, ip, nj, jp, static_cast<double>(dt), static_cast<long>(flag), op);
+    for (int k = 0; k < 6; ++k)
+        for (size_t i = 0; i < ni; ++i) oh[k][i] = static_cast<float>(op[k][i]);
+    return g_error.empty() ? TUPAN_OK : TUPAN_ERR_HIP;
+}
+
+int tupan_host_kepler_f32(const float* const* in8, float dt, float* const* out6)
+{
+    double in[8][2], o[6][2];
+    const double* ip[8];
+    double* op[6];
+    for (int f = 0; f < 8; ++f) {
+        in[f][0] = in8[f][0];
+        in[f][1] = in8[f][1];
+        ip[f] = in[f];
+    }
+    for (int k = 0; k < 6; ++k) op[k] = o[k];
+    g_error.clear();
+    kepler_solver_kernel(ip[0], ip[1], ip[2], ip[3], ip[4], ip[5], ip[6], ip[7],
+                         static_cast<double>(dt), op[0], op[1], op[2], op[3], op[4], op[5]);
+    for (int k = 0; k < 6; ++k) {
+        out6[k][0] = static_cast<float>(o[k][0]);
+        out6[k][1] = static_cast<float>(o[k][1]);
+    }
+    return g_error.empty() ? TUPAN_OK : TUPAN_ERR_HIP;
+}
+
+// ---- device-resident layer --------------------------------------------------
+
+int tupan_pairwise(int kernel, int mode, size_t ni, const double* const* ifields, size_t nj,
+                   const double* const* jfields, double* const* out, double eta, void* stream)
+{
+    return pairwise_call<double>(kernel, mode, ni, ifields, nj, jfields, out, eta, stream);
+}
+
+int tupan_pairwise_f32(int kernel, int mode, size_t ni, const float* const* ifields, size_t nj,
+                       const float* const* jfields, float* const* out, float eta, void* stream)
+{
+    return pairwise_call<float>(kernel, mode, ni, ifields, nj, jfields, out, eta, stream);
+}
+
+int tupan_phi(int mode, size_t ni, const double* im, const double* irx, const double* iry,
+              const double* irz, const double* ie2, size_t nj, const double* jm,
+              const double* jrx, const double* jry, const double* jrz, const double* je2,
+              double* iphi, void* stream)
+{
+    const double* ip[5] = {im, irx, iry, irz, ie2};
+    const double* jp[5] = {jm, jrx, jry, jrz, je2};
+    double* op[1] = {iphi};
+    return pairwise_call<double>(K_PHI, mode, ni, ip, nj, jp, op, 0.0, stream);
+}
+
+int tupan_acc(int mode, size_t ni, const double* im, const double* irx, const double* iry,
+              const double* irz, const double* ie2, size_t nj, const double* jm,
+              const double* jrx, const double* jry, const double* jrz, const double* je2,
+              double* iax, double* iay, double* iaz, void* stream)
+{
+    const double* ip[5] = {im, irx, iry, irz, ie2};
+    const double* jp[5] = {jm, jrx, jry, jrz, je2};
+    double* op[3] = {iax, iay, iaz};
+    return pairwise_call<double>(K_ACC, mode, ni, ip, nj, jp, op, 0.0, stream);
+}
+
+int tupan_acc_jerk(int mode, size_t ni, const double* im, const double* irx, const double* iry,
+                   const double* irz, const double* ie2, const double* ivx, const double* ivy,
+                   const double* ivz, size_t nj, const double* jm, const double* jrx,
+                   const double* jry, const double* jrz, const double* je2, const double* jvx,
+                   const double* jvy, const double* jvz, double* iax, double* iay, double* iaz,
+                   double* ijx, double* ijy, double* ijz, void* stream)
+{
+    const double* ip[8] = {im, irx, iry, irz, ie2, ivx, ivy, ivz};
+    const double* jp[8] = {jm, jrx, jry, jrz, je2, jvx, jvy, jvz};
+    double* op[6] = {iax, iay, iaz, ijx, ijy, ijz};
+    return pairwise_call<double>(K_ACC_JERK, mode, ni, ip, nj, jp, op, 0.0, stream);
+}
+
+int tupan_snap_crackle(int mode, size_t ni, const double* const* ifields, size_t nj,
+                       const double* const* jfields, double* isx, double* isy, double* isz,
+                       double* icx, double* icy, double* icz, void* stream)
+{
+    double* op[6] = {isx, isy, isz, icx, icy, icz};
+    return pairwise_call<double>(K_SNAP_CRACKLE, mode, ni, ifields, nj, jfields, op, 0.0,
+                                 stream);
+}
+
+int tupan_tstep(int mode, size_t ni, const double* im, const double* irx, const double* iry,
+                const double* irz, const double* ie2, const double* ivx, const double* ivy,
+                const double* ivz, size_t nj, const double* jm, const double* jrx,
+                const double* jry, const double* jrz, const double* je2, const double* jvx,
+                const double* jvy, const double* jvz, double eta, double* idt_a, double* idt_b,
+                void* stream)
+{
+    const double* ip[8] = {im, irx, iry, irz, ie2, ivx, ivy, ivz};
+    const double* jp[8] = {jm, jrx, jry, jrz, je2, jvx, jvy, jvz};
+    double* op[2] = {idt_a, idt_b};
+    return pairwise_call<double>(K_TSTEP, mode, ni, ip, nj, jp, op, eta, stream);
+}
+
+int tupan_sakura(size_t ni, const double* im, const double* irx, const double* iry,
+                 const double* irz, const double* ie2, const double* ivx, const double* ivy,
+                 const double* ivz, size_t nj, const double* jm, const double* jrx,
+                 const double* jry, const double* jrz, const double* je2, const double* jvx,
+                 const double* jvy, const double* jvz, double dt, long flag, double* idrx,
+                 double* idry, double* idrz, double* idvx, double* idvy, double* idvz,
+                 unsigned long long* counters, int* status, void* stream)
+{
+    const double* ip[8] = {im, irx, iry, irz, ie2, ivx, ivy, ivz};
+    const double* jp[8] = {jm, jrx, jry, jrz, je2, jvx, jvy, jvz};
+    double* op[6] = {idrx, idry, idrz, idvx, idvy, idvz};
+    return sakura_call(ni, ip, nj, jp, dt, flag, op, counters, status, stream);
+}
+
+int tupan_kepler_pairs(size_t npairs, const int* idx, const double* m, double* rx, double* ry,
+                       double* rz, const double* e2, double* vx, double* vy, double* vz,
+                       double dt, int* status, void* stream)
+{
+    return kepler_pairs_call(npairs, idx, m, rx, ry, rz, e2, vx, vy, vz, dt, status, stream);
+}
+
+}  // extern "C"

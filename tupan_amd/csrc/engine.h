@@ -1,0 +1,151 @@
+// All-pairs tile engine for gfx950.
+//
+// One workgroup owns BLOCK*WPT i-particles (WPT per thread, held in VGPRs)
+// and one contiguous range of j-particles (a "j-split").  The j range is
+// streamed through LDS in tiles of TILE slots: the workgroup loads a tile
+// with coalesced SoA reads from HBM/L2, converts it to the interaction's
+// slot format (AoS, 16-byte aligned, so each wave reads a slot with 128-bit
+// broadcast LDS reads), and every thread applies the slot to each of its WPT
+// i-particles.  With nsplit > 1 every workgroup writes its partial
+// accumulators to a scratch array and a second kernel combines the splits in
+// a fixed order, so results are deterministic run to run.
+//
+// An interaction policy `Op` supplies:
+//   Args                        kernel argument block
+//   Slot, IReg, Acc, Frame      LDS slot, i registers, accumulators, the
+//                               per-workgroup frame (centre for split modes)
+//   kNacc                       number of FP64 partial accumulators
+//   kFlush                      j count between Acc flushes (split modes)
+//   frame(a, ibase)             per-workgroup frame
+//   load_i(a, fr, i, r)         i registers
+//   load_j(a, fr, j, s)         j slot
+//   zero(acc), interact(r, s, acc), flush(acc)
+//   get(acc, k)                 FP64 value of accumulator k (after flush)
+//   combine(tot, part)          merge two partial accumulator sets
+//   store(a, i, tot)            final outputs from combined accumulators
+#pragma once
+
+#include "common.h"
+
+namespace tupan {
+
+template <class Op, int BLOCK, int WPT, int TILE, int UNROLL>
+__global__ __launch_bounds__(BLOCK) void pair_kernel(typename Op::Args a)
+{
+    using Slot = typename Op::Slot;
+    using IReg = typename Op::IReg;
+    using Acc = typename Op::Acc;
+    __shared__ Slot tile[TILE];
+
+    const GridPos g = grid_pos(a.nsplit);
+    const int tid = static_cast<int>(threadIdx.x);
+    const int ibase = g.iblock * (BLOCK * WPT);
+    const typename Op::Frame fr = Op::frame(a, ibase);
+
+    IReg ir[WPT];
+    Acc acc[WPT];
+#pragma unroll
+    for (int w = 0; w < WPT; ++w) {
+        int i = ibase + w * BLOCK + tid;
+        Op::load_i(a, fr, i < a.ni ? i : a.ni - 1, ir[w]);
+        Op::zero(acc[w]);
+    }
+
+    const int j0 = g.jsplit * a.jchunk;
+    const int j1 = min(a.nj, j0 + a.jchunk);
+    constexpr int CH = Op::kFlush < TILE ? Op::kFlush : TILE;
+
+    for (int jt = j0; jt < j1; jt += TILE) {
+        const int cnt = min(TILE, j1 - jt);
+        __syncthreads();
+        for (int k = tid; k < cnt; k += BLOCK) Op::load_j(a, fr, jt + k, tile[k]);
+        __syncthreads();
+        for (int k0 = 0; k0 < cnt; k0 += CH) {
+            const int kend = min(cnt, k0 + CH);
+            int k = k0;
+            for (; k + UNROLL <= kend; k += UNROLL) {
+#pragma unroll
+                for (int u = 0; u < UNROLL; ++u) {
+                    const Slot s = tile[k + u];
+#pragma unroll
+                    for (int w = 0; w < WPT; ++w) Op::interact(ir[w], s, acc[w]);
+                }
+            }
+            for (; k < kend; ++k) {
+                const Slot s = tile[k];
+#pragma unroll
+                for (int w = 0; w < WPT; ++w) Op::interact(ir[w], s, acc[w]);
+            }
+#pragma unroll
+            for (int w = 0; w < WPT; ++w) Op::flush(acc[w]);
+        }
+    }
+
+#pragma unroll
+    for (int w = 0; w < WPT; ++w) {
+        const int i = ibase + w * BLOCK + tid;
+        if (i >= a.ni) continue;
+        double tot[Op::kNacc];
+#pragma unroll
+        for (int k = 0; k < Op::kNacc; ++k) tot[k] = Op::get(acc[w], k);
+        if (a.nsplit == 1) {
+            Op::store(a, i, tot);
+        } else {
+            double* part = a.partial + static_cast<size_t>(g.jsplit) * Op::kNacc * a.ni;
+#pragma unroll
+            for (int k = 0; k < Op::kNacc; ++k) part[static_cast<size_t>(k) * a.ni + i] = tot[k];
+        }
+    }
+}
+
+// Combine the j-split partials of each i in split order and store.
+template <class Op>
+__global__ __launch_bounds__(256) void reduce_kernel(typename Op::Args a)
+{
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= a.ni) return;
+    double tot[Op::kNacc];
+    const size_t stride = static_cast<size_t>(Op::kNacc) * a.ni;
+#pragma unroll
+    for (int k = 0; k < Op::kNacc; ++k) tot[k] = a.partial[static_cast<size_t>(k) * a.ni + i];
+    for (int s = 1; s < a.nsplit; ++s) {
+        double part[Op::kNacc];
+#pragma unroll
+        for (int k = 0; k < Op::kNacc; ++k)
+            part[k] = a.partial[s * stride + static_cast<size_t>(k) * a.ni + i];
+        Op::combine(tot, part);
+    }
+    Op::store(a, i, tot);
+}
+
+// Launch geometry of one all-pairs call.
+struct Launch {
+    int nblock_i;  // i blocks
+    int nsplit;    // j splits
+    int jchunk;    // j per split (multiple of the tile)
+};
+
+// Pick the j-split so that the grid has at least `target_wg` workgroups
+// (several per CU) without cutting j ranges below one tile.
+inline Launch plan_launch(int ni, int nj, int block, int wpt, int tile, int target_wg,
+                          int max_split)
+{
+    Launch L;
+    const int per_block = block * wpt;
+    L.nblock_i = (ni + per_block - 1) / per_block;
+    int want = (target_wg + L.nblock_i - 1) / L.nblock_i;
+    int max_by_j = (nj + tile - 1) / tile;
+    int ns = want < max_by_j ? want : max_by_j;
+    if (ns > max_split) ns = max_split;
+    if (ns < 1) ns = 1;
+    // j range per split, rounded to whole tiles, then the split count that
+    // range actually needs
+    int chunk = (nj + ns - 1) / ns;
+    chunk = ((chunk + tile - 1) / tile) * tile;
+    if (chunk < tile) chunk = tile;
+    L.jchunk = chunk;
+    L.nsplit = nj > 0 ? (nj + chunk - 1) / chunk : 1;
+    return L;
+}
+
+}  // namespace tupan

@@ -1,0 +1,329 @@
+// Device universal-variable Kepler solver (FP64), following
+// tupan/lib/src/universal_kepler_solver.h:17-614 and the Sakura pair drift of
+// sakura_kernel_common.h:45-243.
+//
+// Differences from the reference, none of which changes a converged value:
+//   - The reference evaluates six Stumpff functions per Laguerre step, each
+//     with its own sqrt and cos/cosh or sin/sinh of the same argument
+//     (f, f', f'' at one s, :201-222).  Here the four c_k share one sqrt and
+//     one sincos (or sinh, cosh) per step; the values are the same numbers.
+//   - The two step-subdivision retry loops (:486-499, :580-606) are
+//     unbounded in the reference.  Here each is capped at kMaxDoublings
+//     halvings; a pair that reaches the cap keeps its last state and raises
+//     the status flag, which the host turns into an error.
+//   - Work counters (solves, Laguerre steps, subdivisions, leapfrogs).
+// The file is compiled with -ffp-contract=off so every expression keeps the
+// reference's operation order; transcendentals come from the device math
+// library and may differ from the host libm in the last bit.
+#pragma once
+
+#include "common.h"
+
+namespace tupan {
+namespace kep {
+
+constexpr double kTol = 2.2737367544323205948e-13;  // 2^-42
+constexpr int kMaxIter = 64;
+constexpr int kMaxDoublings = 20;
+// Solves one pair may spend across both retry loops; past it every further
+// solve reports failure at once, so the loops run out within a few hundred
+// trivial steps and the pair is flagged.
+constexpr unsigned int kSolveBudget = 1u << 16;
+constexpr double kPi = 3.141592653589793;
+
+struct Counters {
+    unsigned int budget = kSolveBudget;
+    unsigned int solves = 0;
+    unsigned int iters = 0;
+    unsigned int retries = 0;
+    unsigned int leapfrog = 0;
+    int fail = 0;
+};
+
+__device__ __forceinline__ int sgn(double x) { return (x > 0) - (x < 0); }
+
+// S_k(s, alpha) = s^k c_k(alpha s^2), k = 0..3 (:17-120).
+struct Stumpff {
+    double S0, S1, S2, S3;
+};
+
+__device__ __forceinline__ Stumpff stumpff(double s, double alpha)
+{
+    const double s2 = s * s;
+    const double s3 = s * s2;
+    const double z = alpha * s2;
+    double c0, c1, c2, c3;
+    if (z < 0) {
+        const double t = sqrt(-z);
+        double sn, cs;
+        sincos(t, &sn, &cs);
+        c0 = cs;
+        c1 = sn / t;
+        c2 = (cs - 1) / z;
+        c3 = (sn / t - 1) / z;
+    } else if (z > 0) {
+        const double t = sqrt(z);
+        const double ch = cosh(t);
+        const double sh = sinh(t);
+        c0 = ch;
+        c1 = sh / t;
+        c2 = (ch - 1) / z;
+        c3 = (sh / t - 1) / z;
+    } else {
+        c0 = 1.0;
+        c1 = 1.0;
+        c2 = 1 / 2.0;
+        c3 = 1 / 6.0;
+    }
+    Stumpff r;
+    r.S0 = c0;
+    r.S1 = s * c1;
+    r.S2 = s2 * c2;
+    r.S3 = s3 * c3;
+    return r;
+}
+
+// Laguerre iteration of order 5 on Kepler's equation in s (:225-254).
+// k = {dt, r0, r0.v0, m, alpha}.  Returns 0, -1 (h == 0), -2 (MAXITER),
+// -3 (sign of s changed).
+__device__ inline int laguerre5(double x0, double& x, const double* k, Counters& cnt)
+{
+    int it = 0;
+    double delta;
+    x = x0;
+    const double mar = k[3] + k[4] * k[1];
+    do {
+        const Stumpff S = stumpff(x, k[4]);
+        const double fv = (k[1] * x + k[2] * S.S2 + mar * S.S3) - k[0];
+        const double dfv = k[1] + k[2] * S.S1 + mar * S.S2;
+        const double ddfv = k[2] * S.S0 + mar * S.S1;
+        const double a = dfv;
+        const double a2 = a * a;
+        const double b = a2 - fv * ddfv;
+        const double g = 5 * fv;
+        const double h = a + sgn(a) * sqrt(fabs(4 * (5 * b - a2)));
+        cnt.iters++;
+        if (h == 0) return -1;
+        delta = -g / h;
+        x += delta;
+        it += 1;
+        if (it > kMaxIter) return -2;
+    } while (fabs(delta) > kTol);
+    if (sgn(x) != sgn(x0)) return -3;
+    return 0;
+}
+
+// Lagrange f, g and their derivatives applied to (p, v) (:312-357).
+__device__ inline void lagrange_update(double dt, double s, double r0, double m, double e2,
+                                       double alpha, double* p, double* v)
+{
+    const Stumpff S = stumpff(s, alpha);
+    const double x0 = p[0], y0 = p[1], z0 = p[2];
+    const double u0 = v[0], w0 = v[1], t0 = v[2];
+    const double lf = 1 - m * S.S2 / r0;
+    const double lg = dt - m * S.S3;
+    const double x1 = x0 * lf + u0 * lg;
+    const double y1 = y0 * lf + w0 * lg;
+    const double z1 = z0 * lf + t0 * lg;
+    const double r1sq = x1 * x1 + y1 * y1 + z1 * z1;
+    const double r1 = sqrt(r1sq + e2);
+    const double ldf = -m * S.S1 / (r0 * r1);
+    const double ldg = (1 + lg * ldf) / lf;
+    p[0] = x1;
+    p[1] = y1;
+    p[2] = z1;
+    v[0] = x0 * ldf + u0 * ldg;
+    v[1] = y0 * ldf + w0 * ldg;
+    v[2] = z0 * ldf + t0 * ldg;
+}
+
+// One solve over dt0 (:360-463).  p, v in/out.
+__device__ inline int kepler_once(double dt0, double m, double e2, double* p, double* v,
+                                  Counters& cnt)
+{
+    double r2 = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+    if (!(r2 > 0)) return 0;
+    if (cnt.budget == 0) return -4;
+    cnt.budget--;
+    cnt.solves++;
+    r2 += e2;
+    const double r = sqrt(r2);
+    const double v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    const double rv = p[0] * v[0] + p[1] * v[1] + p[2] * v[2];
+    const double beta = 2 - (e2 / r2);
+    const double alpha = v2 - beta * m / r;
+
+    double dt = dt0;
+    if (alpha < 0) {
+        const double a = m / fabs(alpha);
+        const double T = 2 * kPi * a * sqrt(a / m);
+        const double ratio = dt0 / T;
+        dt = (ratio - static_cast<double>(static_cast<long>(ratio))) * T;
+    }
+
+    double s0 = dt / r;
+    if (alpha > 0) {
+        // Bate, Mueller & White (1971) eq. 4.5.11 guess for hyperbolae
+        const double sal = sqrt(alpha);
+        const double ss = fabs(2 * alpha * dt / (rv + (m + alpha * r) / sal));
+        if (ss > 1) s0 = sgn(dt) * log(ss) / sal;
+    }
+
+    const double k[5] = {dt, r, rv, m, alpha};
+    double s;
+    const int err = laguerre5(s0, s, k, cnt);
+    if (err == 0) lagrange_update(dt, s, r, m, e2, alpha, p, v);
+    return err;
+}
+
+// Retry with n = 2, 4, ... equal substeps on any solver error (:466-507).
+__device__ inline void kepler_subdivided(double dt, double m, double e2, const double* p0,
+                                         const double* v0, double* p, double* v, Counters& cnt)
+{
+    for (int lvl = 0; lvl <= kMaxDoublings; ++lvl) {
+        const long n = 1L << lvl;
+        bool ok = true;
+        p[0] = p0[0]; p[1] = p0[1]; p[2] = p0[2];
+        v[0] = v0[0]; v[1] = v0[1]; v[2] = v0[2];
+        for (long i = 0; i < n; ++i) {
+            if (kepler_once(dt / n, m, e2, p, v, cnt) != 0) {
+                ok = false;
+                break;
+            }
+        }
+        if (ok) return;
+        cnt.retries++;
+        if (cnt.budget == 0) break;
+    }
+    cnt.fail = 1;
+}
+
+// Full solver with the softened-energy check and step doubling (:510-614).
+__device__ inline void kepler_solve(double dt, double m, double e2, const double* p0,
+                                    const double* v0, double* p1, double* v1, Counters& cnt)
+{
+    cnt.budget = kSolveBudget;
+    double p[3], v[3];
+    kepler_subdivided(dt, m, e2, p0, v0, p, v, cnt);
+    if (e2 != 0) {
+        double r2 = p0[0] * p0[0] + p0[1] * p0[1] + p0[2] * p0[2];
+        if (!(r2 > 0)) {
+            p[0] = p0[0]; p[1] = p0[1]; p[2] = p0[2];
+            v[0] = v0[0]; v[1] = v0[1]; v[2] = v0[2];
+        } else {
+            r2 += e2;
+            double r = sqrt(r2);
+            double v2 = v0[0] * v0[0] + v0[1] * v0[1] + v0[2] * v0[2];
+            const double u0 = 2 * m / r;
+            const double en0 = v2 - u0;
+
+            r2 = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+            r2 += e2;
+            r = sqrt(r2);
+            v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+            double u1 = 2 * m / r;
+            double en1 = v2 - u1;
+
+            const double tol = 64 * kTol;
+            double err = fabs(en1 - en0);
+            if (!(2 * err < tol * (u1 + u0))) {
+                long n = 1;
+                int lvl;
+                for (lvl = 1; lvl <= kMaxDoublings; ++lvl) {
+                    n *= 2;
+                    bool ok = true;
+                    p[0] = p0[0]; p[1] = p0[1]; p[2] = p0[2];
+                    v[0] = v0[0]; v[1] = v0[1]; v[2] = v0[2];
+                    for (long i = 0; i < n; ++i) {
+                        const double q[3] = {p[0], p[1], p[2]};
+                        const double w[3] = {v[0], v[1], v[2]};
+                        kepler_subdivided(dt / n, m, e2, q, w, p, v, cnt);
+                        r2 = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+                        r2 += e2;
+                        r = sqrt(r2);
+                        v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+                        u1 = 2 * m / r;
+                        en1 = v2 - u1;
+                        err = fabs(en1 - en0);
+                        if (2 * err > tol * (u1 + u0)) {
+                            ok = false;
+                            break;
+                        }
+                    }
+                    if (ok) break;
+                    cnt.retries++;
+                }
+                if (lvl > kMaxDoublings) cnt.fail = 1;
+                if (cnt.budget == 0) cnt.fail = 1;
+            }
+        }
+    }
+    p1[0] = p[0]; p1[1] = p[1]; p1[2] = p[2];
+    v1[0] = v[0]; v1[1] = v[1]; v1[2] = v[2];
+}
+
+// Leapfrog DKD with one softened kick, for far pairs (:45-91).
+__device__ inline void leapfrog(double dt, double m, double e2, double* p, double* v,
+                                Counters& cnt)
+{
+    const double h = dt / 2;
+    double x = p[0], y = p[1], z = p[2];
+    double u = v[0], w = v[1], t = v[2];
+    x += u * h;
+    y += w * h;
+    z += t * h;
+    const double r2 = x * x + y * y + z * z;
+    double q2 = 1 / (r2 + e2);
+    q2 = r2 > 0 ? q2 : 0.0;
+    const double q3 = q2 * sqrt(q2);
+    const double mq = m * q3;
+    u += (-mq * x) * dt;
+    w += (-mq * y) * dt;
+    t += (-mq * z) * dt;
+    x += u * h;
+    y += w * h;
+    z += t * h;
+    p[0] = x; p[1] = y; p[2] = z;
+    v[0] = u; v[1] = w; v[2] = t;
+    cnt.leapfrog++;
+}
+
+// Far pairs (r > 64 m / v^2) take one leapfrog step (:94-123).
+__device__ inline void twobody(double dt, double m, double e2, double* p, double* v,
+                               Counters& cnt)
+{
+    const double r2 = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+    const double v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    const double R = 64 * (m / v2);
+    if (r2 > R * R) {
+        leapfrog(dt, m, e2, p, v, cnt);
+    } else {
+        const double p0[3] = {p[0], p[1], p[2]};
+        const double v0[3] = {v[0], v[1], v[2]};
+        kepler_solve(dt, m, e2, p0, v0, p, v, cnt);
+    }
+}
+
+// evolve_twobody flags (:126-191); any other flag leaves the pair unchanged.
+__device__ inline void drift_twobody(double dt, long flag, double m, double e2, double* p,
+                                     double* v, Counters& cnt)
+{
+    if (flag == -1) {
+        p[0] -= v[0] * dt; p[1] -= v[1] * dt; p[2] -= v[2] * dt;
+        twobody(dt, m, e2, p, v, cnt);
+    } else if (flag == 1) {
+        twobody(dt, m, e2, p, v, cnt);
+        p[0] -= v[0] * dt; p[1] -= v[1] * dt; p[2] -= v[2] * dt;
+    } else if (flag == -2) {
+        p[0] -= v[0] * dt / 2; p[1] -= v[1] * dt / 2; p[2] -= v[2] * dt / 2;
+        twobody(dt, m, e2, p, v, cnt);
+        p[0] -= v[0] * dt / 2; p[1] -= v[1] * dt / 2; p[2] -= v[2] * dt / 2;
+    } else if (flag == 2) {
+        twobody(dt / 2, m, e2, p, v, cnt);
+        p[0] -= v[0] * dt; p[1] -= v[1] * dt; p[2] -= v[2] * dt;
+        twobody(dt / 2, m, e2, p, v, cnt);
+    }
+}
+
+}  // namespace kep
+}  // namespace tupan

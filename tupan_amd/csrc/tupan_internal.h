@@ -1,0 +1,84 @@
+// Internal declarations shared by the translation units of libtupan_hip.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+#include "../../include/tupan_hip.h"
+
+namespace tupan {
+
+enum KernelId {
+    K_PHI = TUPAN_K_PHI,
+    K_ACC = TUPAN_K_ACC,
+    K_ACC_JERK = TUPAN_K_ACC_JERK,
+    K_SNAP_CRACKLE = TUPAN_K_SNAP_CRACKLE,
+    K_TSTEP = TUPAN_K_TSTEP,
+};
+
+// Number of i/j input fields and of outputs of each pairwise kernel, in the
+// reference's argument order (tupan/lib/extensions.py:100-387).
+inline int kernel_in_fields(int k)
+{
+    switch (k) {
+    case K_PHI: return 5;
+    case K_ACC: return 5;
+    case K_ACC_JERK: return 8;
+    case K_SNAP_CRACKLE: return 14;
+    case K_TSTEP: return 8;
+    default: return -1;
+    }
+}
+
+inline int kernel_out_fields(int k)
+{
+    switch (k) {
+    case K_PHI: return 1;
+    case K_ACC: return 3;
+    case K_ACC_JERK: return 6;
+    case K_SNAP_CRACKLE: return 6;
+    case K_TSTEP: return 2;
+    default: return -1;
+    }
+}
+
+// Per-device state.
+struct Device {
+    int id = -1;
+    int cu_count = 0;
+    int target_wg = 0;  // workgroups the launch planner aims for
+    void* ws = nullptr;  // scratch workspace (j-split partials, staging)
+    size_t ws_bytes = 0;
+};
+
+// State of the current device (initialised on first use).
+const Device& current_device();
+
+// Scratch workspace of at least `bytes` on the current device.  Growing it
+// synchronises the device before the old buffer is released.  Returns null
+// on allocation failure.
+void* workspace(size_t bytes, hipStream_t stream);
+
+// Error state (thread-local message).  set_error returns `code`.
+int set_error(int code, const char* msg);
+int check_hip(hipError_t e, const char* what);
+int check_launch(const char* what);
+
+// Optional HIP-event timing of the main (all-pairs) kernel of each call.
+// timing_begin/timing_end bracket that launch on its stream when enabled.
+void timing_begin(hipStream_t stream);
+void timing_end(hipStream_t stream);
+
+template <class T>
+int pairwise_call(int kernel, int mode, size_t ni, const T* const* ip, size_t nj,
+                  const T* const* jp, T* const* out, T eta, void* stream);
+
+int sakura_call(size_t ni, const double* const* ip, size_t nj, const double* const* jp,
+                double dt, long flag, double* const* out, unsigned long long* counters,
+                int* status, void* stream);
+
+int kepler_pairs_call(size_t npairs, const int* idx, const double* m, double* rx, double* ry,
+                      double* rz, const double* e2, double* vx, double* vy, double* vz,
+                      double dt, int* status, void* stream);
+
+}  // namespace tupan
