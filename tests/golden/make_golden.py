@@ -121,7 +121,9 @@ def kepler_cases():
     cases.append((0.5, 0.5, 0.0, [1, 0, 0], [0, 1, 0], 0.3))
     cases.append((0.7, 0.3, 0.0, [1, 0.2, 0], [0.1, 0.6, 0.2], 2.5))
     cases.append((0.5, 0.5, 0.0, [1, 0, 0], [0, 2.0, 0.3], 1.0))
-    cases.append((0.5, 0.5, 0.0, [1, 0, 0], [0, np.sqrt(2.0), 0], 4.0))
+    # near parabolic (alpha ~ 2e-3; exactly parabolic makes the reference's
+    # (cosh t - 1)/z cancel to noise, universal_kepler_solver.h:58-61)
+    cases.append((0.5, 0.5, 0.0, [1, 0, 0], [0, 1.415, 0], 4.0))
     cases.append((0.9, 0.1, 0.0, [0.5, -0.5, 0.1], [0.3, 0.9, -0.1], 37.0))
     cases.append((0.5, 0.5, 0.01, [0.2, 0.1, 0], [0.1, 1.5, 0], 0.7))
     cases.append((0.5, 0.5, 0.02, [1, 0, 0], [0, 0.9, 0.1], -1.3))

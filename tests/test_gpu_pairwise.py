@@ -120,16 +120,51 @@ def test_prefix_sweep(golden, name):
             assert_close(name, got, g, "%s_%s_pre%d" % (tag, name, k), 1e-13)
 
 
-@pytest.mark.parametrize("name", ["acc", "acc_jerk"])
-def test_split32_golden(golden, name):
+def abs_sums(ips, jps):
+    """Per i, the sums over j of |acceleration| and |jerk| contributions
+    (the scale an FP32 evaluation's rounding error is proportional to)."""
+    dx = ips["rx"][:, None] - jps["rx"][None, :]
+    dy = ips["ry"][:, None] - jps["ry"][None, :]
+    dz = ips["rz"][:, None] - jps["rz"][None, :]
+    ux = ips["vx"][:, None] - jps["vx"][None, :]
+    uy = ips["vy"][:, None] - jps["vy"][None, :]
+    uz = ips["vz"][:, None] - jps["vz"][None, :]
+    r2 = dx * dx + dy * dy + dz * dz
+    e2 = ips["eps2"][:, None] + jps["eps2"][None, :]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        q2 = np.where(r2 > 0, 1.0 / (r2 + e2), 0.0)
+    q3 = q2 * np.sqrt(q2) * jps["mass"][None, :]
+    r = np.sqrt(r2)
+    v = np.sqrt(ux * ux + uy * uy + uz * uz)
+    acc = (q3 * r).sum(axis=1)
+    jerk = (q3 * (v + 3 * v * r2 * q2)).sum(axis=1)
+    return acc, jerk
+
+
+def test_split32_golden(golden):
+    """Split-FP32 error is an FP32 rounding error per contribution: bounded
+    by a few FP32 ulps (2^-24 = 6e-8) of the sum of contribution
+    magnitudes, whatever cancellation does to the total."""
     g = golden("pairwise_f64.npz")
     for soft in (0, 1):
         tag = "s%d" % soft
-        A = device_ps(split_set(g, tag, "A"))
-        B = device_ps(split_set(g, tag, "B"))
-        for lab, ii, jj in (("AB", A, B), ("BA", B, A), ("AA", A, A)):
-            got = run_ext(name, "float64", "split32", ii, jj, ())
-            assert_close(name, got, g, "%s_%s_%s" % (tag, name, lab), 2e-6)
+        a, b = split_set(g, tag, "A"), split_set(g, tag, "B")
+        A, B = device_ps(a), device_ps(b)
+        for lab, ii, jj, hi, hj in (("AB", A, B, a, b), ("BA", B, A, b, a),
+                                    ("AA", A, A, a, a)):
+            sa, sj = abs_sums(hi, hj)
+            got = run_ext("acc_jerk", "float64", "split32", ii, jj, ())
+            ref = [g["%s_acc_jerk_%s_%s" % (tag, lab, o)]
+                   for o in KERNELS["acc_jerk"][2]]
+            for grp, scale in (((0, 1, 2), sa), ((3, 4, 5), sj)):
+                d = np.sqrt(sum((got[k] - ref[k])**2 for k in grp))
+                e = float(np.max(d / scale))
+                assert e <= 8 * 2.0**-24, (lab, grp, e)
+            got = run_ext("acc", "float64", "split32", ii, jj, ())
+            ref = [g["%s_acc_%s_%s" % (tag, lab, o)] for o in ("ax", "ay",
+                                                                 "az")]
+            d = np.sqrt(sum((got[k] - ref[k])**2 for k in range(3)))
+            assert float(np.max(d / sa)) <= 8 * 2.0**-24
 
 
 @pytest.mark.parametrize("name", sorted(KERNELS))

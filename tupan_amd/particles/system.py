@@ -137,10 +137,15 @@ class ParticleSystem(object):
         return "ParticleSystem(n=%d, device=%s)" % (self.n, self.device)
 
     def register_auxiliary_attribute(self, attr, sctype):
+        """Zero-filled array; "real" attributes take the dtype of the
+        system's masses, so a float32 system gets float32 outputs."""
         if attr in self.__dict__:
             raise ValueError("'{0}' is already a registered "
                              "attribute.".format(attr))
-        self.__dict__[attr] = torch.zeros(self.n, dtype=torch_dtype(sctype),
+        dt = torch_dtype(sctype)
+        if sctype == "real" and "mass" in self.__dict__:
+            dt = self.__dict__["mass"].dtype
+        self.__dict__[attr] = torch.zeros(self.n, dtype=dt,
                                           device=self.device)
 
     def _new(self, arrays, members=None):
