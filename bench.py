@@ -34,6 +34,13 @@ FLOPS_PER_PAIR = 42          # acc_jerk_kernel_common.h:56
 PEAK_FP64 = 78.6e12          # MI355X vector FP64
 PEAK_FP32 = 157.3e12         # MI355X vector FP32 (counts packed FP32)
 HBM_PEAK = 8.0e12
+# HBM bytes per launch of the all-pairs kernel at N = 65536, from the two
+# calibrated rocprofv3 counter runs in profiles/accjerk_plummer65536_pmc_
+# traffic.md (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, KiB -> bytes).
+TRAFFIC_65536 = {
+    "split32": (14870.6 * 2.0 + 24576.0) * 1024,
+    "fp64": (14634.5 * 2.0 + 24576.0) * 1024,
+}
 
 
 def parse():
@@ -47,6 +54,11 @@ def parse():
     ap.add_argument("--cpu-rows", type=int, default=0,
                     help="i-rows of the CPU baseline sample (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="run one elementwise copy of this many MiB first "
+                         "(counter calibration under rocprofv3 --pmc)")
+    ap.add_argument("--modes", default="split32,fp64,ieee64",
+                    help="force modes to time (the first is the headline)")
     return ap.parse_args()
 
 
@@ -174,16 +186,29 @@ def main():
     assert torch.cuda.is_available(), "no HIP device"
     cu, wg = _native.device_info()
     n = args.n
+    if args.copy_probe_mb > 0:
+        nel = args.copy_probe_mb * (1 << 20) // 8
+        a = torch.ones(nel, dtype=torch.float64, device="cuda:0")
+        b = torch.empty_like(a)
+        torch.mul(a, 1.0, out=b)
+        torch.cuda.synchronize()
+        log("bench: copy probe %d MiB read + %d MiB written"
+            % (args.copy_probe_mb, args.copy_probe_mb))
+        del a, b
     log("bench: building Plummer N=%d eps=1e-4 seed=1" % n)
     ps = make_plummer(n, 1e-4, ("equalmass",), seed=1, device="cuda:0")
     pairs = float(n) * n
 
-    log("bench: split-FP32 AccJerk, %d steps" % args.steps)
-    t_split, k_split = time_accjerk(ps, "split32", args.steps, args.warmup)
-    log("bench: FP64 AccJerk, %d steps" % args.steps)
-    t_fp64, k_fp64 = time_accjerk(ps, "fp64", args.steps, args.warmup)
-    log("bench: FP64 (reference operation order) AccJerk")
-    t_ieee, k_ieee = time_accjerk(ps, "ieee64", max(2, args.steps // 4), 1)
+    modes = args.modes.split(",")
+    res = {}
+    for mode in modes:
+        k = args.steps if mode != "ieee64" else max(2, args.steps // 4)
+        log("bench: %s AccJerk, %d steps" % (mode, k))
+        res[mode] = time_accjerk(ps, mode, k, args.warmup)
+    nan = (float("nan"), float("nan"))
+    t_split, k_split = res.get("split32", nan)
+    t_fp64, k_fp64 = res.get("fp64", nan)
+    t_ieee, k_ieee = res.get("ieee64", nan)
 
     out = {
         "metric": "Gint/s AccJrk N=64k split-FP32 vs FP64 at |dE/E|<=1e-10; "
@@ -219,7 +244,9 @@ def main():
         "peak": PEAK_FP32 / 1e12,
         "unit": "TFLOP/s",
         "frac": flops / k_split / PEAK_FP32,
-        "traffic": None,
+        "traffic": TRAFFIC_65536["split32"] if n == 65536 else None,
+        "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE, WRITE_SIZE)",
+        "algorithmic_bytes": (8 + 6) * 8 * n,
         "fp64_kernel": {"kernel": "pair_kernel<OpAccJerk<Fast64>>",
                         "achieved": flops / k_fp64 / 1e12,
                         "peak": PEAK_FP64 / 1e12,

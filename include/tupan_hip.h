@@ -262,6 +262,38 @@ int tupan_kepler_pairs(size_t npairs, const int *idx,
                        double dt, int *status, void *stream);
 
 /* ---------------------------------------------------------------------------
+ * Ensembles of independent few-body systems (no counterpart in the reference,
+ * which integrates one system per process; the per-pair arithmetic is
+ * sakura_kernel.c:5-63 and phi_kernel.c:5-34 restricted to pairs inside a
+ * system).  The particles of all systems are concatenated in one SoA set;
+ * system s is [offsets[s], offsets[s+1]), each of at most 64 particles.
+ * A plan (created once from HOST offsets) packs systems of similar size into
+ * wavefronts, min_lanes (8, 16, 32 or 64) lanes per system at least; every
+ * call on the plan is one kernel launch over all systems.
+ * ------------------------------------------------------------------------- */
+typedef struct tupan_ensemble tupan_ensemble;
+int tupan_ensemble_create(int nsys, const int *offsets, int min_lanes,
+                          tupan_ensemble **plan);
+int tupan_ensemble_destroy(tupan_ensemble *plan);
+int tupan_ensemble_info(const tupan_ensemble *plan, int *nsys, int *ntot,
+                        int *nwaves);
+/* Sakura pair drift inside every system (flag and counters/status as for
+ * tupan_sakura). */
+int tupan_ensemble_sakura(const tupan_ensemble *plan, const double *m,
+                          const double *rx, const double *ry, const double *rz,
+                          const double *e2, const double *vx, const double *vy,
+                          const double *vz, double dt, long flag,
+                          double *drx, double *dry, double *drz,
+                          double *dvx, double *dvy, double *dvz,
+                          unsigned long long *counters, int *status,
+                          void *stream);
+/* Potential of every particle due to its own system (reference operation
+ * order, IEEE division and square root). */
+int tupan_ensemble_phi(const tupan_ensemble *plan, const double *m,
+                       const double *rx, const double *ry, const double *rz,
+                       const double *e2, double *phi, void *stream);
+
+/* ---------------------------------------------------------------------------
  * Host layer of the float build.  libtupan_hip_f32.so implements the
  * reference's float ABI (phi_kernel, ... with unsigned int / int / float)
  * on top of these; ifields/jfields/out as for tupan_pairwise.  Sakura and

@@ -1,0 +1,223 @@
+"""Host logic of the drivers on the CPU: block time steps, Hermite order of
+convergence, SIA hierarchy, Sakura, the Plummer generator and the extension
+layer's call pattern.  Forces come from the test-only oracle backend
+(tests/oracle_backend.py) on CPU-resident particle systems."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import Oracle
+from oracle_backend import OracleKernel, install_hip, install_oracle
+
+
+@pytest.fixture(autouse=True)
+def oracle_forces():
+    install_oracle("float64")
+    yield
+    install_hip("float64")
+
+
+def make_ps(d):
+    from tupan_amd.particles import ParticleSystem
+    return ParticleSystem.from_numpy(d, device="cpu")
+
+
+def kepler_pair(e=0.0):
+    """Two equal masses (total 1) on a circular (e=0) or eccentric orbit,
+    centre of mass at rest at the origin."""
+    r = 1.0
+    v = math.sqrt((1 + e) / r)
+    d = {"mass": np.array([0.5, 0.5]), "eps2": np.zeros(2),
+         "rx": np.array([0.5, -0.5]), "ry": np.zeros(2), "rz": np.zeros(2),
+         "vx": np.zeros(2), "vy": np.array([v / 2, -v / 2]),
+         "vz": np.zeros(2)}
+    return d
+
+
+def test_min_block_tstep():
+    from tupan_amd.integrator.hermite import Hermite
+    ps = make_ps(kepler_pair())
+    it = Hermite(0.01, 0.0, ps, "ahermite4")
+    ps.tstep[...] = 0.01
+    type(ps).t_curr = 0.0
+    assert it.get_min_block_tstep(ps, 1.0) == 2.0**-7
+    type(ps).t_curr = 2.0**-8
+    assert it.get_min_block_tstep(ps, 1.0) == 2.0**-8
+    type(ps).t_curr = 0.0
+    assert it.get_min_block_tstep(ps, 2.0**-9) == 2.0**-9
+    assert it.get_min_block_tstep(ps, -1.0) == -(2.0**-7)
+    type(ps).t_curr = 0.0
+
+
+def test_base_tstep():
+    from tupan_amd.integrator.hermite import Hermite
+    ps = make_ps(kepler_pair())
+    it = Hermite(0.125, 0.0, ps, "hermite4")
+    assert it.get_base_tstep(1.0) == 0.125
+    type(ps).t_curr = 0.95
+    assert abs(it.get_base_tstep(1.0) - 0.05) < 1e-15
+    type(ps).t_curr = 0.0
+
+
+def run_fixed(method, dt, t_end, d):
+    from tupan_amd.integrator import Integrator
+    ps = make_ps(d)
+    it = Integrator(dt, 0.0, ps, method=method)
+    while it.time < t_end - 1e-12:
+        it.evolve_step(t_end)
+    return it.particle_system
+
+
+@pytest.mark.parametrize("method,order", [("hermite2", 2), ("hermite4", 4),
+                                          ("hermite6", 6), ("hermite8", 8)])
+def test_hermite_order_of_convergence(method, order):
+    """Position error after t = 1 on an eccentric two-body orbit against
+    the exact Kepler solution: halving the step divides it by ~2^order."""
+    d = kepler_pair(0.3)
+    orc = Oracle("float64")
+    exact = orc.kepler(d, 1.0)
+    errs = []
+    steps = (1 / 16, 1 / 32) if order < 6 else (1 / 8, 1 / 16)
+    for dt in steps:
+        ps = run_fixed(method, dt, 1.0, d)
+        err = abs(float(ps.rx[0]) - exact[0][0]) + \
+            abs(float(ps.ry[0]) - exact[1][0])
+        errs.append(err)
+    ratio = errs[0] / errs[1]
+    assert ratio > 2**order * 0.6, (errs, ratio)
+
+
+def test_sakura_two_body_is_exact():
+    """Sakura advances each pair exactly: a lone pair follows the Kepler
+    orbit to solver precision whatever the step."""
+    d = kepler_pair(0.5)
+    ps = run_fixed("sakura", 0.25, 1.0, d)
+    exact = Oracle("float64").kepler(d, 1.0)
+    assert abs(float(ps.rx[0]) - exact[0][0]) < 1e-11
+    assert abs(float(ps.vy[1]) - exact[4][1]) < 1e-11
+
+
+def test_sia_leaf_is_kepler_and_hierarchy_aligns():
+    """sia21h: a pair ends in the FewBody Kepler leaf (exact), and a larger
+    system ends every particle at the common end time."""
+    d = kepler_pair(0.2)
+    ps = run_fixed("sia21h.dkd", 0.125, 0.5, d)
+    exact = Oracle("float64").kepler(d, 0.5)
+    assert abs(float(ps.rx[0]) - exact[0][0]) < 1e-11
+    from tupan_amd.ics.plummer import make_plummer
+    p = make_plummer(48, 0.05, ("equalmass",), seed=3, device="cpu")
+    from tupan_amd.integrator import Integrator
+    type(p).t_curr = 0.0
+    it = Integrator(0.05, 0.0, p, method="sia21h.dkd")
+    e0 = p.kinetic_energy + p.potential_energy
+    it.evolve_step(0.25)
+    q = it.particle_system
+    steps = q.tstep.numpy()
+    # every level's step is the base step (eta) over a power of two
+    lg = np.log2(np.abs(steps) / 0.05)
+    assert np.allclose(lg, np.round(lg))
+    assert q.n == 48 and sorted(q.id.tolist()) == list(range(48))
+    e1 = q.kinetic_energy + q.potential_energy
+    assert abs((e1 - e0) / e0) < 1e-3
+
+
+def test_plummer_invariants_and_rng_sequence():
+    from tupan_amd.ics.plummer import make_plummer
+    n = 64
+    eps = 0.01
+    ps = make_plummer(n, eps, ("equalmass",), seed=1, device="cpu")
+    assert abs(ps.total_mass - 1.0) < 1e-14
+    np.testing.assert_allclose(ps.com_r, 0, atol=1e-15)
+    np.testing.assert_allclose(ps.com_v, 0, atol=1e-15)
+    ke, pe = ps.kinetic_energy, ps.potential_energy
+    assert abs(2 * ke / -pe - 1) < 1e-12
+    # eps2 is not rescaled with the positions (plummer.py:131,
+    # body.py:449-461), so r_vir = 1 only up to the softening's effect
+    assert abs(ps.virial_radius - 1) < 1e-3
+    np.testing.assert_array_equal(ps.eps2.numpy(), np.full(n, eps**2 / 2))
+    np.testing.assert_array_equal(ps.mass.numpy(), np.full(n, 1.0 / n))
+    hard = make_plummer(n, 0.0, ("equalmass",), seed=1, device="cpu")
+    assert abs(hard.virial_radius - 1) < 1e-12
+
+
+def test_von_neumann_batches_equal_scalar_loop():
+    """The batched rejection sampler returns the reference's scalar loop
+    values (plummer.py:77-86) and leaves the generator where that loop
+    does."""
+    from tupan_amd.ics.plummer import _von_neumann
+    for seed, n in ((1, 5), (2, 100), (3, 1000)):
+        a = np.random.RandomState(seed)
+        b = np.random.RandomState(seed)
+        got = _von_neumann(a, n)
+        ref = []
+        while len(ref) < n:
+            r1 = b.random_sample()
+            r2 = b.random_sample()
+            if r2 < r1:
+                ref.append(r2)
+        np.testing.assert_array_equal(got, ref)
+        assert a.random_sample() == b.random_sample()
+
+
+def test_plummer_positions_follow_reference_sequence():
+    """Positions are the reference's draws after the state restore: a
+    permutation, random(n), uniform(-1,1,n), uniform(0,2pi,n), scaled by the
+    virial rescaling (plummer.py:65-75)."""
+    from tupan_amd.ics.plummer import make_plummer
+    n = 32
+    ps = make_plummer(n, 0.0, ("equalmass",), seed=5, device="cpu")
+    rs = np.random.RandomState(5)
+    irand = rs.permutation(np.arange(n))
+    mrand = (irand + rs.random_sample(n)) * 0.999 / n
+    radius = 1.0 / np.sqrt(np.power(mrand, -2.0 / 3.0) - 1.0)
+    theta = np.arccos(rs.uniform(-1.0, 1.0, size=n))
+    phi = rs.uniform(0.0, 2.0 * np.pi, size=n)
+    rz = radius * np.cos(theta)
+    rx = radius * np.sin(theta) * np.cos(phi)
+    # the model is shifted to its centre of mass and scaled by one factor
+    got = ps.rz.numpy()
+    gx = ps.rx.numpy()
+    ref = rz - rz.mean()
+    refx = rx - rx.mean()
+    s = np.dot(got, ref) / np.dot(ref, ref)
+    np.testing.assert_allclose(got, s * ref, rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(gx, s * refx, rtol=1e-10, atol=1e-12)
+
+
+def test_extension_call_pattern_hermite4():
+    """ahermite4 does 1 Tstep and 3 AccJerk all-pairs calls per step
+    (hermite.py:80,97,347,396)."""
+    from tupan_amd.ics.plummer import make_plummer
+    from tupan_amd.integrator import Integrator
+    p = make_plummer(16, 0.01, ("equalmass",), seed=2, device="cpu")
+    type(p).t_curr = 0.0
+    it = Integrator(0.01, 0.0, p, method="ahermite4")
+    it.evolve_step(1.0)
+    OracleKernel.calls.clear()
+    it.evolve_step(1.0)
+    names = [c[0] for c in OracleKernel.calls]
+    assert names.count("acc_jerk") == 3 and names.count("tstep") == 1
+
+
+def test_particle_system_structure():
+    from tupan_amd.particles import ParticleSystem
+    ps = ParticleSystem(3, 2, device="cpu")
+    assert ps.n == 5 and set(ps.members) == {"bodies", "stars"}
+    ps.rx[...] = torch.arange(5, dtype=torch.float64)
+    assert ps.stars.rx.tolist() == [3.0, 4.0]
+    ps.stars.rx[...] = 9.0
+    assert ps.rx.tolist() == [0, 1, 2, 9, 9]
+    sub = ps[torch.tensor([True, False, True, False, True])]
+    assert sub.n == 3 and sub.rx.tolist() == [0, 2, 9]
+    sub.rx[...] = -1
+    assert ps.rx.tolist() == [0, 1, 2, 9, 9]
+    c = ps.copy()
+    c.append(sub)
+    assert c.n == 8 and c.rx.tolist()[-3:] == [-1, -1, -1]
+    ps.register_auxiliary_attribute("phi", "real")
+    with pytest.raises(ValueError):
+        ps.register_auxiliary_attribute("phi", "real")
+    st = ps.get_state()
+    assert st.dtype.names[:3] == ("id", "mass", "eps2")

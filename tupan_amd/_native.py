@@ -52,6 +52,19 @@ def _declare(lib):
     lib.tupan_sakura.argtypes = ([_S] + [_P] * 8 + [_S] + [_P] * 8
                                  + [ctypes.c_double, ctypes.c_long]
                                  + [_P] * 6 + [_P, _P, _P])
+    lib.tupan_ensemble_create.restype = ctypes.c_int
+    lib.tupan_ensemble_create.argtypes = [ctypes.c_int, _P, ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_void_p)]
+    lib.tupan_ensemble_destroy.restype = ctypes.c_int
+    lib.tupan_ensemble_destroy.argtypes = [_P]
+    lib.tupan_ensemble_info.restype = ctypes.c_int
+    lib.tupan_ensemble_info.argtypes = [_P] + [ctypes.POINTER(ctypes.c_int)] * 3
+    lib.tupan_ensemble_sakura.restype = ctypes.c_int
+    lib.tupan_ensemble_sakura.argtypes = ([_P] + [_P] * 8
+                                          + [ctypes.c_double, ctypes.c_long]
+                                          + [_P] * 6 + [_P, _P, _P])
+    lib.tupan_ensemble_phi.restype = ctypes.c_int
+    lib.tupan_ensemble_phi.argtypes = [_P] + [_P] * 5 + [_P, _P]
     lib.tupan_kepler_pairs.restype = ctypes.c_int
     lib.tupan_kepler_pairs.argtypes = ([_S, _P] + [_P] * 8
                                        + [ctypes.c_double, _P, _P])
@@ -116,4 +129,7 @@ DEVICE_ABI = ("tupan_last_error", "tupan_version", "tupan_device_info",
               "tupan_phi", "tupan_acc", "tupan_acc_jerk", "tupan_snap_crackle",
               "tupan_tstep", "tupan_pairwise", "tupan_pairwise_f32",
               "tupan_sakura", "tupan_kepler_pairs", "tupan_host_pairwise_f32",
-              "tupan_host_sakura_f32", "tupan_host_kepler_f32")
+              "tupan_host_sakura_f32", "tupan_host_kepler_f32",
+              "tupan_ensemble_create", "tupan_ensemble_destroy",
+              "tupan_ensemble_info", "tupan_ensemble_sakura",
+              "tupan_ensemble_phi")

@@ -575,4 +575,41 @@ int tupan_kepler_pairs(size_t npairs, const int* idx, const double* m, double* r
     return kepler_pairs_call(npairs, idx, m, rx, ry, rz, e2, vx, vy, vz, dt, status, stream);
 }
 
+// ---- ensembles of independent systems --------------------------------------
+
+int tupan_ensemble_create(int nsys, const int* offsets, int min_lanes, tupan_ensemble** plan)
+{
+    void* p = nullptr;
+    int rc = ensemble_plan_create(nsys, offsets, min_lanes, &p);
+    if (plan) *plan = static_cast<tupan_ensemble*>(p);
+    return rc;
+}
+
+int tupan_ensemble_destroy(tupan_ensemble* plan) { return ensemble_plan_destroy(plan); }
+
+int tupan_ensemble_info(const tupan_ensemble* plan, int* nsys, int* ntot, int* nwaves)
+{
+    return ensemble_plan_info(plan, nsys, ntot, nwaves);
+}
+
+int tupan_ensemble_sakura(const tupan_ensemble* plan, const double* m, const double* rx,
+                          const double* ry, const double* rz, const double* e2,
+                          const double* vx, const double* vy, const double* vz, double dt,
+                          long flag, double* drx, double* dry, double* drz, double* dvx,
+                          double* dvy, double* dvz, unsigned long long* counters, int* status,
+                          void* stream)
+{
+    const double* in[8] = {m, rx, ry, rz, e2, vx, vy, vz};
+    double* out[6] = {drx, dry, drz, dvx, dvy, dvz};
+    return ensemble_sakura_call(plan, in, dt, flag, out, counters, status, stream);
+}
+
+int tupan_ensemble_phi(const tupan_ensemble* plan, const double* m, const double* rx,
+                       const double* ry, const double* rz, const double* e2, double* phi,
+                       void* stream)
+{
+    const double* in[8] = {m, rx, ry, rz, e2, rx, ry, rz};
+    return ensemble_phi_call(plan, in, phi, stream);
+}
+
 }  // extern "C"

@@ -1,0 +1,306 @@
+// Ensembles of independent few-body systems in one launch: the Sakura pair
+// drift (sakura_kernel.c:5-63) and the potential Phi (phi_kernel.c:5-34)
+// evaluated inside each system only.
+//
+// Layout: the particles of all systems are concatenated in one SoA set;
+// offsets[s] .. offsets[s+1] is system s.  A plan packs systems into waves:
+// systems are sorted by size and each size bucket gets L lanes per system
+// (L in {8, 16, 32, 64}, at least the system size), so a wave holds 64/L
+// systems of similar size.  Lane l of a wave serves particle l % L of
+// system l / L and loops over that system's j particles, so the cost of
+// Kepler solves of uneven iteration counts is summed per lane over j rather
+// than paid at the slowest pair of a wave.  Every lane stays in the wave to
+// the end (inactive lanes compute nothing), so the wave reductions of the
+// work counters see all 64 lanes.
+//
+// Compiled with -ffp-contract=off (reference operation order).
+#include <algorithm>
+#include <vector>
+
+#include "kepler.h"
+#include "tupan_internal.h"
+
+namespace tupan {
+
+struct EnsWave {
+    int first;  // index into order[]
+    int count;  // systems in this wave
+    int lanes;  // lanes per system (L)
+    int pad;
+};
+
+struct EnsPlanImpl {
+    int nsys = 0;
+    int ntot = 0;
+    int nwaves = 0;
+    int min_lanes = 8;
+    int* d_offsets = nullptr;
+    int* d_order = nullptr;
+    EnsWave* d_waves = nullptr;
+    int device = -1;
+};
+
+struct EnsArgs {
+    const int* offsets;
+    const int* order;
+    const EnsWave* waves;
+    int nwaves;
+    const double* p[8];  // m rx ry rz e2 vx vy vz
+    double* out[6];
+    double dt;
+    long flag;
+    unsigned long long* counters;
+    int* status;
+};
+
+__device__ __forceinline__ unsigned int ens_wave_sum(unsigned int v)
+{
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// Lane -> (particle index, system range); active false for idle lanes.
+struct EnsLane {
+    bool active;
+    int i, j0, j1;
+};
+
+__device__ __forceinline__ EnsLane ens_lane(const EnsArgs& a, int wave, int lane)
+{
+    EnsLane r{false, 0, 0, 0};
+    if (wave >= a.nwaves) return r;
+    const EnsWave w = a.waves[wave];
+    const int slot = lane / w.lanes;
+    const int il = lane % w.lanes;
+    if (slot >= w.count) return r;
+    const int s = a.order[w.first + slot];
+    const int b = a.offsets[s];
+    const int n = a.offsets[s + 1] - b;
+    if (il >= n) return r;
+    r.active = true;
+    r.i = b + il;
+    r.j0 = b;
+    r.j1 = b + n;
+    return r;
+}
+
+__global__ __launch_bounds__(256) void ensemble_sakura_kernel(EnsArgs a)
+{
+    const int gtid = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const int wave = gtid / kWave;
+    const int lane = gtid % kWave;
+    const EnsLane L = ens_lane(a, wave, lane);
+    kep::Counters cnt;
+    double d[6] = {0, 0, 0, 0, 0, 0};
+    if (L.active) {
+        const int i = L.i;
+        const double im = a.p[0][i], ix = a.p[1][i], iy = a.p[2][i], iz = a.p[3][i];
+        const double ie2 = a.p[4][i], ivx = a.p[5][i], ivy = a.p[6][i], ivz = a.p[7][i];
+        for (int j = L.j0; j < L.j1; ++j) {
+            const double jm = a.p[0][j];
+            const double p0[3] = {ix - a.p[1][j], iy - a.p[2][j], iz - a.p[3][j]};
+            const double e2 = ie2 + a.p[4][j];
+            const double v0[3] = {ivx - a.p[5][j], ivy - a.p[6][j], ivz - a.p[7][j]};
+            const double m = im + jm;
+            double p[3] = {p0[0], p0[1], p0[2]};
+            double v[3] = {v0[0], v0[1], v0[2]};
+            kep::drift_twobody(a.dt, a.flag, m, e2, p, v, cnt);
+            const double mu = jm / m;
+            d[0] += mu * (p[0] - p0[0]);
+            d[1] += mu * (p[1] - p0[1]);
+            d[2] += mu * (p[2] - p0[2]);
+            d[3] += mu * (v[0] - v0[0]);
+            d[4] += mu * (v[1] - v0[1]);
+            d[5] += mu * (v[2] - v0[2]);
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) a.out[k][i] = d[k];
+    }
+    if (a.counters) {
+        const unsigned int s = ens_wave_sum(L.active ? cnt.solves : 0u);
+        const unsigned int it = ens_wave_sum(L.active ? cnt.iters : 0u);
+        const unsigned int rt = ens_wave_sum(L.active ? cnt.retries : 0u);
+        const unsigned int lf = ens_wave_sum(L.active ? cnt.leapfrog : 0u);
+        if (lane == 0 && wave < a.nwaves) {
+            atomicAdd(a.counters + 0, static_cast<unsigned long long>(s));
+            atomicAdd(a.counters + 1, static_cast<unsigned long long>(it));
+            atomicAdd(a.counters + 2, static_cast<unsigned long long>(rt));
+            atomicAdd(a.counters + 3, static_cast<unsigned long long>(lf));
+        }
+    }
+    if (a.status && L.active && cnt.fail) atomicOr(a.status, 1);
+}
+
+// Potential of each particle due to its own system (phi_kernel_common.h:7-32),
+// FP64 with IEEE division and square root in the reference's order.
+__global__ __launch_bounds__(256) void ensemble_phi_kernel(EnsArgs a)
+{
+    const int gtid = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const EnsLane L = ens_lane(a, gtid / kWave, gtid % kWave);
+    if (!L.active) return;
+    const int i = L.i;
+    const double ix = a.p[1][i], iy = a.p[2][i], iz = a.p[3][i], ie2 = a.p[4][i];
+    double phi = 0;
+    for (int j = L.j0; j < L.j1; ++j) {
+        const double dx = ix - a.p[1][j];
+        const double dy = iy - a.p[2][j];
+        const double dz = iz - a.p[3][j];
+        const double e2 = ie2 + a.p[4][j];
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        double q = 1.0 / (r2 + e2);
+        q = r2 > 0 ? q : 0.0;
+        phi -= a.p[0][j] * __builtin_sqrt(q);
+    }
+    a.out[0][i] = phi;
+}
+
+// ---------------------------------------------------------------------------
+// Plans.
+// ---------------------------------------------------------------------------
+static int lanes_for(int n, int min_lanes)
+{
+    int L = min_lanes;
+    while (L < n && L < 64) L *= 2;
+    return L;
+}
+
+int ensemble_plan_create(int nsys, const int* h_offsets, int min_lanes, void** plan_out)
+{
+    if (!plan_out) return set_error(TUPAN_ERR_ARG, "null plan pointer");
+    *plan_out = nullptr;
+    if (nsys < 0 || (nsys > 0 && !h_offsets)) return set_error(TUPAN_ERR_ARG, "bad offsets");
+    if (min_lanes != 8 && min_lanes != 16 && min_lanes != 32 && min_lanes != 64)
+        return set_error(TUPAN_ERR_ARG, "min_lanes must be 8, 16, 32 or 64");
+    std::vector<int> order(nsys);
+    for (int s = 0; s < nsys; ++s) {
+        const int n = h_offsets[s + 1] - h_offsets[s];
+        if (n < 0 || n > 64) return set_error(TUPAN_ERR_ARG, "system sizes must be in [0, 64]");
+        if (h_offsets[s] < 0) return set_error(TUPAN_ERR_ARG, "negative offset");
+        order[s] = s;
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        return (h_offsets[x + 1] - h_offsets[x]) < (h_offsets[y + 1] - h_offsets[y]);
+    });
+    std::vector<EnsWave> waves;
+    int k = 0;
+    while (k < nsys) {
+        const int n = h_offsets[order[k] + 1] - h_offsets[order[k]];
+        const int L = lanes_for(n, min_lanes);
+        const int per = kWave / L;
+        EnsWave w{k, 0, L, 0};
+        while (k < nsys && w.count < per &&
+               lanes_for(h_offsets[order[k] + 1] - h_offsets[order[k]], min_lanes) == L) {
+            ++w.count;
+            ++k;
+        }
+        waves.push_back(w);
+    }
+    EnsPlanImpl* p = new EnsPlanImpl();
+    p->nsys = nsys;
+    p->ntot = nsys > 0 ? h_offsets[nsys] : 0;
+    p->nwaves = static_cast<int>(waves.size());
+    p->min_lanes = min_lanes;
+    (void)hipGetDevice(&p->device);
+    int rc = TUPAN_OK;
+    if (nsys > 0) {
+        rc = check_hip(hipMalloc(&p->d_offsets, sizeof(int) * (nsys + 1)), "plan alloc");
+        if (rc == TUPAN_OK) rc = check_hip(hipMalloc(&p->d_order, sizeof(int) * nsys), "plan alloc");
+        if (rc == TUPAN_OK)
+            rc = check_hip(hipMalloc(&p->d_waves, sizeof(EnsWave) * waves.size()), "plan alloc");
+        if (rc == TUPAN_OK)
+            rc = check_hip(hipMemcpy(p->d_offsets, h_offsets, sizeof(int) * (nsys + 1),
+                                     hipMemcpyHostToDevice), "plan copy");
+        if (rc == TUPAN_OK)
+            rc = check_hip(hipMemcpy(p->d_order, order.data(), sizeof(int) * nsys,
+                                     hipMemcpyHostToDevice), "plan copy");
+        if (rc == TUPAN_OK)
+            rc = check_hip(hipMemcpy(p->d_waves, waves.data(), sizeof(EnsWave) * waves.size(),
+                                     hipMemcpyHostToDevice), "plan copy");
+    }
+    if (rc != TUPAN_OK) {
+        ensemble_plan_destroy(p);
+        return rc;
+    }
+    *plan_out = p;
+    return TUPAN_OK;
+}
+
+int ensemble_plan_destroy(void* plan)
+{
+    EnsPlanImpl* p = static_cast<EnsPlanImpl*>(plan);
+    if (!p) return TUPAN_OK;
+    (void)hipFree(p->d_offsets);
+    (void)hipFree(p->d_order);
+    (void)hipFree(p->d_waves);
+    delete p;
+    return TUPAN_OK;
+}
+
+int ensemble_plan_info(const void* plan, int* nsys, int* ntot, int* nwaves)
+{
+    const EnsPlanImpl* p = static_cast<const EnsPlanImpl*>(plan);
+    if (!p) return set_error(TUPAN_ERR_ARG, "null plan");
+    if (nsys) *nsys = p->nsys;
+    if (ntot) *ntot = p->ntot;
+    if (nwaves) *nwaves = p->nwaves;
+    return TUPAN_OK;
+}
+
+static int ens_args(const void* plan, const double* const* in8, EnsArgs& a)
+{
+    const EnsPlanImpl* p = static_cast<const EnsPlanImpl*>(plan);
+    if (!p) return set_error(TUPAN_ERR_ARG, "null plan");
+    a = EnsArgs{};
+    a.offsets = p->d_offsets;
+    a.order = p->d_order;
+    a.waves = p->d_waves;
+    a.nwaves = p->nwaves;
+    for (int f = 0; f < 8; ++f) {
+        if (p->ntot > 0 && !in8[f]) return set_error(TUPAN_ERR_ARG, "null input array");
+        a.p[f] = in8[f];
+    }
+    return TUPAN_OK;
+}
+
+int ensemble_sakura_call(const void* plan, const double* const* in8, double dt, long flag,
+                         double* const* out6, unsigned long long* counters, int* status,
+                         void* stream)
+{
+    EnsArgs a;
+    int rc = ens_args(plan, in8, a);
+    if (rc != TUPAN_OK) return rc;
+    for (int k = 0; k < 6; ++k) {
+        if (!out6[k] && static_cast<const EnsPlanImpl*>(plan)->ntot > 0)
+            return set_error(TUPAN_ERR_ARG, "null output array");
+        a.out[k] = out6[k];
+    }
+    a.dt = dt;
+    a.flag = flag;
+    a.counters = counters;
+    a.status = status;
+    if (a.nwaves == 0) return TUPAN_OK;
+    const unsigned blocks = static_cast<unsigned>((a.nwaves + 3) / 4);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    timing_begin(s);
+    hipLaunchKernelGGL(ensemble_sakura_kernel, dim3(blocks), dim3(256), 0, s, a);
+    timing_end(s);
+    return check_launch("ensemble_sakura");
+}
+
+int ensemble_phi_call(const void* plan, const double* const* in8, double* phi, void* stream)
+{
+    EnsArgs a;
+    int rc = ens_args(plan, in8, a);
+    if (rc != TUPAN_OK) return rc;
+    if (!phi && static_cast<const EnsPlanImpl*>(plan)->ntot > 0)
+        return set_error(TUPAN_ERR_ARG, "null output array");
+    a.out[0] = phi;
+    if (a.nwaves == 0) return TUPAN_OK;
+    const unsigned blocks = static_cast<unsigned>((a.nwaves + 3) / 4);
+    hipLaunchKernelGGL(ensemble_phi_kernel, dim3(blocks), dim3(256), 0,
+                       static_cast<hipStream_t>(stream), a);
+    return check_launch("ensemble_phi");
+}
+
+}  // namespace tupan

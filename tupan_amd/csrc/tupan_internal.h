@@ -77,6 +77,14 @@ int sakura_call(size_t ni, const double* const* ip, size_t nj, const double* con
                 double dt, long flag, double* const* out, unsigned long long* counters,
                 int* status, void* stream);
 
+int ensemble_plan_create(int nsys, const int* h_offsets, int min_lanes, void** plan_out);
+int ensemble_plan_destroy(void* plan);
+int ensemble_plan_info(const void* plan, int* nsys, int* ntot, int* nwaves);
+int ensemble_sakura_call(const void* plan, const double* const* in8, double dt, long flag,
+                         double* const* out6, unsigned long long* counters, int* status,
+                         void* stream);
+int ensemble_phi_call(const void* plan, const double* const* in8, double* phi, void* stream);
+
 int kepler_pairs_call(size_t npairs, const int* idx, const double* m, double* rx, double* ry,
                       double* rz, const double* e2, double* vx, double* vy, double* vz,
                       double dt, int* status, void* stream);
