@@ -23,6 +23,8 @@ Also reported:
 """
 import argparse
 import json
+
+import numpy as np
 import os
 import sys
 import time
@@ -51,6 +53,9 @@ def parse():
     ap.add_argument("--n", type=int, default=65536)
     ap.add_argument("--energy-steps", type=int, default=20)
     ap.add_argument("--sakura-n", type=int, default=16384)
+    ap.add_argument("--ensemble-systems", type=int, default=4096)
+    ap.add_argument("--ensemble-lanes", type=int, default=0,
+                    help="lanes per system of the ensemble plan (0 = tuned)")
     ap.add_argument("--cpu-rows", type=int, default=0,
                     help="i-rows of the CPU baseline sample (0 = auto)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -135,6 +140,48 @@ def sakura_rate(n):
             "leapfrog_pairs": leap, "kernel_s": kms / 1e3, "call_s": wall,
             "solves_per_s": solves / (kms / 1e3),
             "pairs_per_s": n * n / (kms / 1e3)}
+
+
+def ensemble_rate(nsys, lanes):
+    """The ensemble workload: nsys Plummer systems with sizes U[3,64] and
+    log-uniform masses in [0.1, 1], Sakura with dt = 1/64 to t = 1, every
+    pair-drift stage one launch over all systems."""
+    import torch
+    from tupan_amd import _native
+    from tupan_amd.ics.ensemble import make_ensemble
+    from tupan_amd.integrator.ensemble import SakuraEnsemble
+    from tupan_amd.tune import tuned_ensemble_lanes
+    if not lanes:
+        lanes = tuned_ensemble_lanes()
+    t0 = time.perf_counter()
+    ps, ens = make_ensemble(nsys=nsys, nmin=3, nmax=64, seed=1,
+                            device="cuda:0", min_lanes=lanes)
+    t_ic = time.perf_counter() - t0
+    drv = SakuraEnsemble(ps, ens)
+    e0 = drv.energies()
+    ens.counters = torch.zeros(4, dtype=torch.int64, device="cuda:0")
+    torch.cuda.synchronize()
+    _native.timing(True)
+    t0 = time.perf_counter()
+    drv.evolve(1.0, 1.0 / 64)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kms, launches = _native.timing_read()
+    _native.timing(False)
+    solves, iters, retries, leap = [int(x) for x in
+                                    ens.counters.cpu().tolist()]
+    sizes = np.diff(ens.offsets)
+    pairs = int(np.sum(sizes.astype(np.int64)**2))
+    return {"systems": nsys, "particles": int(ens.ntot),
+            "sum_n2": pairs, "steps": drv.nsteps, "dt": 1.0 / 64,
+            "lanes_per_system_min": lanes, "launches": launches,
+            "kepler_solves": solves, "laguerre_iterations": iters,
+            "subdivisions": retries, "leapfrog_pairs": leap,
+            "kernel_s": kms / 1e3, "wall_s": wall, "ic_s": t_ic,
+            "solves_per_s": solves / (kms / 1e3),
+            "pair_drifts_per_s": 2 * drv.nsteps * pairs / (kms / 1e3),
+            "iterations_per_solve": iters / max(solves, 1),
+            "max_dE_E": drv.max_rel_energy_error(e0)}
 
 
 def cpu_baseline(ps, rows):
@@ -265,6 +312,11 @@ def main():
     if args.sakura_n > 0:
         log("bench: Sakura N=%d" % args.sakura_n)
         out["sakura"] = sakura_rate(args.sakura_n)
+
+    if args.ensemble_systems > 0:
+        log("bench: Sakura ensemble of %d systems" % args.ensemble_systems)
+        out["sakura_ensemble"] = ensemble_rate(args.ensemble_systems,
+                                               args.ensemble_lanes)
 
     if not args.no_cpu:
         rows = args.cpu_rows or 2048

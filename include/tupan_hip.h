@@ -69,6 +69,24 @@ int tupan_device_info(int *cu_count, int *target_workgroups);
 int tupan_timing(int enable);
 int tupan_timing_read(double *total_ms, int *count);
 
+/* Launch tuning of the pairwise kernels (no counterpart in the reference,
+ * whose OpenCL build fixes its tile with -D LSIZE/WPT/UNROLL macros,
+ * opencl_backend.py:78-88).  Every kernel is compiled for TUPAN_TUNE_SHAPES
+ * tile shapes; tupan_tune_shape describes shape s (threads per workgroup,
+ * i-particles per thread, j-slots per LDS tile, j per unrolled step).  The
+ * table entry for (kernel TUPAN_K_*, slot, bucket) picks the shape and the
+ * workgroups-per-CU target of the launch planner.  slot: 0 FP64, 1
+ * split-FP32, 2 reference-order FP64, 3 float build.  bucket by pair count
+ * ni*nj: 0 below 2^22, 1 below 2^30, 2 above.  Default: shape 0, 4
+ * workgroups per CU.  The table is process-wide. */
+#define TUPAN_TUNE_SHAPES 5
+#define TUPAN_TUNE_BUCKETS 3
+int tupan_tune_set(int kernel, int slot, int bucket, int shape, int wg_per_cu);
+int tupan_tune_get(int kernel, int slot, int bucket, int *shape,
+                   int *wg_per_cu);
+int tupan_tune_shape(int kernel, int shape, int *block, int *wpt, int *tile,
+                     int *unroll);
+
 /* ---------------------------------------------------------------------------
  * Host-pointer layer (reference ABI, FP64).  libtupan.h line numbers cite
  * the declaration each function replaces.

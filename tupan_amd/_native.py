@@ -42,6 +42,14 @@ def _declare(lib):
     lib.tupan_timing_read.restype = ctypes.c_int
     lib.tupan_timing_read.argtypes = [ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_int)]
+    for fn in ("tupan_tune_set",):
+        getattr(lib, fn).restype = ctypes.c_int
+        getattr(lib, fn).argtypes = [ctypes.c_int] * 5
+    IP = ctypes.POINTER(ctypes.c_int)
+    lib.tupan_tune_get.restype = ctypes.c_int
+    lib.tupan_tune_get.argtypes = [ctypes.c_int] * 3 + [IP, IP]
+    lib.tupan_tune_shape.restype = ctypes.c_int
+    lib.tupan_tune_shape.argtypes = [ctypes.c_int] * 2 + [IP] * 4
     lib.tupan_pairwise.restype = ctypes.c_int
     lib.tupan_pairwise.argtypes = [ctypes.c_int, ctypes.c_int, _S, _PP, _S, _PP,
                                    _PP, ctypes.c_double, _P]
@@ -80,6 +88,8 @@ def lib():
                 "libtupan_hip.so is not built at %s; run "
                 "`python -m tupan_amd.csrc.build`" % LIB_PATH)
         _lib = _declare(ctypes.CDLL(LIB_PATH))
+        from .tune import apply
+        apply(_lib)
     return _lib
 
 
@@ -125,6 +135,7 @@ REFERENCE_ABI = ("phi_kernel", "acc_kernel", "acc_jerk_kernel",
                  "snap_crackle_kernel", "tstep_kernel", "sakura_kernel",
                  "kepler_solver_kernel")
 DEVICE_ABI = ("tupan_last_error", "tupan_version", "tupan_device_info",
+              "tupan_tune_set", "tupan_tune_get", "tupan_tune_shape",
               "tupan_timing", "tupan_timing_read",
               "tupan_phi", "tupan_acc", "tupan_acc_jerk", "tupan_snap_crackle",
               "tupan_tstep", "tupan_pairwise", "tupan_pairwise_f32",

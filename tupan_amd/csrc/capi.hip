@@ -268,6 +268,21 @@ int tupan_timing_read(double* total_ms, int* count)
     return TUPAN_OK;
 }
 
+int tupan_tune_set(int kernel, int slot, int bucket, int shape, int wg_per_cu)
+{
+    return tune_set(kernel, slot, bucket, shape, wg_per_cu);
+}
+
+int tupan_tune_get(int kernel, int slot, int bucket, int* shape, int* wg_per_cu)
+{
+    return tune_get(kernel, slot, bucket, shape, wg_per_cu);
+}
+
+int tupan_tune_shape(int kernel, int shape, int* block, int* wpt, int* tile, int* unroll)
+{
+    return tune_shape(kernel, shape, block, wpt, tile, unroll);
+}
+
 int tupan_device_info(int* cu_count, int* target_workgroups)
 {
     int n = 0;

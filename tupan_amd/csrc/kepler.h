@@ -25,10 +25,11 @@ namespace kep {
 constexpr double kTol = 2.2737367544323205948e-13;  // 2^-42
 constexpr int kMaxIter = 64;
 constexpr int kMaxDoublings = 20;
-// Solves one pair may spend across both retry loops; past it every further
+// Solves one pair may spend across both retry loops (enough for the energy
+// check to double its step count up to 2^kMaxDoublings); past it every further
 // solve reports failure at once, so the loops run out within a few hundred
 // trivial steps and the pair is flagged.
-constexpr unsigned int kSolveBudget = 1u << 16;
+constexpr unsigned int kSolveBudget = 1u << 22;
 constexpr double kPi = 3.141592653589793;
 
 struct Counters {

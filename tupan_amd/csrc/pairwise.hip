@@ -647,14 +647,20 @@ struct OpSplitAccJerk {
 };
 
 // ---------------------------------------------------------------------------
-// Launch plumbing.
+// Launch plumbing and tile configurations.
+//
+// Each interaction is compiled for a few tile shapes (BLOCK threads, WPT
+// i-particles per thread, TILE j-slots per LDS tile, UNROLL j per unrolled
+// step).  A run-time table, filled from the autotuner's cache, picks the
+// shape and the workgroups-per-CU target per kernel, mode and size bucket.
 // ---------------------------------------------------------------------------
 template <class Op, int BLOCK, int WPT, int TILE, int UNROLL>
-static int run_pairwise(typename Op::Args a, hipStream_t stream)
+static int run_pairwise(typename Op::Args a, hipStream_t stream, int wg_per_cu)
 {
     if (a.ni <= 0) return 0;
     const Device& dev = current_device();
-    Launch L = plan_launch(a.ni, a.nj, BLOCK, WPT, TILE, dev.target_wg, 64);
+    const int target = (wg_per_cu > 0 ? wg_per_cu : 4) * dev.cu_count;
+    Launch L = plan_launch(a.ni, a.nj, BLOCK, WPT, TILE, target, 64);
     a.nsplit = L.nsplit;
     a.jchunk = L.jchunk;
     a.partial = nullptr;
@@ -675,32 +681,99 @@ static int run_pairwise(typename Op::Args a, hipStream_t stream)
     return check_launch("pairwise");
 }
 
-// Kernel selection by name and mode.  Tile shapes: BLOCK threads, WPT i per
-// thread, TILE j per LDS tile, UNROLL j per unrolled step.
+template <class Op>
+using Launcher = int (*)(typename Op::Args, hipStream_t, int);
+
+// Shapes for the light interactions (Phi, Acc, AccJerk, Tstep, split).
+#define TUPAN_LIGHT_SHAPES(Op)                                                   \
+    {                                                                            \
+        &run_pairwise<Op, 256, 2, 256, 4>, &run_pairwise<Op, 256, 1, 256, 8>,    \
+            &run_pairwise<Op, 256, 4, 256, 2>, &run_pairwise<Op, 128, 2, 128, 4>, \
+            &run_pairwise<Op, 512, 1, 512, 4>                                    \
+    }
+// Shapes for SnapCrackle (twice the registers per i).
+#define TUPAN_HEAVY_SHAPES(Op)                                                   \
+    {                                                                            \
+        &run_pairwise<Op, 256, 1, 128, 2>, &run_pairwise<Op, 128, 1, 128, 2>,    \
+            &run_pairwise<Op, 256, 2, 128, 1>, &run_pairwise<Op, 64, 1, 64, 2>,  \
+            &run_pairwise<Op, 256, 1, 256, 4>                                    \
+    }
+constexpr int kShapes = 5;
+static const int kLightShape[kShapes][4] = {
+    {256, 2, 256, 4}, {256, 1, 256, 8}, {256, 4, 256, 2}, {128, 2, 128, 4}, {512, 1, 512, 4}};
+static const int kHeavyShape[kShapes][4] = {
+    {256, 1, 128, 2}, {128, 1, 128, 2}, {256, 2, 128, 1}, {64, 1, 64, 2}, {256, 1, 256, 4}};
+
+// Size buckets by pair count ni * nj.
+constexpr int kBuckets = 3;
+static int size_bucket(int ni, int nj)
+{
+    const double pairs = static_cast<double>(ni) * static_cast<double>(nj);
+    if (pairs < 4194304.0) return 0;        // < 2^22
+    if (pairs < 1073741824.0) return 1;     // < 2^30
+    return 2;
+}
+
+// Tuning table: [kernel][mode slot][bucket] -> (shape, workgroups per CU).
+// Mode slots: 0 fp64, 1 split32, 2 ieee64, 3 float build.
+struct TuneEntry {
+    int shape = 0;
+    int wg_per_cu = 4;
+};
+static TuneEntry g_tune[5][4][kBuckets];
+
+static int mode_slot(int mode, bool f32)
+{
+    if (f32) return 3;
+    if (mode == TUPAN_MODE_SPLIT32) return 1;
+    if (mode == TUPAN_MODE_IEEE64) return 2;
+    return 0;
+}
+
+template <class Op>
+static int launch_tuned(const Launcher<Op> (&table)[kShapes], int kernel, int slot,
+                        typename Op::Args& a, hipStream_t s)
+{
+    const TuneEntry& t = g_tune[kernel][slot][size_bucket(a.ni, a.nj)];
+    const int shape = (t.shape >= 0 && t.shape < kShapes) ? t.shape : 0;
+    return table[shape](a, s, t.wg_per_cu);
+}
+
+#define TUPAN_RUN(Op, SHAPES, kernel, slot)                                      \
+    do {                                                                         \
+        static const Launcher<Op> table[kShapes] = SHAPES(Op);                  \
+        return launch_tuned<Op>(table, kernel, slot, a, s);                      \
+    } while (0)
+
+// Kernel selection by name and mode.
 template <class T>
 static int dispatch(int kernel, int mode, PairArgs<T>& a, hipStream_t s);
 
 template <>
 int dispatch<double>(int kernel, int mode, PairArgs<double>& a, hipStream_t s)
 {
+    const int slot = mode_slot(mode, false);
     switch (kernel) {
     case K_PHI:
-        if (mode == TUPAN_MODE_IEEE64) return run_pairwise<OpPhi<Ieee64>, 256, 2, 256, 4>(a, s);
-        return run_pairwise<OpPhi<Fast64>, 256, 2, 256, 4>(a, s);
+        if (mode == TUPAN_MODE_IEEE64) TUPAN_RUN(OpPhi<Ieee64>, TUPAN_LIGHT_SHAPES, K_PHI, slot);
+        TUPAN_RUN(OpPhi<Fast64>, TUPAN_LIGHT_SHAPES, K_PHI, slot);
     case K_ACC:
-        if (mode == TUPAN_MODE_SPLIT32) return run_pairwise<OpSplitAcc, 256, 2, 256, 4>(a, s);
-        if (mode == TUPAN_MODE_IEEE64) return run_pairwise<OpAcc<Ieee64>, 256, 2, 256, 4>(a, s);
-        return run_pairwise<OpAcc<Fast64>, 256, 2, 256, 4>(a, s);
+        if (mode == TUPAN_MODE_SPLIT32) TUPAN_RUN(OpSplitAcc, TUPAN_LIGHT_SHAPES, K_ACC, slot);
+        if (mode == TUPAN_MODE_IEEE64) TUPAN_RUN(OpAcc<Ieee64>, TUPAN_LIGHT_SHAPES, K_ACC, slot);
+        TUPAN_RUN(OpAcc<Fast64>, TUPAN_LIGHT_SHAPES, K_ACC, slot);
     case K_ACC_JERK:
-        if (mode == TUPAN_MODE_SPLIT32) return run_pairwise<OpSplitAccJerk, 256, 2, 256, 4>(a, s);
-        if (mode == TUPAN_MODE_IEEE64) return run_pairwise<OpAccJerk<Ieee64>, 256, 2, 256, 4>(a, s);
-        return run_pairwise<OpAccJerk<Fast64>, 256, 2, 256, 4>(a, s);
+        if (mode == TUPAN_MODE_SPLIT32)
+            TUPAN_RUN(OpSplitAccJerk, TUPAN_LIGHT_SHAPES, K_ACC_JERK, slot);
+        if (mode == TUPAN_MODE_IEEE64)
+            TUPAN_RUN(OpAccJerk<Ieee64>, TUPAN_LIGHT_SHAPES, K_ACC_JERK, slot);
+        TUPAN_RUN(OpAccJerk<Fast64>, TUPAN_LIGHT_SHAPES, K_ACC_JERK, slot);
     case K_SNAP_CRACKLE:
-        if (mode == TUPAN_MODE_IEEE64) return run_pairwise<OpSnapCrackle<Ieee64>, 256, 1, 128, 2>(a, s);
-        return run_pairwise<OpSnapCrackle<Fast64>, 256, 1, 128, 2>(a, s);
+        if (mode == TUPAN_MODE_IEEE64)
+            TUPAN_RUN(OpSnapCrackle<Ieee64>, TUPAN_HEAVY_SHAPES, K_SNAP_CRACKLE, slot);
+        TUPAN_RUN(OpSnapCrackle<Fast64>, TUPAN_HEAVY_SHAPES, K_SNAP_CRACKLE, slot);
     case K_TSTEP:
-        if (mode == TUPAN_MODE_IEEE64) return run_pairwise<OpTstep<Ieee64>, 256, 2, 256, 4>(a, s);
-        return run_pairwise<OpTstep<Fast64>, 256, 2, 256, 4>(a, s);
+        if (mode == TUPAN_MODE_IEEE64) TUPAN_RUN(OpTstep<Ieee64>, TUPAN_LIGHT_SHAPES, K_TSTEP, slot);
+        TUPAN_RUN(OpTstep<Fast64>, TUPAN_LIGHT_SHAPES, K_TSTEP, slot);
     default:
         return set_error(TUPAN_ERR_ARG, "unknown pairwise kernel");
     }
@@ -709,14 +782,47 @@ int dispatch<double>(int kernel, int mode, PairArgs<double>& a, hipStream_t s)
 template <>
 int dispatch<float>(int kernel, int, PairArgs<float>& a, hipStream_t s)
 {
+    const int slot = mode_slot(0, true);
     switch (kernel) {
-    case K_PHI: return run_pairwise<OpPhi<Fast32>, 256, 2, 256, 4>(a, s);
-    case K_ACC: return run_pairwise<OpAcc<Fast32>, 256, 2, 256, 4>(a, s);
-    case K_ACC_JERK: return run_pairwise<OpAccJerk<Fast32>, 256, 2, 256, 4>(a, s);
-    case K_SNAP_CRACKLE: return run_pairwise<OpSnapCrackle<Fast32>, 256, 1, 128, 2>(a, s);
-    case K_TSTEP: return run_pairwise<OpTstep<Fast32>, 256, 2, 256, 4>(a, s);
+    case K_PHI: TUPAN_RUN(OpPhi<Fast32>, TUPAN_LIGHT_SHAPES, K_PHI, slot);
+    case K_ACC: TUPAN_RUN(OpAcc<Fast32>, TUPAN_LIGHT_SHAPES, K_ACC, slot);
+    case K_ACC_JERK: TUPAN_RUN(OpAccJerk<Fast32>, TUPAN_LIGHT_SHAPES, K_ACC_JERK, slot);
+    case K_SNAP_CRACKLE:
+        TUPAN_RUN(OpSnapCrackle<Fast32>, TUPAN_HEAVY_SHAPES, K_SNAP_CRACKLE, slot);
+    case K_TSTEP: TUPAN_RUN(OpTstep<Fast32>, TUPAN_LIGHT_SHAPES, K_TSTEP, slot);
     default: return set_error(TUPAN_ERR_ARG, "unknown pairwise kernel");
     }
+}
+
+int tune_set(int kernel, int slot, int bucket, int shape, int wg_per_cu)
+{
+    if (kernel < 0 || kernel > 4 || slot < 0 || slot > 3 || bucket < 0 || bucket >= kBuckets ||
+        shape < 0 || shape >= kShapes || wg_per_cu < 1 || wg_per_cu > 64)
+        return set_error(TUPAN_ERR_ARG, "tuning entry out of range");
+    g_tune[kernel][slot][bucket].shape = shape;
+    g_tune[kernel][slot][bucket].wg_per_cu = wg_per_cu;
+    return TUPAN_OK;
+}
+
+int tune_get(int kernel, int slot, int bucket, int* shape, int* wg_per_cu)
+{
+    if (kernel < 0 || kernel > 4 || slot < 0 || slot > 3 || bucket < 0 || bucket >= kBuckets)
+        return set_error(TUPAN_ERR_ARG, "tuning entry out of range");
+    if (shape) *shape = g_tune[kernel][slot][bucket].shape;
+    if (wg_per_cu) *wg_per_cu = g_tune[kernel][slot][bucket].wg_per_cu;
+    return TUPAN_OK;
+}
+
+int tune_shape(int kernel, int shape, int* block, int* wpt, int* tile, int* unroll)
+{
+    if (kernel < 0 || kernel > 4 || shape < 0 || shape >= kShapes)
+        return set_error(TUPAN_ERR_ARG, "shape out of range");
+    const int* d = kernel == K_SNAP_CRACKLE ? kHeavyShape[shape] : kLightShape[shape];
+    if (block) *block = d[0];
+    if (wpt) *wpt = d[1];
+    if (tile) *tile = d[2];
+    if (unroll) *unroll = d[3];
+    return TUPAN_OK;
 }
 
 template <class T>

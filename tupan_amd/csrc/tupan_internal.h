@@ -73,6 +73,11 @@ template <class T>
 int pairwise_call(int kernel, int mode, size_t ni, const T* const* ip, size_t nj,
                   const T* const* jp, T* const* out, T eta, void* stream);
 
+// Tuning table of the pairwise kernels (pairwise.hip).
+int tune_set(int kernel, int slot, int bucket, int shape, int wg_per_cu);
+int tune_get(int kernel, int slot, int bucket, int* shape, int* wg_per_cu);
+int tune_shape(int kernel, int shape, int* block, int* wpt, int* tile, int* unroll);
+
 int sakura_call(size_t ni, const double* const* ip, size_t nj, const double* const* jp,
                 double dt, long flag, double* const* out, unsigned long long* counters,
                 int* status, void* stream);
