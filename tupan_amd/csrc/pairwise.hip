@@ -15,8 +15,8 @@
 //   Fast32  FP32 throughout (the reference's float build, `--use_sp`).
 // plus the split-FP32 interactions (SplitAcc, SplitAccJerk): FP64 inputs and
 // outputs, pair arithmetic in FP32 on hi+lo coordinates relative to the
-// workgroup's first i-particle, FP32 partial sums flushed into FP64 every
-// kFlush j-particles.
+// workgroup's first i-particle, 1/R from the v_rsq_f32 estimate, FP32 partial
+// sums flushed into FP64 every kFlush j-particles.
 //
 // This file is compiled with -ffp-contract=off: every fused multiply-add
 // below is explicit.
@@ -485,7 +485,8 @@ struct SplitFrame {
     double cx, cy, cz, cvx, cvy, cvz;
 };
 
-struct OpSplitAcc {
+template <bool kRefine>
+struct OpSplitAccT {
     using Args = PairArgs<double>;
     using Frame = SplitFrame;
     // xh yh zh m | xl yl zl e2
@@ -529,7 +530,7 @@ struct OpSplitAcc {
         float dz = (r.zh - s.v[2]) + (r.zl - s.v[6]);
         float e2 = r.e2 + s.v[7];
         float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-        float y = rsqrt_refined(r2 + e2);
+        float y = kRefine ? rsqrt_refined(r2 + e2) : __builtin_amdgcn_rsqf(r2 + e2);
         y = r2 > 0.0f ? y : 0.0f;
         float q3 = (y * y) * (y * s.v[3]);
         c.ax = __builtin_fmaf(-q3, dx, c.ax);
@@ -555,7 +556,8 @@ struct OpSplitAcc {
     }
 };
 
-struct OpSplitAccJerk {
+template <bool kRefine>
+struct OpSplitAccJerkT {
     using Args = PairArgs<double>;
     using Frame = SplitFrame;
     // xh yh zh m | xl yl zl e2 | vx vy vz pad
@@ -610,7 +612,7 @@ struct OpSplitAccJerk {
         float uz = r.vz - s.v[10];
         float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
         float rv = __builtin_fmaf(dz, uz, __builtin_fmaf(dy, uy, dx * ux));
-        float y = rsqrt_refined(r2 + e2);
+        float y = kRefine ? rsqrt_refined(r2 + e2) : __builtin_amdgcn_rsqf(r2 + e2);
         y = r2 > 0.0f ? y : 0.0f;
         float q2 = y * y;
         float q3 = q2 * (y * s.v[3]);
@@ -645,6 +647,13 @@ struct OpSplitAccJerk {
         for (int k = 0; k < 6; ++k) a.out[k][i] = t[k];
     }
 };
+
+// The split path uses the raw v_rsq_f32 estimate (within 1.6 ulp): on the
+// 65536-particle Plummer state a Newton step changes neither the median
+// (1.7e-8) nor the 99.9th percentile (2.8e-7) of the relative force error,
+// and costs three more instructions per pair.
+using OpSplitAcc = OpSplitAccT<false>;
+using OpSplitAccJerk = OpSplitAccJerkT<false>;
 
 // ---------------------------------------------------------------------------
 // Launch plumbing and tile configurations.
