@@ -17,6 +17,10 @@ Also reported:
                 the same initial state
   sakura        Kepler solves/s of one Sakura pair-drift call (flag -1,
                 dt 1/64) on a Plummer sphere of N = 16384 with eps = 0
+  sakura_ensemble  4096 few-body systems, Sakura dt 1/64 to t = 1
+  block_steps   sia21h.dkd hierarchical block steps on a 262144-particle
+                power-law-mass Plummer sphere, pair interactions per second
+                over whole steps
   cpu_baseline  the CPU oracle (serial C99 restatement of the reference C
                 backend, same operation order, gcc -O3 -std=c99) on a bounded
                 sample of i-rows against all N j-particles, scaled per pair
@@ -54,6 +58,12 @@ def parse():
     ap.add_argument("--energy-steps", type=int, default=20)
     ap.add_argument("--sakura-n", type=int, default=16384)
     ap.add_argument("--ensemble-systems", type=int, default=4096)
+    ap.add_argument("--sia-n", type=int, default=262144,
+                    help="block-step workload size (0 skips it)")
+    ap.add_argument("--sia-tau", type=float, default=0.02 / 64,
+                    help="base step of the timed sia21h.dkd block steps")
+    ap.add_argument("--sia-steps", type=int, default=2)
+    ap.add_argument("--sia-modes", default="split32,fp64")
     ap.add_argument("--ensemble-lanes", type=int, default=0,
                     help="lanes per system of the ensemble plan (0 = tuned)")
     ap.add_argument("--cpu-rows", type=int, default=0,
@@ -182,6 +192,62 @@ def ensemble_rate(nsys, lanes):
             "pair_drifts_per_s": 2 * drv.nsteps * pairs / (kms / 1e3),
             "iterations_per_solve": iters / max(solves, 1),
             "max_dE_E": drv.max_rel_energy_error(e0)}
+
+
+def block_step_rate(n, tau, steps, modes):
+    """The block-step workload: Plummer sphere of n particles with masses
+    dN/dm ~ m^-2.35 on [0.1, 10] (normalised to total mass 1), eps = 1e-4,
+    seed 1, integrated by sia21h.dkd (eta = 0.02, hierarchical power-of-two
+    slow/fast levels) with base step `tau`: one untimed step, then `steps`
+    timed ones.  Throughput counts every n_i * n_j of every kernel call of
+    the timed steps (Tstep on each level, Acc for kicks and slow<->fast
+    coupling) over the wall time."""
+    import torch
+    from tupan_amd import _native
+    from tupan_amd.diagnostics import energy
+    from tupan_amd.ics.imf import IMF
+    from tupan_amd.ics.plummer import make_plummer
+    from tupan_amd.integrator import Integrator
+    from tupan_amd.lib import extensions
+    from tupan_amd.lib.utils import ctype
+    ps0 = make_plummer(n, 1e-4, IMF.power_law(-2.35, 0.1, 10.0), seed=1,
+                       device="cuda:0")
+    out = {"n": n, "imf": "dN/dm ~ m^-2.35 on [0.1, 10]", "eps": 1e-4,
+           "seed": 1, "method": "sia21h.dkd", "eta": 0.02, "tau": tau,
+           "steps": steps}
+    old = ctype.force_mode
+    for mode in modes:
+        log("bench: block steps, %s" % mode)
+        ps = ps0.copy()
+        type(ps).t_curr = 0.0
+        e0 = energy(ps)
+        ctype.force_mode = mode
+        extensions.install("HIP", "float64")
+        try:
+            it = Integrator(0.02, 0.0, ps, method="sia21h.dkd")
+            it.evolve_step(tau)
+            torch.cuda.synchronize()
+            cnt = extensions.count_pairs(True)
+            _native.timing(True)
+            t0 = time.perf_counter()
+            for k in range(steps):
+                it.evolve_step(tau * (k + 2))
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+            kms, launches = _native.timing_read()
+            _native.timing(False)
+            extensions.count_pairs(False)
+            ps = it.particle_system
+        finally:
+            ctype.force_mode = old
+            extensions.install("HIP", "float64")
+        pairs = int(sum(cnt.values()))
+        out[mode] = {"gint_s": pairs / wall / 1e9, "pairs": pairs,
+                     "pairs_by_kernel": {k: int(v) for k, v in cnt.items()},
+                     "wall_s": wall, "pairwise_kernel_s": kms / 1e3,
+                     "pairwise_launches": launches,
+                     "dE_E": abs((energy(ps) - e0) / e0)}
+    return out
 
 
 def cpu_baseline(ps, rows):
@@ -317,6 +383,12 @@ def main():
         log("bench: Sakura ensemble of %d systems" % args.ensemble_systems)
         out["sakura_ensemble"] = ensemble_rate(args.ensemble_systems,
                                                args.ensemble_lanes)
+
+    if args.sia_n > 0:
+        log("bench: block steps N=%d" % args.sia_n)
+        out["block_steps"] = block_step_rate(
+            args.sia_n, args.sia_tau, args.sia_steps,
+            [m for m in args.sia_modes.split(",") if m])
 
     if not args.no_cpu:
         rows = args.cpu_rows or 2048

@@ -46,7 +46,8 @@ extern "C" {
 /* Arithmetic modes of the FP64 device entry points. */
 #define TUPAN_MODE_FP64 0     /* FP64, refined hardware rsqrt, fused ops      */
 #define TUPAN_MODE_SPLIT32 1  /* split-FP32 pair arithmetic, FP64 sums (Acc,
-                                 AccJerk; other kernels fall back to FP64)    */
+                                 AccJerk, Tstep; Phi and SnapCrackle fall
+                                 back to FP64)                               */
 #define TUPAN_MODE_IEEE64 2   /* FP64 in the reference's operation order with
                                  IEEE division and square root               */
 

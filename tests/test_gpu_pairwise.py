@@ -8,9 +8,10 @@ scalar outputs relative):
                         launch uses one j split (all golden cases), since the
                         operation order, IEEE division and square root are
                         the reference's
-  split-FP32:           Acc, AccJerk <= 2e-6 on the golden sets (close
-                        random pairs; the Plummer error budget is in
-                        test_gpu_precision.py)
+  split-FP32:           Acc, AccJerk within 8 FP32 ulps of the sum of
+                        contribution magnitudes; Tstep within 8 units of its
+                        first-order FP32 error model and 1e-6 relative (the
+                        Plummer error budget is in test_gpu_precision.py)
   FP32 (float build):   <= 2e-5 against the reference's float build
 """
 import ctypes
@@ -165,6 +166,61 @@ def test_split32_golden(golden):
                                                                  "az")]
             d = np.sqrt(sum((got[k] - ref[k])**2 for k in range(3)))
             assert float(np.max(d / sa)) <= 8 * 2.0**-24
+
+
+def tstep_error_scale(hi, hj, eta):
+    """First-order FP32 error model of split-FP32 Tstep, per i-particle, for
+    tstep and tstepij (relative, in units of 2^-24).  Each pair term
+    w = w0 - ga*rv carries errors of a few ulps of the magnitudes entering
+    it: |w0| with the velocity difference widened by U = |v_i - v_0| +
+    |v_j - v_0| (velocities are rounded to FP32 relative to the first
+    i-particle v_0), and |ga| |r| (|u| + U); tstep = eta / sqrt(1 + S) then
+    halves the relative error of 1 + S."""
+    v0 = np.array([hi[k][0] for k in ("vx", "vy", "vz")])
+    vj = np.stack([hj[k] for k in ("vx", "vy", "vz")]).T
+    sc_t, sc_m = [], []
+    for i in range(len(hi["mass"])):
+        d = np.stack([hi[k][i] - hj[k] for k in ("rx", "ry", "rz")])
+        u = np.stack([hi[k][i] - hj[k] for k in ("vx", "vy", "vz")])
+        vi = np.array([hi[k][i] for k in ("vx", "vy", "vz")])
+        U = np.linalg.norm(vi - v0) + np.linalg.norm(vj - v0, axis=1)
+        r2 = (d * d).sum(0)
+        mask = r2 > 0
+        with np.errstate(all="ignore"):
+            q2 = np.where(mask, 1 / (r2 + hi["eps2"][i] + hj["eps2"]), 0)
+            phi = (hi["mass"][i] + hj["mass"]) * np.sqrt(q2)
+            un = np.sqrt((u * u).sum(0))
+            w0 = q2 * (2 * phi + (u * u).sum(0))
+            ga = q2 * (2 * q2 * phi + w0) * eta / np.sqrt(w0)
+            w = np.where(mask, w0 - ga * (d * u).sum(0), 0)
+            mag = np.where(mask, q2 * (2 * phi + (un + U)**2)
+                           + np.abs(ga) * np.sqrt(r2) * (un + U), 0)
+        sc_t.append(0.5 * mag.sum() / (1 + w.sum()))
+        j = int(np.argmax(w))
+        sc_m.append(0.5 * mag[j] / (1 + w[j]))
+    return np.array(sc_t), np.array(sc_m)
+
+
+def test_split32_tstep_golden(golden):
+    """Split-FP32 Tstep against the reference: within 8 units of the
+    first-order FP32 error model (an FP32 emulation of the kernel on these
+    sets reaches 1.4 units), and within 1e-6 relative."""
+    g = golden("pairwise_f64.npz")
+    eta = KERNELS["tstep"][1][0]
+    for soft in (0, 1):
+        tag = "s%d" % soft
+        a, b = split_set(g, tag, "A"), split_set(g, tag, "B")
+        A, B = device_ps(a), device_ps(b)
+        for lab, ii, jj, hi, hj in (("AB", A, B, a, b), ("BA", B, A, b, a),
+                                    ("AA", A, A, a, a)):
+            got = run_ext("tstep", "float64", "split32", ii, jj, (eta,))
+            scales = tstep_error_scale(hi, hj, eta)
+            for k, o in enumerate(("tstep", "tstepij")):
+                ref = g["%s_tstep_%s_%s" % (tag, lab, o)]
+                e = np.abs(got[k] - ref) / ref
+                assert float(e.max()) <= 1e-6, (lab, o, float(e.max()))
+                units = float(np.max(e / (scales[k] * 2.0**-24)))
+                assert units <= 8, (lab, o, units)
 
 
 @pytest.mark.parametrize("name", sorted(KERNELS))

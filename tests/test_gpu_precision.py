@@ -7,6 +7,9 @@ seed 1) through size-independent properties:
   * split-FP32 against FP64: the per-particle relative force error
     |da_i|/|a_i| has median <= 1e-7 and 99.9th percentile <= 1e-6; the
     jerk's median <= 1e-6 (the jerk is a derivative and cancels harder);
+  * split-FP32 Tstep against FP64: relative error median <= 1e-7, 99.9th
+    percentile <= 1e-6, block-step levels unchanged for all but <= 1e-4 of
+    the particles;
   * Newton's third law: sum m_i a_i = 0 to rounding in both paths.
 """
 import numpy as np
@@ -68,6 +71,32 @@ def test_split32_force_error(plummer64k):
     assert np.median(ea) <= 1e-7
     assert np.percentile(ea, 99.9) <= 1e-6
     assert np.median(ej) <= 1e-6
+
+
+def test_split32_tstep_error_and_levels(plummer64k):
+    """Split-FP32 Tstep against FP64 on the Plummer state: relative error
+    median <= 1e-7, 99.9th percentile <= 1e-6, and the power-of-two level a
+    block-step scheme assigns (floor(log2(tstep / eta))) differs for at most
+    one particle in 10^4."""
+    from tupan_amd.lib import extensions
+    eta = 0.02
+    out = {}
+    for mode in ("fp64", "split32"):
+        res = extensions.Tstep("HIP", "float64", mode=mode).calc(
+            plummer64k, plummer64k, eta)
+        out[mode] = [r.detach().cpu().numpy().copy() for r in res]
+    for k, name in enumerate(("tstep", "tstepij")):
+        ref, got = out["fp64"][k], out["split32"][k]
+        e = np.abs(got - ref) / ref
+        print("split32 %s: median %.3e p99.9 %.3e max %.3e"
+              % (name, np.median(e), np.percentile(e, 99.9), e.max()))
+        assert np.median(e) <= 1e-7
+        assert np.percentile(e, 99.9) <= 1e-6
+        lv_ref = np.floor(np.log2(ref / eta))
+        lv_got = np.floor(np.log2(got / eta))
+        flips = int(np.sum(lv_ref != lv_got))
+        print("split32 %s: %d level changes of %d" % (name, flips, ref.size))
+        assert flips <= ref.size // 10000
 
 
 def test_momentum_conservation(plummer64k):

@@ -123,6 +123,35 @@ def test_sia_leaf_is_kepler_and_hierarchy_aligns():
     assert abs((e1 - e0) / e0) < 1e-3
 
 
+@pytest.mark.parametrize("method", ["sia21h.dkd", "sia22h.kdk",
+                                    "sia22a.dkd", "sia44s.kdk"])
+def test_sia_inplace_partition_equals_copy_join(method):
+    """The in-place [slow | fast] partition gives the particle order and
+    every value of the reference's ps[mask] copies + append, bit for bit,
+    over several steps of a hierarchy several levels deep."""
+    from tupan_amd.ics.imf import IMF
+    from tupan_amd.ics.plummer import make_plummer
+    from tupan_amd.integrator import sia
+    runs = []
+    for inplace in (False, True):
+        p = make_plummer(40, 0.02, IMF.power_law(-2.35, 0.1, 10.0), seed=7,
+                         device="cpu")
+        type(p).t_curr = 0.0
+        it = sia.SIA(0.05, 0.0, p, method, inplace=inplace)
+        for k in range(3):
+            it.evolve_step(0.05 * (k + 1))
+        runs.append((it.ps, type(p).t_curr))
+    (a, ta), (b, tb) = runs
+    assert ta == tb
+    for k in ("id", "mass", "rx", "ry", "rz", "vx", "vy", "vz", "time",
+              "nstep", "tstep"):
+        assert torch.equal(getattr(a, k), getattr(b, k)), k
+    if method.startswith("sia21h"):
+        # the hierarchy went several levels deep and reordered particles
+        assert len(set(a.tstep.tolist())) >= 3
+        assert not torch.equal(a.id, torch.arange(a.n))
+
+
 def test_plummer_invariants_and_rng_sequence():
     from tupan_amd.ics.plummer import make_plummer
     n = 64

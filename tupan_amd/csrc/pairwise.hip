@@ -13,9 +13,9 @@
 //           reference's operation order, no contraction.  With one j split
 //           its per-i sums equal the reference C backend bit for bit.
 //   Fast32  FP32 throughout (the reference's float build, `--use_sp`).
-// plus the split-FP32 interactions (SplitAcc, SplitAccJerk): FP64 inputs and
-// outputs, pair arithmetic in FP32 on hi+lo coordinates relative to the
-// workgroup's first i-particle, 1/R from the v_rsq_f32 estimate, FP32 partial
+// plus the split-FP32 interactions (SplitAcc, SplitAccJerk, SplitTstep): FP64
+// inputs and outputs, pair arithmetic in FP32 on hi+lo coordinates relative to
+// the workgroup's first i-particle, 1/R from the v_rsq_f32 estimate, FP32 partial
 // sums flushed into FP64 every kFlush j-particles.
 //
 // This file is compiled with -ffp-contract=off: every fused multiply-add
@@ -648,6 +648,93 @@ struct OpSplitAccJerkT {
     }
 };
 
+// Split-FP32 Tstep: the pair terms of OpTstep in FP32 on split relative
+// coordinates (frame as for AccJerk), the sum of w^2 flushed into FP64 every
+// kFlush j-particles, the maximum kept in FP32 (it is one pair's term, so it
+// carries that term's FP32 error and no accumulated one).  The FP32 range
+// bounds the pair terms: ga ~ m^2 / R^6, so R must stay above ~1e-6 in N-body
+// units for total masses near 1 (softening of 1e-4 guarantees it).
+struct OpSplitTstep {
+    using Args = PairArgs<double>;
+    using Frame = SplitFrame;
+    // xh yh zh m | xl yl zl e2 | vx vy vz pad
+    using Slot = Slots<float, 12>;
+    struct IReg { float xh, yh, zh, xl, yl, zl, e2, m, vx, vy, vz, eta; };
+    struct Acc { float sum, mx; double tsum; };
+    static constexpr int kNacc = 2;
+    static constexpr int kFlush = 64;
+
+    __device__ static Frame frame(const Args& a, int ibase)
+    {
+        return OpSplitAccJerkT<false>::frame(a, ibase);
+    }
+    __device__ static void load_i(const Args& a, const Frame& f, int i, IReg& r)
+    {
+        split2(a.ip[1][i] - f.cx, r.xh, r.xl);
+        split2(a.ip[2][i] - f.cy, r.yh, r.yl);
+        split2(a.ip[3][i] - f.cz, r.zh, r.zl);
+        r.e2 = static_cast<float>(a.ip[4][i]);
+        r.m = static_cast<float>(a.ip[0][i]);
+        r.vx = static_cast<float>(a.ip[5][i] - f.cvx);
+        r.vy = static_cast<float>(a.ip[6][i] - f.cvy);
+        r.vz = static_cast<float>(a.ip[7][i] - f.cvz);
+        r.eta = static_cast<float>(a.eta);
+    }
+    __device__ static void load_j(const Args& a, const Frame& f, int j, Slot& s)
+    {
+        OpSplitAccJerkT<false>::load_j(a, f, j, s);
+    }
+    __device__ static void zero(Acc& c)
+    {
+        c.sum = 0.0f;
+        c.mx = 0.0f;
+        c.tsum = 0.0;
+    }
+    __device__ static __forceinline__ void interact(const IReg& r, const Slot& s, Acc& c)
+    {
+        float dx = (r.xh - s.v[0]) + (r.xl - s.v[4]);
+        float dy = (r.yh - s.v[1]) + (r.yl - s.v[5]);
+        float dz = (r.zh - s.v[2]) + (r.zl - s.v[6]);
+        float e2 = r.e2 + s.v[7];
+        float ux = r.vx - s.v[8];
+        float uy = r.vy - s.v[9];
+        float uz = r.vz - s.v[10];
+        float mt = r.m + s.v[3];
+        float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+        float rv = __builtin_fmaf(dz, uz, __builtin_fmaf(dy, uy, dx * ux));
+        float v2 = __builtin_fmaf(uz, uz, __builtin_fmaf(uy, uy, ux * ux));
+        const bool mask = r2 > 0.0f;
+        float q1 = __builtin_amdgcn_rsqf(r2 + e2);
+        q1 = mask ? q1 : 0.0f;
+        float q2 = q1 * q1;
+        float phi = mt * q1;
+        float w2 = q2 * __builtin_fmaf(2.0f, phi, v2);
+        float ga = q2 * __builtin_fmaf(2.0f * q2, phi, w2);
+        ga *= r.eta * __builtin_amdgcn_rsqf(w2);
+        w2 = __builtin_fmaf(-ga, rv, w2);
+        w2 = mask ? w2 : 0.0f;
+        c.sum += w2;
+        c.mx = __builtin_fmaxf(w2, c.mx);
+    }
+    __device__ static void flush(Acc& c)
+    {
+        c.tsum += static_cast<double>(c.sum);
+        c.sum = 0.0f;
+    }
+    __device__ static double get(const Acc& c, int k)
+    {
+        return k == 0 ? c.tsum : static_cast<double>(c.mx);
+    }
+    __device__ static void combine(double* t, const double* p)
+    {
+        OpTstep<Fast64>::combine(t, p);
+    }
+    __device__ static void store(const Args& a, int i, const double* t)
+    {
+        OpTstep<Fast64>::store(a, i, t);
+    }
+};
+
 // The split path uses the raw v_rsq_f32 estimate (within 1.6 ulp): on the
 // 65536-particle Plummer state a Newton step changes neither the median
 // (1.7e-8) nor the 99.9th percentile (2.8e-7) of the relative force error,
@@ -781,6 +868,7 @@ int dispatch<double>(int kernel, int mode, PairArgs<double>& a, hipStream_t s)
             TUPAN_RUN(OpSnapCrackle<Ieee64>, TUPAN_HEAVY_SHAPES, K_SNAP_CRACKLE, slot);
         TUPAN_RUN(OpSnapCrackle<Fast64>, TUPAN_HEAVY_SHAPES, K_SNAP_CRACKLE, slot);
     case K_TSTEP:
+        if (mode == TUPAN_MODE_SPLIT32) TUPAN_RUN(OpSplitTstep, TUPAN_LIGHT_SHAPES, K_TSTEP, slot);
         if (mode == TUPAN_MODE_IEEE64) TUPAN_RUN(OpTstep<Ieee64>, TUPAN_LIGHT_SHAPES, K_TSTEP, slot);
         TUPAN_RUN(OpTstep<Fast64>, TUPAN_LIGHT_SHAPES, K_TSTEP, slot);
     default:
@@ -858,8 +946,9 @@ int pairwise_call(int kernel, int mode, size_t ni, const T* const* ip, size_t nj
     a.ni = static_cast<int>(ni);
     a.nj = static_cast<int>(nj);
     a.eta = eta;
-    if (mode == TUPAN_MODE_SPLIT32 && kernel != K_ACC && kernel != K_ACC_JERK)
-        mode = TUPAN_MODE_FP64;  // split FP32 exists for the forces only
+    if (mode == TUPAN_MODE_SPLIT32 && kernel != K_ACC && kernel != K_ACC_JERK &&
+        kernel != K_TSTEP)
+        mode = TUPAN_MODE_FP64;  // split FP32 exists for the forces and Tstep
     return dispatch<T>(kernel, mode, a, static_cast<hipStream_t>(stream));
 }
 

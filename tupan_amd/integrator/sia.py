@@ -15,8 +15,16 @@ Methods 'siaNNx.dkd' / 'siaNNx.kdk' with x = 's' (shared fixed step),
 Tstep on its subset, splits it into slow (|tstep| > tau) and fast parts,
 advances the slow part with the scheme, recurses on the fast part with the
 scheme's drift substeps, and couples them by slow<->fast kicks
-(sia.py:1089-1123).  Subsets are device-resident copies (``ps[mask]``), as
-the reference's are numpy copies; joins append fast after slow.
+(sia.py:1089-1123).
+
+Subsets.  The reference splits with copies (``ps[mask]``) and joins by
+appending fast after slow, so after every level the particle order is
+[slow | fast], recursively.  By default this driver produces that order in
+place instead: ``partition`` stably permutes the level's range of the device
+arrays to [slow | fast] and hands out slice views of the two parts, so every
+kernel call sees the same particles in the same order as the reference's
+copies, no subset is ever copied out, and the join is a no-op.
+``SIA(..., inplace=False)`` runs the reference's copy/append form.
 """
 import logging
 
@@ -77,6 +85,22 @@ def split(ps, condition):
         logger.error("slow.n + fast.n != ps.n: %d, %d, %d.",
                      slow.n, fast.n, ps.n)
     return slow, fast
+
+
+def partition(ps, condition):
+    """In-place form of ``split``: stably permute ps's range of every array
+    to [slow | fast] and return slice views of the two parts.  ``ps`` must
+    be a system or a slice view of one whose arrays are all registered."""
+    n = ps.n
+    if n <= 2:
+        return ps, ps[n:n]
+    slow_idx = torch.nonzero(condition).squeeze(1)
+    ns = int(slow_idx.shape[0])
+    if 0 < ns < n:
+        perm = torch.cat([slow_idx, torch.nonzero(~condition).squeeze(1)])
+        for t in ps.arrays().values():
+            t.copy_(t.index_select(0, perm))
+    return ps[0:ns], ps[ns:n]
 
 
 def join(slow, fast):
@@ -161,8 +185,13 @@ class SIA(Base):
                         for s in sorted(COEFS) for v in "sah"
                         for t in ("dkd", "kdk")]
 
-    def __init__(self, eta, time, ps, method, **kwargs):
+    # outputs of the kernels the method calls, registered on the whole
+    # system up front so that slice views share them
+    AUX = ("tstepij", "ax", "ay", "az")
+
+    def __init__(self, eta, time, ps, method, inplace=True, **kwargs):
         super(SIA, self).__init__(eta, time, ps, **kwargs)
+        self.inplace = inplace
         self.method = method
         self.scheme = Scheme(method[:5])
         variant = method[5]
@@ -172,6 +201,10 @@ class SIA(Base):
             else self.scheme.kdk
 
     def do_step(self, ps, tau):
+        if self.inplace:
+            for name in self.AUX:
+                if name not in ps.__dict__:
+                    ps.register_auxiliary_attribute(name, "real")
         return self.recurse(ps, tau, self.evolve, self.scheme.bridge_sf)
 
     def recurse(self, ps, tau, evolve, bridge_sf):
@@ -184,7 +217,8 @@ class SIA(Base):
             ps.set_tstep(ps, self.eta)
             if self.shared_tstep:
                 tau = self.get_min_block_tstep(ps, tau)
-        slow, fast = split(ps, abs(ps.tstep) > flag * abs(tau))
+        cond = abs(ps.tstep) > flag * abs(tau)
+        slow, fast = (partition if self.inplace else split)(ps, cond)
         slow, fast = bridge_sf(slow, fast, tau, evolve, self.recurse)
         if fast.n == 0:
             type(ps).t_curr += tau
@@ -192,4 +226,4 @@ class SIA(Base):
             slow.tstep[...] = tau
             slow.time += tau
             slow.nstep += 1
-        return join(slow, fast)
+        return ps if self.inplace else join(slow, fast)

@@ -16,6 +16,7 @@ kernel-object protocol, constructed as ``cls(prec, name)``.
 PNAcc, NREG_X and NREG_V (post-Newtonian and NREG kernels) are not provided
 by this build; their singletons raise ``NotImplementedError`` when used.
 """
+import collections
 import logging
 import sys
 
@@ -83,6 +84,19 @@ def get_kernel(name, backend, prec, **kwargs):
     return Kernel(prec, name, **kwargs)
 
 
+# Interaction counter: None (off) or a Counter of n_i * n_j per extension
+# class, for throughput accounting (e.g. the pairs of one block step).
+pair_counts = None
+
+
+def count_pairs(enable=True):
+    """Start (reset) or stop counting n_i * n_j per kernel call; returns
+    the counter."""
+    global pair_counts
+    pair_counts = collections.Counter() if enable else None
+    return pair_counts
+
+
 def prepare_args(args, argtypes):
     return [argtype(arg) for (arg, argtype) in zip(args, argtypes)]
 
@@ -109,6 +123,8 @@ class AbstractExtension(object):
         return self.get_result()
 
     def _bind(self, ni, nj, inargs, outargs):
+        if pair_counts is not None:
+            pair_counts[type(self).__name__] += ni * nj
         self._inargs = inargs
         self._outargs = outargs
         self.inargs = prepare_args(self._inargs, self.argtypes)

@@ -24,7 +24,7 @@ KERNELS = {  # name -> (kernel id, modes)
     "acc": (_native.K_ACC, ("fp64", "split32")),
     "acc_jerk": (_native.K_ACC_JERK, ("fp64", "split32", "ieee64")),
     "snap_crackle": (_native.K_SNAP_CRACKLE, ("fp64",)),
-    "tstep": (_native.K_TSTEP, ("fp64",)),
+    "tstep": (_native.K_TSTEP, ("fp64", "split32")),
 }
 SLOT = {"fp64": 0, "split32": 1, "ieee64": 2}
 BUCKET_N = (1024, 16384, 65536)
