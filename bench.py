@@ -55,7 +55,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=65536)
-    ap.add_argument("--energy-steps", type=int, default=20)
+    ap.add_argument("--energy-steps", type=int, default=100)
     ap.add_argument("--sakura-n", type=int, default=16384)
     ap.add_argument("--ensemble-systems", type=int, default=4096)
     ap.add_argument("--sia-n", type=int, default=262144,
@@ -191,7 +191,22 @@ def ensemble_rate(nsys, lanes):
             "solves_per_s": solves / (kms / 1e3),
             "pair_drifts_per_s": 2 * drv.nsteps * pairs / (kms / 1e3),
             "iterations_per_solve": iters / max(solves, 1),
-            "max_dE_E": drv.max_rel_energy_error(e0)}
+            "dE_E_quantiles": energy_quantiles(drv, e0)}
+
+
+def energy_quantiles(drv, e0):
+    """|dE/E| over the systems: median, 90th, 99th percentile and maximum.
+    Sakura's pairwise splitting is not exact for three-body encounters, so
+    a few few-body systems with eps = 0 lose energy conservation (the
+    reference algorithm on the CPU oracle gives median 4e-5 and maximum 0.6
+    over the first 600 of these systems)."""
+    import torch
+    e1 = drv.energies()
+    err = torch.abs((e1 - e0) / e0).cpu().numpy()
+    return {"median": float(np.median(err)),
+            "p90": float(np.percentile(err, 90)),
+            "p99": float(np.percentile(err, 99)),
+            "max": float(err.max())}
 
 
 def block_step_rate(n, tau, steps, modes):

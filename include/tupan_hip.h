@@ -280,6 +280,63 @@ int tupan_kepler_pairs(size_t npairs, const int *idx,
                        const double *e2, double *vx, double *vy, double *vz,
                        double dt, int *status, void *stream);
 
+/* Sakura with the options of the two-pass form (no counterpart in the
+ * reference; same arithmetic as tupan_sakura, sakura_kernel.c:5-63).
+ * Fields are the 8 input arrays (m, rx, ry, rz, e2, vx, vy, vz) of each side
+ * and the 6 outputs (drx, dry, drz, dvx, dvy, dvz), device pointers.
+ *   path   0: automatic (two-pass for ni * nj >= 2^22), 1: one-pass,
+ *          2: two-pass (far pairs in the tile loop, near pairs compacted
+ *          into a dense list and solved one per thread)
+ *   net    packed Kepler-guess network (tupan_mlp_pack) or NULL: the first
+ *          solve of each near pair starts from the network's guess and
+ *          falls back to the plain guess after 8 Laguerre steps
+ *   log    NULL or a device log of the near pairs' first solves
+ *   counters  NULL or 5 device counters: solves, Laguerre steps,
+ *          subdivisions, leapfrog pairs, guess fallbacks
+ *   near_pairs  NULL or host int: near pairs listed (-1: one-pass form)
+ * net or log imply the two-pass form.  One host synchronisation per call
+ * (the near-pair count). */
+typedef struct {
+    float *features;  /* [4][capacity]: tau, sigma, kappa, eps (kepler.h) */
+    float *target;    /* [capacity]: s r / dt - 1 of the converged first
+                         solve, NaN if it did not converge undivided     */
+    int *iterations;  /* [capacity]: Laguerre steps of the pair           */
+    int capacity;
+} tupan_solve_log;
+
+int tupan_sakura_ex(size_t ni, const double *const *ifields, size_t nj,
+                    const double *const *jfields, double dt, long flag,
+                    double *const *out, int path, const void *net,
+                    const tupan_solve_log *log, unsigned long long *counters,
+                    int *near_pairs, int *status, void *stream);
+
+/* ---------------------------------------------------------------------------
+ * Kepler-guess network (no counterpart in the reference): 4 -> 64 -> 64 -> 1
+ * perceptron with SiLU, inputs x = (log tau, sigma, kappa, eps) of
+ * tupan_solve_log standardised by norm[0:4] (mean) and norm[4:8] (1/std),
+ * output y mapped to delta = y * norm[8] + norm[9].  Parameters: FP32, flat
+ * W1[64][4] b1[64] W2[64][64] b2[64] W3[64] b3.  All arrays device memory.
+ * ------------------------------------------------------------------------- */
+int tupan_mlp_sizes(int *nparams, int *nnorm, int *packed_bytes);
+/* Mean squared error of y against (target - norm[9]) / norm[8] over n
+ * samples and its gradient (FP32, deterministic). */
+int tupan_mlp_grad(int n, const float *features, const float *target,
+                   const float *params, const float *norm, float *grad,
+                   float *loss, void *stream);
+/* FP32 forward: delta for n samples. */
+int tupan_mlp_forward(int n, const float *features, const float *params,
+                      const float *norm, float *delta, void *stream);
+/* One Adam step (step counts from 1). */
+int tupan_mlp_adam(float *params, const float *grad, float *m1, float *m2,
+                   float lr, float beta1, float beta2, float eps, int step,
+                   void *stream);
+/* BF16 inference block (tupan_mlp_sizes: packed_bytes) from FP32 params. */
+int tupan_mlp_pack(const float *params, const float *norm, void *packed,
+                   void *stream);
+/* BF16 inference with FP32 accumulation (matrix cores for the 64x64 layer). */
+int tupan_mlp_infer(int n, const float *features, const void *packed,
+                    float *delta, void *stream);
+
 /* ---------------------------------------------------------------------------
  * Ensembles of independent few-body systems (no counterpart in the reference,
  * which integrates one system per process; the per-pair arithmetic is

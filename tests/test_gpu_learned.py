@@ -1,0 +1,170 @@
+"""Two-pass Sakura form and the learned Kepler guess on the GPU.
+
+  * two-pass vs one-pass Sakura: the same pair drifts summed far-first, so
+    per-particle results agree to reassociation rounding (<= 1e-13 of each
+    particle's vector norm) and the work counters are identical; the golden
+    Sakura vectors hold at the one-pass bar (1e-11) through the two-pass
+    form, including lists that overflow their slots;
+  * network kernels against the float64 CPU oracle (oracle/mlp.py): FP32
+    forward <= 1e-5, loss and gradient <= 1e-4 of the largest gradient
+    component, Adam step, BF16 inference within 2e-3 of the oracle's BF16
+    emulation (MFMA accumulation order) in units of the output scale;
+  * the learned guess: trained on solves logged from Sakura steps, it
+    leaves converged results unchanged (<= 1e-12 per particle, the
+    Laguerre tolerance 2^-42 relative to s) and its guesses are closer to
+    the converged anomaly than the plain guess; the iteration counts of
+    both are printed.
+"""
+import numpy as np
+import pytest
+
+from conftest import vec_rel_err
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+F8 = ("mass", "rx", "ry", "rz", "eps2", "vx", "vy", "vz")
+
+
+def sakura_run(ps, dt, flag, path, guess=None, counters=True):
+    from tupan_amd.learned import sakura_kernel
+    ext = sakura_kernel(path=path, guess=guess)
+    if counters:
+        ext.kernel.counters = torch.zeros(5, dtype=torch.int64,
+                                          device="cuda:0")
+    res = ext.calc(ps, ps, dt, flag)
+    out = [r.detach().cpu().numpy().copy() for r in res]
+    cnt = ext.kernel.counters.cpu().tolist() if counters else None
+    return out, cnt, ext.kernel.near_pairs
+
+
+@pytest.fixture(scope="module")
+def plummer4k():
+    from oracle_backend import install_hip
+    from tupan_amd.ics.plummer import make_plummer
+    install_hip("float64")
+    return make_plummer(4096, 0.0, ("equalmass",), seed=5, device="cuda:0")
+
+
+@pytest.mark.parametrize("flag", [-1, 1, -2, 2])
+def test_two_pass_equals_one_pass(plummer4k, flag):
+    one, c1, n1 = sakura_run(plummer4k, 1.0 / 64, flag, "one-pass")
+    two, c2, n2 = sakura_run(plummer4k, 1.0 / 64, flag, "two-pass")
+    assert n1 == -1 and n2 > 0
+    assert vec_rel_err(two[:3], one[:3]) <= 1e-13
+    assert vec_rel_err(two[3:], one[3:]) <= 1e-13
+    assert c1[:4] == c2[:4], (c1, c2)
+
+
+def test_two_pass_golden_and_overflow(golden):
+    """Golden Sakura sets through the two-pass form; the few-body set has
+    every pair near, so lists overflow their 16 slots."""
+    from tupan_amd.particles import ParticleSystem
+    g = golden("sakura_f64.npz")
+    checked = 0
+    for tag in ("nb", "r0", "r1"):
+        d = {k: g["%s_%s" % (tag, k)] for k in F8}
+        ps = ParticleSystem.from_numpy(d, device="cuda:0")
+        for flag in (-1, 1, -2, 2):
+            for dt in (1.0 / 64, 0.5):
+                key = "%s_f%d_dt%g" % (tag, flag, dt)
+                ref = [g["%s_%s" % (key, o)] for o in ("drx", "dry", "drz",
+                                                       "dvx", "dvy", "dvz")]
+                two, _, _ = sakura_run(ps, dt, flag, "two-pass")
+                assert vec_rel_err(two[:3], ref[:3]) <= 1e-11, (key,)
+                assert vec_rel_err(two[3:], ref[3:]) <= 1e-11, (key,)
+                checked += 1
+    assert checked == 24
+
+
+def test_dense_cluster_overflow_equals_one_pass():
+    from tupan_amd.particles import ParticleSystem
+    rng = np.random.default_rng(4)
+    n = 300
+    d = {k: rng.normal(0, 1, n) for k in ("rx", "ry", "rz")}
+    d.update({k: rng.normal(0, 0.05, n) for k in ("vx", "vy", "vz")})
+    d["mass"] = np.full(n, 1.0 / n)
+    d["eps2"] = np.zeros(n)
+    ps = ParticleSystem.from_numpy(d, device="cuda:0")
+    one, c1, _ = sakura_run(ps, 1.0 / 32, -1, "one-pass")
+    two, c2, near = sakura_run(ps, 1.0 / 32, -1, "two-pass")
+    assert c1[0] > 16 * n  # most pairs are Kepler pairs: lists overflow
+    assert near < c1[0]
+    assert vec_rel_err(two[:3], one[:3]) <= 1e-13
+    assert vec_rel_err(two[3:], one[3:]) <= 1e-13
+    assert c1[:4] == c2[:4]
+
+
+def random_net_case(n, seed):
+    rng = np.random.RandomState(seed)
+    params = (rng.normal(0, 0.25, 4545)).astype(np.float32)
+    feats = np.stack([np.exp(rng.uniform(-6, 1, n)), rng.normal(0, 1, n),
+                      rng.uniform(0, 4, n), rng.uniform(0, 0.3, n)])
+    feats = feats.astype(np.float32)
+    norm = np.array([-2.5, 0.1, 1.5, 0.1, 0.6, 1.1, 0.8, 3.0, 0.02, -0.01],
+                    dtype=np.float32)
+    target = rng.normal(-0.01, 0.02, n).astype(np.float32)
+    return params, norm, feats, target
+
+
+def test_network_kernels_vs_oracle():
+    from oracle import mlp
+    from tupan_amd.learned import KeplerNet
+    params, norm, feats, target = random_net_case(1000, 7)
+    net = KeplerNet(params=params, norm=norm)
+    f = torch.as_tensor(feats, device="cuda:0").contiguous()
+    t = torch.as_tensor(target, device="cuda:0")
+    ref = mlp.delta(params.astype(np.float64), norm, feats)
+    got = net.forward(f).cpu().numpy()
+    scale = np.max(np.abs(ref - norm[9]))
+    assert np.max(np.abs(got - ref)) <= 1e-5 * scale
+    loss, grad = net.loss_and_grad(f, t)
+    rl, rg = mlp.loss_and_grad(params.astype(np.float64), norm, feats, target)
+    assert abs(float(loss) - rl) <= 1e-5 * rl
+    g = grad.cpu().numpy()
+    assert np.max(np.abs(g - rg)) <= 1e-4 * np.max(np.abs(rg))
+    bf = net.infer(f).cpu().numpy()
+    emu = mlp.delta_bf16(params, norm, feats)
+    assert np.max(np.abs(bf - emu)) <= 2e-3 * scale, \
+        np.max(np.abs(bf - emu)) / scale
+    # one Adam step
+    p0 = params.astype(np.float64)
+    net.adam_step(1e-3)
+    want, _, _ = mlp.adam(p0, g.astype(np.float64), 0, 0, 1e-3, 0.9, 0.999,
+                          1e-8, 1)
+    np.testing.assert_allclose(net.params.cpu().numpy(), want, rtol=1e-6,
+                               atol=1e-7)
+
+
+def test_learned_guess_gain_and_identical_results():
+    from tupan_amd.ics.plummer import make_plummer
+    from tupan_amd.learned import KeplerNet, collect_trajectory
+    train_ps = make_plummer(4096, 0.0, ("equalmass",), seed=11,
+                            device="cuda:0")
+    feats, target, iters = collect_trajectory(train_ps, 6)
+    assert feats.shape[1] > 10000
+    net = KeplerNet(seed=1)
+    net.fit_norm(feats, target)
+    hist = net.train(feats, target, epochs=6, batch=4096, lr=3e-3,
+                     lr_end=3e-4)
+    print("training loss per epoch", ["%.4f" % h for h in hist])
+    assert hist[-1] < 0.5 * hist[0]
+
+    test_ps = make_plummer(4096, 0.0, ("equalmass",), seed=12,
+                           device="cuda:0")
+    plain, cp, near = sakura_run(test_ps, 1.0 / 128, -1, "two-pass")
+    learned, cl, near2 = sakura_run(test_ps, 1.0 / 128, -1, "two-pass",
+                                    guess=net)
+    assert near == near2 and cp[0] == cl[0]
+    print("near pairs %d; Laguerre steps per solve: plain %.3f learned %.3f;"
+          " fallbacks %d" % (near, cp[1] / cp[0], cl[1] / cl[0], cl[4]))
+    assert vec_rel_err(learned[:3], plain[:3]) <= 1e-12
+    assert vec_rel_err(learned[3:], plain[3:]) <= 1e-12
+    # guess quality on the test state's own solves
+    f2, t2, _ = collect_trajectory(test_ps, 1)
+    pred = net.infer(f2)
+    err_learned = torch.median(torch.abs(pred - t2)).item()
+    err_plain = torch.median(torch.abs(t2)).item()
+    print("median |delta - guess|: plain %.3e learned %.3e"
+          % (err_plain, err_learned))
+    assert err_learned < err_plain

@@ -31,6 +31,12 @@ class TupanHipError(RuntimeError):
     pass
 
 
+class SolveLog(ctypes.Structure):
+    """tupan_solve_log (include/tupan_hip.h)."""
+    _fields_ = [("features", ctypes.c_void_p), ("target", ctypes.c_void_p),
+                ("iterations", ctypes.c_void_p), ("capacity", ctypes.c_int)]
+
+
 def _declare(lib):
     lib.tupan_last_error.restype = ctypes.c_char_p
     lib.tupan_last_error.argtypes = []
@@ -76,6 +82,24 @@ def _declare(lib):
     lib.tupan_kepler_pairs.restype = ctypes.c_int
     lib.tupan_kepler_pairs.argtypes = ([_S, _P] + [_P] * 8
                                        + [ctypes.c_double, _P, _P])
+    lib.tupan_sakura_ex.restype = ctypes.c_int
+    lib.tupan_sakura_ex.argtypes = [_S, _PP, _S, _PP, ctypes.c_double,
+                                    ctypes.c_long, _PP, ctypes.c_int, _P,
+                                    ctypes.POINTER(SolveLog), _P,
+                                    ctypes.POINTER(ctypes.c_int), _P, _P]
+    lib.tupan_mlp_sizes.restype = ctypes.c_int
+    lib.tupan_mlp_sizes.argtypes = [ctypes.POINTER(ctypes.c_int)] * 3
+    lib.tupan_mlp_grad.restype = ctypes.c_int
+    lib.tupan_mlp_grad.argtypes = [ctypes.c_int] + [_P] * 7
+    lib.tupan_mlp_forward.restype = ctypes.c_int
+    lib.tupan_mlp_forward.argtypes = [ctypes.c_int] + [_P] * 5
+    lib.tupan_mlp_adam.restype = ctypes.c_int
+    lib.tupan_mlp_adam.argtypes = ([_P] * 4 + [ctypes.c_float] * 4
+                                   + [ctypes.c_int, _P])
+    lib.tupan_mlp_pack.restype = ctypes.c_int
+    lib.tupan_mlp_pack.argtypes = [_P] * 4
+    lib.tupan_mlp_infer.restype = ctypes.c_int
+    lib.tupan_mlp_infer.argtypes = [ctypes.c_int] + [_P] * 4
     return lib
 
 
@@ -143,4 +167,6 @@ DEVICE_ABI = ("tupan_last_error", "tupan_version", "tupan_device_info",
               "tupan_host_sakura_f32", "tupan_host_kepler_f32",
               "tupan_ensemble_create", "tupan_ensemble_destroy",
               "tupan_ensemble_info", "tupan_ensemble_sakura",
-              "tupan_ensemble_phi")
+              "tupan_ensemble_phi", "tupan_sakura_ex", "tupan_mlp_sizes",
+              "tupan_mlp_grad", "tupan_mlp_forward", "tupan_mlp_adam",
+              "tupan_mlp_pack", "tupan_mlp_infer")

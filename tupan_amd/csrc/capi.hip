@@ -50,25 +50,26 @@ Device& device_slot()
 
 const Device& current_device() { return device_slot(); }
 
-void* workspace(size_t bytes, hipStream_t)
+void* workspace(size_t bytes, hipStream_t, int slot)
 {
+    if (slot < 0 || slot > 3) return nullptr;
     Device& d = device_slot();
-    if (d.ws_bytes >= bytes) return d.ws;
+    if (d.ws_bytes[slot] >= bytes) return d.ws[slot];
     size_t want = bytes + bytes / 4;
-    if (d.ws) {
+    if (d.ws[slot]) {
         // earlier launches may still read the old buffer
         (void)hipDeviceSynchronize();
-        (void)hipFree(d.ws);
-        d.ws = nullptr;
-        d.ws_bytes = 0;
+        (void)hipFree(d.ws[slot]);
+        d.ws[slot] = nullptr;
+        d.ws_bytes[slot] = 0;
     }
-    if (hipMalloc(&d.ws, want) != hipSuccess) {
+    if (hipMalloc(&d.ws[slot], want) != hipSuccess) {
         (void)hipGetLastError();
-        d.ws = nullptr;
+        d.ws[slot] = nullptr;
         return nullptr;
     }
-    d.ws_bytes = want;
-    return d.ws;
+    d.ws_bytes[slot] = want;
+    return d.ws[slot];
 }
 
 static void* staging(size_t bytes)
@@ -581,6 +582,34 @@ int tupan_sakura(size_t ni, const double* im, const double* irx, const double* i
     const double* jp[8] = {jm, jrx, jry, jrz, je2, jvx, jvy, jvz};
     double* op[6] = {idrx, idry, idrz, idvx, idvy, idvz};
     return sakura_call(ni, ip, nj, jp, dt, flag, op, counters, status, stream);
+}
+
+int tupan_sakura_ex(size_t ni, const double* const* ifields, size_t nj,
+                    const double* const* jfields, double dt, long flag, double* const* out,
+                    int path, const void* net, const tupan_solve_log* log,
+                    unsigned long long* counters, int* near_pairs, int* status, void* stream)
+{
+    if (path < 0 || path > 2) return set_error(TUPAN_ERR_ARG, "path must be 0, 1 or 2");
+    if (!ifields || !jfields || !out) return set_error(TUPAN_ERR_ARG, "null field list");
+    SakuraOptions opt;
+    opt.path = path;
+    opt.net = net;
+    if (log) {
+        if (log->capacity < 0 || (log->capacity > 0 &&
+                                  (!log->features || !log->target || !log->iterations)))
+            return set_error(TUPAN_ERR_ARG, "bad solve log");
+        if (log->capacity > 0) {
+            opt.log.features = log->features;
+            opt.log.target = log->target;
+            opt.log.iterations = log->iterations;
+            opt.log.capacity = log->capacity;
+        }
+    }
+    opt.fallbacks = counters ? counters + 4 : nullptr;
+    const int rc = sakura_call_ex(ni, ifields, nj, jfields, dt, flag, out, opt, counters,
+                                  status, stream);
+    if (near_pairs) *near_pairs = opt.near_pairs;
+    return rc;
 }
 
 int tupan_kepler_pairs(size_t npairs, const int* idx, const double* m, double* rx, double* ry,

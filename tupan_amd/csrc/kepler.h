@@ -38,6 +38,7 @@ struct Counters {
     unsigned int iters = 0;
     unsigned int retries = 0;
     unsigned int leapfrog = 0;
+    unsigned int fallbacks = 0;
     int fail = 0;
 };
 
@@ -87,7 +88,8 @@ __device__ __forceinline__ Stumpff stumpff(double s, double alpha)
 // Laguerre iteration of order 5 on Kepler's equation in s (:225-254).
 // k = {dt, r0, r0.v0, m, alpha}.  Returns 0, -1 (h == 0), -2 (MAXITER),
 // -3 (sign of s changed).
-__device__ inline int laguerre5(double x0, double& x, const double* k, Counters& cnt)
+__device__ inline int laguerre5(double x0, double& x, const double* k, Counters& cnt,
+                                int max_iter)
 {
     int it = 0;
     double delta;
@@ -108,7 +110,7 @@ __device__ inline int laguerre5(double x0, double& x, const double* k, Counters&
         delta = -g / h;
         x += delta;
         it += 1;
-        if (it > kMaxIter) return -2;
+        if (it > max_iter) return -2;
     } while (fabs(delta) > kTol);
     if (sgn(x) != sgn(x0)) return -3;
     return 0;
@@ -138,15 +140,31 @@ __device__ inline void lagrange_update(double dt, double s, double r0, double m,
     v[2] = z0 * ldf + t0 * ldg;
 }
 
-// One solve over dt0 (:360-463).  p, v in/out.
-__device__ inline int kepler_once(double dt0, double m, double e2, double* p, double* v,
-                                  Counters& cnt)
+// Dimensionless description of one solve, the inputs of a learned initial
+// guess: with r the (softened) separation, tau = dt sqrt(m / r^3) (dt after
+// the period reduction), sigma = r.v / sqrt(m r), kappa = v^2 r / m,
+// eps = e2 / (r^2 + e2); the target is delta = s r / dt - 1 for the converged
+// universal anomaly s (0 when s = dt / r, the plain guess).
+struct SolveFeatures {
+    float tau, sigma, kappa, eps;
+};
+
+// Learned guess for one solve: use = 1 replaces the plain s0 by
+// (dt / r) (1 + delta); the Laguerre iteration then gets kLearnedIter steps
+// before it restarts from the plain s0 (fallback counted in cnt.fallbacks).
+struct Guess {
+    int use = 0;
+    double delta = 0.0;
+};
+constexpr int kLearnedIter = 8;
+
+__device__ __forceinline__ bool solve_setup(double dt0, double m, double e2, const double* p,
+                                            const double* v, double k[5], double& s0,
+                                            SolveFeatures* f)
 {
     double r2 = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
-    if (!(r2 > 0)) return 0;
-    if (cnt.budget == 0) return -4;
-    cnt.budget--;
-    cnt.solves++;
+    if (!(r2 > 0)) return false;
+    const double rr2 = r2;
     r2 += e2;
     const double r = sqrt(r2);
     const double v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
@@ -162,24 +180,54 @@ __device__ inline int kepler_once(double dt0, double m, double e2, double* p, do
         dt = (ratio - static_cast<double>(static_cast<long>(ratio))) * T;
     }
 
-    double s0 = dt / r;
+    s0 = dt / r;
     if (alpha > 0) {
         // Bate, Mueller & White (1971) eq. 4.5.11 guess for hyperbolae
         const double sal = sqrt(alpha);
         const double ss = fabs(2 * alpha * dt / (rv + (m + alpha * r) / sal));
         if (ss > 1) s0 = sgn(dt) * log(ss) / sal;
     }
+    k[0] = dt; k[1] = r; k[2] = rv; k[3] = m; k[4] = alpha;
+    if (f) {
+        f->tau = static_cast<float>(dt * sqrt(m / (r2 * r)));
+        f->sigma = static_cast<float>(rv / sqrt(m * r));
+        f->kappa = static_cast<float>(v2 * r / m);
+        f->eps = static_cast<float>(e2 / (rr2 + e2));
+    }
+    return true;
+}
 
-    const double k[5] = {dt, r, rv, m, alpha};
+// One solve over dt0 (:360-463).  p, v in/out.  With a learned guess the
+// iteration starts from it and falls back to the plain s0 after
+// kLearnedIter steps without convergence (or on any solver error).
+__device__ inline int kepler_once(double dt0, double m, double e2, double* p, double* v,
+                                  Counters& cnt, const Guess* guess = nullptr,
+                                  double* s_out = nullptr)
+{
+    double k[5], s0;
+    if (!solve_setup(dt0, m, e2, p, v, k, s0, nullptr)) return 0;
+    if (cnt.budget == 0) return -4;
+    cnt.budget--;
+    cnt.solves++;
     double s;
-    const int err = laguerre5(s0, s, k, cnt);
-    if (err == 0) lagrange_update(dt, s, r, m, e2, alpha, p, v);
+    int err = 1;
+    if (guess && guess->use) {
+        const double sg = (k[0] / k[1]) * (1 + guess->delta);
+        if (sgn(sg) == sgn(s0)) err = laguerre5(sg, s, k, cnt, kLearnedIter);
+        if (err != 0) cnt.fallbacks++;
+    }
+    if (err != 0) err = laguerre5(s0, s, k, cnt, kMaxIter);
+    if (err == 0) {
+        if (s_out) *s_out = s;
+        lagrange_update(k[0], s, k[1], m, e2, k[4], p, v);
+    }
     return err;
 }
 
 // Retry with n = 2, 4, ... equal substeps on any solver error (:466-507).
 __device__ inline void kepler_subdivided(double dt, double m, double e2, const double* p0,
-                                         const double* v0, double* p, double* v, Counters& cnt)
+                                         const double* v0, double* p, double* v, Counters& cnt,
+                                         const Guess* guess = nullptr, double* s_out = nullptr)
 {
     for (int lvl = 0; lvl <= kMaxDoublings; ++lvl) {
         const long n = 1L << lvl;
@@ -187,7 +235,11 @@ __device__ inline void kepler_subdivided(double dt, double m, double e2, const d
         p[0] = p0[0]; p[1] = p0[1]; p[2] = p0[2];
         v[0] = v0[0]; v[1] = v0[1]; v[2] = v0[2];
         for (long i = 0; i < n; ++i) {
-            if (kepler_once(dt / n, m, e2, p, v, cnt) != 0) {
+            // only the undivided first solve uses the learned guess, and
+            // only it reports its converged anomaly
+            const bool first = lvl == 0;
+            if (kepler_once(dt / n, m, e2, p, v, cnt, first ? guess : nullptr,
+                            first ? s_out : nullptr) != 0) {
                 ok = false;
                 break;
             }
@@ -201,11 +253,12 @@ __device__ inline void kepler_subdivided(double dt, double m, double e2, const d
 
 // Full solver with the softened-energy check and step doubling (:510-614).
 __device__ inline void kepler_solve(double dt, double m, double e2, const double* p0,
-                                    const double* v0, double* p1, double* v1, Counters& cnt)
+                                    const double* v0, double* p1, double* v1, Counters& cnt,
+                                    const Guess* guess = nullptr, double* s_out = nullptr)
 {
     cnt.budget = kSolveBudget;
     double p[3], v[3];
-    kepler_subdivided(dt, m, e2, p0, v0, p, v, cnt);
+    kepler_subdivided(dt, m, e2, p0, v0, p, v, cnt, guess, s_out);
     if (e2 != 0) {
         double r2 = p0[0] * p0[0] + p0[1] * p0[1] + p0[2] * p0[2];
         if (!(r2 > 0)) {
@@ -291,7 +344,8 @@ __device__ inline void leapfrog(double dt, double m, double e2, double* p, doubl
 
 // Far pairs (r > 64 m / v^2) take one leapfrog step (:94-123).
 __device__ inline void twobody(double dt, double m, double e2, double* p, double* v,
-                               Counters& cnt)
+                               Counters& cnt, const Guess* guess = nullptr,
+                               double* s_out = nullptr)
 {
     const double r2 = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
     const double v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
@@ -301,29 +355,89 @@ __device__ inline void twobody(double dt, double m, double e2, double* p, double
     } else {
         const double p0[3] = {p[0], p[1], p[2]};
         const double v0[3] = {v[0], v[1], v[2]};
-        kepler_solve(dt, m, e2, p0, v0, p, v, cnt);
+        kepler_solve(dt, m, e2, p0, v0, p, v, cnt, guess, s_out);
     }
 }
 
 // evolve_twobody flags (:126-191); any other flag leaves the pair unchanged.
+// A learned guess (and the anomaly report) serves the first twobody stage.
 __device__ inline void drift_twobody(double dt, long flag, double m, double e2, double* p,
-                                     double* v, Counters& cnt)
+                                     double* v, Counters& cnt, const Guess* guess = nullptr,
+                                     double* s_out = nullptr)
 {
     if (flag == -1) {
         p[0] -= v[0] * dt; p[1] -= v[1] * dt; p[2] -= v[2] * dt;
-        twobody(dt, m, e2, p, v, cnt);
+        twobody(dt, m, e2, p, v, cnt, guess, s_out);
     } else if (flag == 1) {
-        twobody(dt, m, e2, p, v, cnt);
+        twobody(dt, m, e2, p, v, cnt, guess, s_out);
         p[0] -= v[0] * dt; p[1] -= v[1] * dt; p[2] -= v[2] * dt;
     } else if (flag == -2) {
         p[0] -= v[0] * dt / 2; p[1] -= v[1] * dt / 2; p[2] -= v[2] * dt / 2;
-        twobody(dt, m, e2, p, v, cnt);
+        twobody(dt, m, e2, p, v, cnt, guess, s_out);
         p[0] -= v[0] * dt / 2; p[1] -= v[1] * dt / 2; p[2] -= v[2] * dt / 2;
     } else if (flag == 2) {
-        twobody(dt / 2, m, e2, p, v, cnt);
+        twobody(dt / 2, m, e2, p, v, cnt, guess, s_out);
         p[0] -= v[0] * dt; p[1] -= v[1] * dt; p[2] -= v[2] * dt;
         twobody(dt / 2, m, e2, p, v, cnt);
     }
+}
+
+// Far-pair part of drift_twobody: runs the flag's stages as long as every
+// twobody stage is a far pair (leapfrog) and returns true with p, v
+// advanced; returns false (p, v partly advanced, to be discarded) at the
+// first stage that needs a Kepler solve.  The leapfrogs of a pair that
+// returns false are not counted.
+__device__ inline bool far_stage(double dt, double m, double e2, double* p, double* v,
+                                 Counters& cnt)
+{
+    const double r2 = p[0] * p[0] + p[1] * p[1] + p[2] * p[2];
+    const double v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    const double R = 64 * (m / v2);
+    if (!(r2 > R * R)) return false;
+    leapfrog(dt, m, e2, p, v, cnt);
+    return true;
+}
+
+__device__ inline bool drift_twobody_far(double dt, long flag, double m, double e2, double* p,
+                                         double* v, Counters& cnt)
+{
+    Counters c;
+    bool ok = true;
+    if (flag == -1) {
+        p[0] -= v[0] * dt; p[1] -= v[1] * dt; p[2] -= v[2] * dt;
+        ok = far_stage(dt, m, e2, p, v, c);
+    } else if (flag == 1) {
+        ok = far_stage(dt, m, e2, p, v, c);
+        if (ok) { p[0] -= v[0] * dt; p[1] -= v[1] * dt; p[2] -= v[2] * dt; }
+    } else if (flag == -2) {
+        p[0] -= v[0] * dt / 2; p[1] -= v[1] * dt / 2; p[2] -= v[2] * dt / 2;
+        ok = far_stage(dt, m, e2, p, v, c);
+        if (ok) { p[0] -= v[0] * dt / 2; p[1] -= v[1] * dt / 2; p[2] -= v[2] * dt / 2; }
+    } else if (flag == 2) {
+        ok = far_stage(dt / 2, m, e2, p, v, c);
+        if (ok) {
+            p[0] -= v[0] * dt; p[1] -= v[1] * dt; p[2] -= v[2] * dt;
+            ok = far_stage(dt / 2, m, e2, p, v, c);
+        }
+    }
+    if (ok) cnt.leapfrog += c.leapfrog;
+    return ok;
+}
+
+// State of a pair's first twobody stage (the one a learned guess serves):
+// the relative position after the flag's pre-drift and that stage's dt.
+__device__ inline double first_stage(double dt, long flag, double* p, const double* v)
+{
+    if (flag == -1) {
+        p[0] -= v[0] * dt; p[1] -= v[1] * dt; p[2] -= v[2] * dt;
+        return dt;
+    }
+    if (flag == -2) {
+        p[0] -= v[0] * dt / 2; p[1] -= v[1] * dt / 2; p[2] -= v[2] * dt / 2;
+        return dt;
+    }
+    if (flag == 2) return dt / 2;
+    return dt;
 }
 
 }  // namespace kep

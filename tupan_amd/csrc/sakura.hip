@@ -28,7 +28,17 @@ struct SakuraArgs {
     long flag;
     unsigned long long* counters;
     int* status;
+    // two-pass form (null in the one-pass form): per (j-split, i) the count
+    // of deferred near pairs and their j indices (kNearSlots slots each)
+    int* near_count;
+    int* near_j;
 };
+
+// Near pairs listed per (j-split, i); further ones are solved by
+// near_overflow_kernel.
+constexpr int kNearSlots = 16;
+// Calls with at least this many pairs take the two-pass form by default.
+constexpr double kTwoPassPairs = 4194304.0;
 
 // Wave-level sum so that one lane per wave issues the counter atomics.
 __device__ __forceinline__ unsigned int wave_sum(unsigned int v)
@@ -152,9 +162,322 @@ __global__ __launch_bounds__(BLOCK) void sakura_kernel_dev(SakuraArgs a)
     if (a.status && valid && acc.c.fail) atomicOr(a.status, 1);
 }
 
+// Counter flush shared by the kernels: wave sums, one atomic per wave.
+__device__ __forceinline__ void flush_counters(unsigned long long* counters, bool valid,
+                                               const kep::Counters& c, int tid)
+{
+    if (!counters) return;
+    const unsigned int s = wave_sum(valid ? c.solves : 0u);
+    const unsigned int it = wave_sum(valid ? c.iters : 0u);
+    const unsigned int rt = wave_sum(valid ? c.retries : 0u);
+    const unsigned int lf = wave_sum(valid ? c.leapfrog : 0u);
+    if ((tid & (kWave - 1)) == 0) {
+        atomicAdd(counters + 0, static_cast<unsigned long long>(s));
+        atomicAdd(counters + 1, static_cast<unsigned long long>(it));
+        atomicAdd(counters + 2, static_cast<unsigned long long>(rt));
+        atomicAdd(counters + 3, static_cast<unsigned long long>(lf));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Two-pass form.  Lanes of one wave hold 64 different i against the same j,
+// so in the one-pass kernel a single near pair in a wave makes the other 63
+// lanes wait through its Kepler solve.  Pass 1 runs the tile loop with far
+// pairs only (leapfrog stages) and records the near pairs; the near pairs
+// are then compacted into one dense list and solved one per thread (pass 2);
+// pass 3 adds each (j-split, i)'s near contributions, in j order, to its far
+// partial sum, and the split reduction finishes as in the one-pass form.
+// Per (split, i) the far pairs are summed first, then the near pairs, each
+// in j order: deterministic, and equal to the one-pass sums up to that
+// reassociation.
+// ---------------------------------------------------------------------------
+template <int BLOCK, int TILE>
+__global__ __launch_bounds__(BLOCK) void sakura_far_kernel(SakuraArgs a)
+{
+    using Op = OpSakura;
+    __shared__ Op::Slot tile[TILE];
+    const GridPos g = grid_pos(a.nsplit);
+    const int tid = static_cast<int>(threadIdx.x);
+    const int i = g.iblock * BLOCK + tid;
+    const bool valid = i < a.ni;
+    Op::IReg r;
+    Op::Acc acc;
+    Op::load_i(a, {}, valid ? i : a.ni - 1, r);
+    Op::zero(acc);
+    const size_t entry = static_cast<size_t>(g.jsplit) * a.ni + (valid ? i : 0);
+    int* slots = a.near_j + entry * kNearSlots;
+    int ndef = 0;
+    const int j0 = g.jsplit * a.jchunk;
+    const int j1 = min(a.nj, j0 + a.jchunk);
+    for (int jt = j0; jt < j1; jt += TILE) {
+        const int cnt = min(TILE, j1 - jt);
+        __syncthreads();
+        for (int k = tid; k < cnt; k += BLOCK) Op::load_j(a, {}, jt + k, tile[k]);
+        __syncthreads();
+        if (!valid) continue;
+        for (int k = 0; k < cnt; ++k) {
+            const Op::Slot& sj = tile[k];
+            const double p0[3] = {r.x - sj.v[1], r.y - sj.v[2], r.z - sj.v[3]};
+            const double e2 = r.e2 + sj.v[4];
+            const double v0[3] = {r.vx - sj.v[5], r.vy - sj.v[6], r.vz - sj.v[7]};
+            const double m = r.m + sj.v[0];
+            double p[3] = {p0[0], p0[1], p0[2]};
+            double v[3] = {v0[0], v0[1], v0[2]};
+            if (!kep::drift_twobody_far(r.dt, r.flag, m, e2, p, v, acc.c)) {
+                // near pair: the first kNearSlots go to the list, the rest
+                // to near_overflow_kernel
+                if (ndef < kNearSlots) slots[ndef] = jt + k;
+                ++ndef;
+                continue;
+            }
+            const double mu = sj.v[0] / m;
+            acc.d[0] += mu * (p[0] - p0[0]);
+            acc.d[1] += mu * (p[1] - p0[1]);
+            acc.d[2] += mu * (p[2] - p0[2]);
+            acc.d[3] += mu * (v[0] - v0[0]);
+            acc.d[4] += mu * (v[1] - v0[1]);
+            acc.d[5] += mu * (v[2] - v0[2]);
+        }
+    }
+    if (valid) {
+        double* part = a.partial + static_cast<size_t>(g.jsplit) * 6 * a.ni;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) part[static_cast<size_t>(k) * a.ni + i] = acc.d[k];
+        a.near_count[entry] = ndef;
+    }
+    flush_counters(a.counters, valid, acc.c, tid);
+    if (a.status && valid && acc.c.fail) atomicOr(a.status, 1);
+}
+
+// Exclusive scan of the listed counts min(count, kNearSlots) by one
+// workgroup: each thread sums a contiguous chunk, the chunk totals are
+// scanned in LDS, and each thread writes its chunk's offsets.  total[0]
+// receives the listed sum, total[1] the number of entries with more near
+// pairs than slots.
+__global__ __launch_bounds__(1024) void near_scan_kernel(int n, const int* count, int* off,
+                                                         int* total)
+{
+    __shared__ int part[1024];
+    __shared__ int over[1024];
+    const int t = static_cast<int>(threadIdx.x);
+    const int chunk = (n + 1023) / 1024;
+    const int b = min(n, t * chunk);
+    const int e = min(n, b + chunk);
+    int sum = 0, ov = 0;
+    for (int k = b; k < e; ++k) {
+        const int c = count[k];
+        sum += min(c, kNearSlots);
+        ov += c > kNearSlots;
+    }
+    part[t] = sum;
+    over[t] = ov;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        const int v = t >= d ? part[t - d] : 0;
+        const int w = t >= d ? over[t - d] : 0;
+        __syncthreads();
+        part[t] += v;
+        over[t] += w;
+        __syncthreads();
+    }
+    int run = part[t] - sum;
+    for (int k = b; k < e; ++k) {
+        off[k] = run;
+        run += min(count[k], kNearSlots);
+    }
+    if (t == 1023) {
+        total[0] = part[1023];
+        total[1] = over[1023];
+    }
+}
+
+// Near-pair list: item = (i, j), grouped by (split, i), j ascending.
+__global__ __launch_bounds__(256) void near_compact_kernel(int n, int ni, const int* count,
+                                                           const int* off, const int* near_j,
+                                                           int2* items)
+{
+    const int e = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (e >= n) return;
+    const int c = min(count[e], kNearSlots);
+    const int o = off[e];
+    const int i = e % ni;
+    for (int k = 0; k < c; ++k) items[o + k] = make_int2(i, near_j[static_cast<size_t>(e) * kNearSlots + k]);
+}
+
+// Features of each near pair's first stage, SoA [4][nitems] (kepler.h:
+// SolveFeatures); NaN for a pair whose first stage has no solve setup.
+__global__ __launch_bounds__(128) void near_features_kernel(SakuraArgs a, int nitems,
+                                                            const int2* items, float* feat)
+{
+    const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= nitems) return;
+    const int2 it = items[t];
+    const int i = it.x, j = it.y;
+    double q[3] = {a.ip[1][i] - a.jp[1][j], a.ip[2][i] - a.jp[2][j], a.ip[3][i] - a.jp[3][j]};
+    const double e2 = a.ip[4][i] + a.jp[4][j];
+    const double v0[3] = {a.ip[5][i] - a.jp[5][j], a.ip[6][i] - a.jp[6][j],
+                          a.ip[7][i] - a.jp[7][j]};
+    const double m = a.ip[0][i] + a.jp[0][j];
+    const double dts = kep::first_stage(a.dt, a.flag, q, v0);
+    double k[5], s0;
+    kep::SolveFeatures f;
+    if (!kep::solve_setup(dts, m, e2, q, v0, k, s0, &f)) {
+        f.tau = f.sigma = f.kappa = f.eps = __builtin_nanf("");
+    }
+    const size_t n = static_cast<size_t>(nitems);
+    feat[0 * n + t] = f.tau;
+    feat[1 * n + t] = f.sigma;
+    feat[2 * n + t] = f.kappa;
+    feat[3 * n + t] = f.eps;
+}
+
+// Pass 2: one near pair per thread.  contrib is SoA [6][nitems].  With
+// `delta` the first solve of the pair starts from the learned guess
+// (kepler.h: Guess).  With a log, the first stage's features, the learned
+// target and the pair's Laguerre steps are recorded per item.
+__global__ __launch_bounds__(128) void sakura_near_kernel(SakuraArgs a, int nitems,
+                                                          const int2* items, double* contrib,
+                                                          const float* delta, SolveLogDev log,
+                                                          unsigned long long* fallbacks)
+{
+    const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const bool valid = t < nitems;
+    kep::Counters c;
+    if (valid) {
+        const int2 it = items[t];
+        const int i = it.x, j = it.y;
+        const double p0[3] = {a.ip[1][i] - a.jp[1][j], a.ip[2][i] - a.jp[2][j],
+                              a.ip[3][i] - a.jp[3][j]};
+        const double e2 = a.ip[4][i] + a.jp[4][j];
+        const double v0[3] = {a.ip[5][i] - a.jp[5][j], a.ip[6][i] - a.jp[6][j],
+                              a.ip[7][i] - a.jp[7][j]};
+        const double mj = a.jp[0][j];
+        const double m = a.ip[0][i] + mj;
+        double p[3] = {p0[0], p0[1], p0[2]};
+        double v[3] = {v0[0], v0[1], v0[2]};
+        kep::Guess gs;
+        if (delta) {
+            const float d = delta[t];
+            gs.use = d == d && d > -0.9f && d < 8.0f;
+            gs.delta = static_cast<double>(d);
+        }
+        const bool logging = log.target != nullptr && t < log.capacity;
+        double s_conv = __builtin_nan("");
+        kep::drift_twobody(a.dt, a.flag, m, e2, p, v, c, delta ? &gs : nullptr,
+                           logging ? &s_conv : nullptr);
+        const double mu = mj / m;
+        const size_t n = static_cast<size_t>(nitems);
+        contrib[0 * n + t] = mu * (p[0] - p0[0]);
+        contrib[1 * n + t] = mu * (p[1] - p0[1]);
+        contrib[2 * n + t] = mu * (p[2] - p0[2]);
+        contrib[3 * n + t] = mu * (v[0] - v0[0]);
+        contrib[4 * n + t] = mu * (v[1] - v0[1]);
+        contrib[5 * n + t] = mu * (v[2] - v0[2]);
+        if (logging) {
+            double q[3] = {p0[0], p0[1], p0[2]};
+            const double dts = kep::first_stage(a.dt, a.flag, q, v0);
+            double k[5], s0;
+            kep::SolveFeatures f{};
+            float target = __builtin_nanf("");
+            if (kep::solve_setup(dts, m, e2, q, v0, k, s0, &f) && s_conv == s_conv)
+                target = static_cast<float>(s_conv * k[1] / k[0] - 1);
+            const size_t cap = static_cast<size_t>(log.capacity);
+            log.features[0 * cap + t] = f.tau;
+            log.features[1 * cap + t] = f.sigma;
+            log.features[2 * cap + t] = f.kappa;
+            log.features[3 * cap + t] = f.eps;
+            log.target[t] = target;
+            log.iterations[t] = static_cast<int>(c.iters);
+        }
+    }
+    flush_counters(a.counters, valid, c, static_cast<int>(threadIdx.x));
+    if (fallbacks) {
+        const unsigned int fb = wave_sum(valid ? c.fallbacks : 0u);
+        if ((threadIdx.x & (kWave - 1)) == 0 && fb)
+            atomicAdd(fallbacks, static_cast<unsigned long long>(fb));
+    }
+    if (a.status && valid && c.fail) atomicOr(a.status, 1);
+}
+
+// Pass 3: add each (split, i)'s near contributions, in list order, to its
+// far partial sum.
+__global__ __launch_bounds__(256) void near_add_kernel(int n, int ni, int nitems,
+                                                       const int* count, const int* off,
+                                                       const double* contrib, double* partial)
+{
+    const int e = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (e >= n) return;
+    const int c = min(count[e], kNearSlots);
+    if (c == 0) return;
+    const int o = off[e];
+    const int split = e / ni;
+    const int i = e % ni;
+    double* part = partial + static_cast<size_t>(split) * 6 * ni;
+    for (int k = 0; k < 6; ++k) {
+        double sum = part[static_cast<size_t>(k) * ni + i];
+        for (int q = 0; q < c; ++q) sum += contrib[static_cast<size_t>(k) * nitems + o + q];
+        part[static_cast<size_t>(k) * ni + i] = sum;
+    }
+}
+
+// Entries with more near pairs than list slots: one thread per (split, i)
+// walks its j range again and solves, in place and in j order, the near
+// pairs after the first kNearSlots, adding them to the partial sum (after
+// the listed ones).
+__global__ __launch_bounds__(64) void near_overflow_kernel(SakuraArgs a, int n)
+{
+    const int e = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    kep::Counters c;
+    const bool valid = e < n && a.near_count[e] > kNearSlots;
+    if (valid) {
+        const int split = e / a.ni;
+        const int i = e % a.ni;
+        const int j0 = split * a.jchunk;
+        const int j1 = min(a.nj, j0 + a.jchunk);
+        double d[6] = {0, 0, 0, 0, 0, 0};
+        int seen = 0;
+        for (int j = j0; j < j1; ++j) {
+            const double p0[3] = {a.ip[1][i] - a.jp[1][j], a.ip[2][i] - a.jp[2][j],
+                                  a.ip[3][i] - a.jp[3][j]};
+            const double e2 = a.ip[4][i] + a.jp[4][j];
+            const double v0[3] = {a.ip[5][i] - a.jp[5][j], a.ip[6][i] - a.jp[6][j],
+                                  a.ip[7][i] - a.jp[7][j]};
+            const double mj = a.jp[0][j];
+            const double m = a.ip[0][i] + mj;
+            double p[3] = {p0[0], p0[1], p0[2]};
+            double v[3] = {v0[0], v0[1], v0[2]};
+            kep::Counters scratch;
+            if (kep::drift_twobody_far(a.dt, a.flag, m, e2, p, v, scratch)) continue;
+            if (++seen <= kNearSlots) continue;
+            p[0] = p0[0]; p[1] = p0[1]; p[2] = p0[2];
+            v[0] = v0[0]; v[1] = v0[1]; v[2] = v0[2];
+            kep::drift_twobody(a.dt, a.flag, m, e2, p, v, c);
+            const double mu = mj / m;
+            d[0] += mu * (p[0] - p0[0]);
+            d[1] += mu * (p[1] - p0[1]);
+            d[2] += mu * (p[2] - p0[2]);
+            d[3] += mu * (v[0] - v0[0]);
+            d[4] += mu * (v[1] - v0[1]);
+            d[5] += mu * (v[2] - v0[2]);
+        }
+        double* part = a.partial + static_cast<size_t>(split) * 6 * a.ni;
+        for (int k = 0; k < 6; ++k) part[static_cast<size_t>(k) * a.ni + i] += d[k];
+    }
+    flush_counters(a.counters, valid, c, static_cast<int>(threadIdx.x));
+    if (a.status && valid && c.fail) atomicOr(a.status, 1);
+}
+
 int sakura_call(size_t ni, const double* const* ip, size_t nj, const double* const* jp,
                 double dt, long flag, double* const* out, unsigned long long* counters,
                 int* status, void* stream)
+{
+    SakuraOptions opt;
+    return sakura_call_ex(ni, ip, nj, jp, dt, flag, out, opt, counters, status, stream);
+}
+
+int sakura_call_ex(size_t ni, const double* const* ip, size_t nj, const double* const* jp,
+                   double dt, long flag, double* const* out, SakuraOptions& opt,
+                   unsigned long long* counters, int* status, void* stream)
 {
     if (ni > static_cast<size_t>(INT32_MAX) || nj > static_cast<size_t>(INT32_MAX))
         return set_error(TUPAN_ERR_ARG, "particle count exceeds 2^31-1");
@@ -181,19 +504,94 @@ int sakura_call(size_t ni, const double* const* ip, size_t nj, const double* con
     a.nsplit = L.nsplit;
     a.jchunk = L.jchunk;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    if (L.nsplit > 1) {
-        a.partial = static_cast<double*>(
-            workspace(static_cast<size_t>(L.nsplit) * 6 * ni * sizeof(double), s));
-        if (!a.partial) return set_error(TUPAN_ERR_ALLOC, "workspace allocation failed");
+    const size_t nent = static_cast<size_t>(L.nsplit) * ni;
+    bool two_pass = opt.path == 2 ||
+                    (opt.path == 0 && static_cast<double>(ni) * static_cast<double>(nj) >=
+                                          static_cast<double>(kTwoPassPairs));
+    if (opt.net || opt.log.target) two_pass = true;
+    if (nent * kNearSlots > static_cast<size_t>(INT32_MAX)) {
+        if (opt.path == 2 || opt.net || opt.log.target)
+            return set_error(TUPAN_ERR_ARG, "too many particles for the two-pass Sakura form");
+        two_pass = false;
     }
+    opt.near_pairs = -1;
     const unsigned grid = static_cast<unsigned>(L.nblock_i) * static_cast<unsigned>(L.nsplit);
-    timing_begin(s);
-    hipLaunchKernelGGL((sakura_kernel_dev<BLOCK, TILE>), dim3(grid), dim3(BLOCK), 0, s, a);
-    timing_end(s);
-    if (L.nsplit > 1) {
-        const unsigned rg = static_cast<unsigned>((a.ni + 255) / 256);
-        hipLaunchKernelGGL((reduce_kernel<OpSakura>), dim3(rg), dim3(256), 0, s, a);
+    const unsigned rg = static_cast<unsigned>((a.ni + 255) / 256);
+    if (!two_pass) {
+        if (L.nsplit > 1) {
+            a.partial = static_cast<double*>(
+                workspace(static_cast<size_t>(L.nsplit) * 6 * ni * sizeof(double), s));
+            if (!a.partial) return set_error(TUPAN_ERR_ALLOC, "workspace allocation failed");
+        }
+        timing_begin(s);
+        hipLaunchKernelGGL((sakura_kernel_dev<BLOCK, TILE>), dim3(grid), dim3(BLOCK), 0, s, a);
+        timing_end(s);
+        if (L.nsplit > 1)
+            hipLaunchKernelGGL((reduce_kernel<OpSakura>), dim3(rg), dim3(256), 0, s, a);
+        return check_launch("sakura");
     }
+
+    // workspace slot 1: partials [nsplit][6][ni], counts, offsets, total,
+    // near j slots
+    auto up = [](size_t b) { return (b + 255) & ~static_cast<size_t>(255); };
+    const size_t b_part = up(nent * 6 * sizeof(double));
+    const size_t b_cnt = up(nent * sizeof(int));
+    const size_t b_tot = 256;
+    const size_t b_slots = up(nent * kNearSlots * sizeof(int));
+    char* w1 = static_cast<char*>(workspace(b_part + 2 * b_cnt + b_tot + b_slots, s, 1));
+    if (!w1) return set_error(TUPAN_ERR_ALLOC, "workspace allocation failed");
+    a.partial = reinterpret_cast<double*>(w1);
+    a.near_count = reinterpret_cast<int*>(w1 + b_part);
+    int* near_off = reinterpret_cast<int*>(w1 + b_part + b_cnt);
+    int* total_dev = reinterpret_cast<int*>(w1 + b_part + 2 * b_cnt);
+    a.near_j = reinterpret_cast<int*>(w1 + b_part + 2 * b_cnt + b_tot);
+    const int n = static_cast<int>(nent);
+
+    timing_begin(s);
+    hipLaunchKernelGGL((sakura_far_kernel<BLOCK, TILE>), dim3(grid), dim3(BLOCK), 0, s, a);
+    hipLaunchKernelGGL(near_scan_kernel, dim3(1), dim3(1024), 0, s, n, a.near_count, near_off,
+                       total_dev);
+    int counts[2] = {0, 0};
+    int rc = check_hip(hipMemcpyAsync(counts, total_dev, 2 * sizeof(int), hipMemcpyDeviceToHost,
+                                      s),
+                       "near-pair count");
+    if (rc == TUPAN_OK) rc = check_hip(hipStreamSynchronize(s), "near-pair count");
+    if (rc != TUPAN_OK) return rc;
+    const int total = counts[0];
+    opt.near_pairs = total;
+    if (total > 0) {
+        // workspace slot 2: items, contributions, features and guesses
+        const size_t nt = static_cast<size_t>(total);
+        const size_t b_items = up(nt * sizeof(int2));
+        const size_t b_con = up(nt * 6 * sizeof(double));
+        const size_t b_feat = opt.net ? up(nt * 4 * sizeof(float)) : 0;
+        const size_t b_delta = opt.net ? up(nt * sizeof(float)) : 0;
+        char* w2 = static_cast<char*>(workspace(b_items + b_con + b_feat + b_delta, s, 2));
+        if (!w2) return set_error(TUPAN_ERR_ALLOC, "workspace allocation failed");
+        int2* items = reinterpret_cast<int2*>(w2);
+        double* contrib = reinterpret_cast<double*>(w2 + b_items);
+        float* feat = opt.net ? reinterpret_cast<float*>(w2 + b_items + b_con) : nullptr;
+        float* delta = opt.net ? reinterpret_cast<float*>(w2 + b_items + b_con + b_feat) : nullptr;
+        const unsigned eg = static_cast<unsigned>((n + 255) / 256);
+        hipLaunchKernelGGL(near_compact_kernel, dim3(eg), dim3(256), 0, s, n, a.ni,
+                           a.near_count, near_off, a.near_j, items);
+        const unsigned ig = static_cast<unsigned>((total + 127) / 128);
+        if (opt.net) {
+            hipLaunchKernelGGL(near_features_kernel, dim3(ig), dim3(128), 0, s, a, total, items,
+                               feat);
+            rc = mlp_infer(total, feat, opt.net, delta, s);
+            if (rc != TUPAN_OK) return rc;
+        }
+        hipLaunchKernelGGL(sakura_near_kernel, dim3(ig), dim3(128), 0, s, a, total, items,
+                           contrib, delta, opt.log, opt.fallbacks);
+        hipLaunchKernelGGL(near_add_kernel, dim3(eg), dim3(256), 0, s, n, a.ni, total,
+                           a.near_count, near_off, contrib, a.partial);
+    }
+    if (counts[1] > 0) {
+        hipLaunchKernelGGL(near_overflow_kernel, dim3((n + 63) / 64), dim3(64), 0, s, a, n);
+    }
+    hipLaunchKernelGGL((reduce_kernel<OpSakura>), dim3(rg), dim3(256), 0, s, a);
+    timing_end(s);
     return check_launch("sakura");
 }
 

@@ -47,17 +47,19 @@ struct Device {
     int id = -1;
     int cu_count = 0;
     int target_wg = 0;  // workgroups the launch planner aims for
-    void* ws = nullptr;  // scratch workspace (j-split partials, staging)
-    size_t ws_bytes = 0;
+    // scratch workspaces (j-split partials, near-pair lists), by slot
+    void* ws[4] = {nullptr, nullptr, nullptr, nullptr};
+    size_t ws_bytes[4] = {0, 0, 0, 0};
 };
 
 // State of the current device (initialised on first use).
 const Device& current_device();
 
-// Scratch workspace of at least `bytes` on the current device.  Growing it
-// synchronises the device before the old buffer is released.  Returns null
-// on allocation failure.
-void* workspace(size_t bytes, hipStream_t stream);
+// Scratch workspace `slot` (0..3) of at least `bytes` on the current device.
+// Slots are independent buffers, so one call can hold several at once.
+// Growing one synchronises the device before the old buffer is released.
+// Returns null on allocation failure.
+void* workspace(size_t bytes, hipStream_t stream, int slot = 0);
 
 // Error state (thread-local message).  set_error returns `code`.
 int set_error(int code, const char* msg);
@@ -81,6 +83,30 @@ int tune_shape(int kernel, int shape, int* block, int* wpt, int* tile, int* unro
 int sakura_call(size_t ni, const double* const* ip, size_t nj, const double* const* jp,
                 double dt, long flag, double* const* out, unsigned long long* counters,
                 int* status, void* stream);
+
+// Per-item log of the near-pair solves of the two-pass Sakura form.
+struct SolveLogDev {
+    float* features = nullptr;  // [4][capacity]: tau, sigma, kappa, eps
+    float* target = nullptr;    // [capacity]: s r / dt - 1, NaN if unconverged
+    int* iterations = nullptr;  // [capacity]: Laguerre steps of the pair
+    int capacity = 0;
+};
+
+struct SakuraOptions {
+    int path = 0;                 // 0 automatic, 1 one-pass, 2 two-pass
+    const void* net = nullptr;    // packed Kepler-guess network (mlp.hip)
+    SolveLogDev log;
+    unsigned long long* fallbacks = nullptr;  // device counter of guess fallbacks
+    int near_pairs = -1;          // out: near pairs listed (-1: one-pass form)
+};
+
+int sakura_call_ex(size_t ni, const double* const* ip, size_t nj, const double* const* jp,
+                   double dt, long flag, double* const* out, SakuraOptions& opt,
+                   unsigned long long* counters, int* status, void* stream);
+
+// Kepler-guess network (mlp.hip).  mlp_infer: delta[k] for the features
+// feat[4][n] (SoA) with the packed BF16 network `net`.
+int mlp_infer(int n, const float* feat, const void* net, float* delta, hipStream_t stream);
 
 int ensemble_plan_create(int nsys, const int* h_offsets, int min_lanes, void** plan_out);
 int ensemble_plan_destroy(void* plan);

@@ -20,6 +20,8 @@ tupan/lib/opencl_backend.py:104-171) expose, per kernel:
 Launches are enqueued on torch's current stream.  There is no CPU path: a
 missing library or device raises.
 """
+import ctypes
+
 import numpy as np
 
 try:
@@ -80,7 +82,7 @@ class DeviceBuffer(object):
 class HIPKernel(object):
     """One extension kernel on libtupan_hip.so (reference protocol)."""
 
-    def __init__(self, prec, name, mode=None):
+    def __init__(self, prec, name, mode=None, path=None, guess=None):
         if name not in KERNEL_SPECS:
             raise ValueError("unknown kernel %r" % (name,))
         if prec not in ("float32", "float64"):
@@ -96,7 +98,17 @@ class HIPKernel(object):
         self._status = None
         self.args = None
         self.ni = self.nj = 0
-        self.counters = None  # optional device uint64[4] for Sakura work
+        # Sakura options: optional device uint64 counters (4, or 5 with the
+        # guess fallbacks), the form (None automatic, "one-pass",
+        # "two-pass"), a Kepler-guess network (tupan_amd.learned.KeplerNet)
+        # and a solve log (tupan_amd.learned.SolveLogBuffer)
+        self.counters = None
+        if path not in (None, "one-pass", "two-pass"):
+            raise ValueError("unknown Sakura path %r" % (path,))
+        self.path = path
+        self.guess = guess
+        self.solve_log = None
+        self.near_pairs = None
         conv = self._convert
         from collections import namedtuple
         types = namedtuple("Types", ["c_int", "c_int_p", "c_uint", "c_uint_p",
@@ -229,10 +241,32 @@ class HIPKernel(object):
                   for b in ob]
         status = self._status_buffer()
         cnt = self.counters.data_ptr() if self.counters is not None else None
-        rc = self.lib.tupan_sakura(
-            ni, *[t.data_ptr() for t in it], nj, *[t.data_ptr() for t in jt],
-            dt, flag, *[t.data_ptr() for t in ot], cnt, status.data_ptr(),
-            current_stream_handle())
+        if self.path is None and self.guess is None and self.solve_log is None:
+            rc = self.lib.tupan_sakura(
+                ni, *[t.data_ptr() for t in it], nj,
+                *[t.data_ptr() for t in jt], dt, flag,
+                *[t.data_ptr() for t in ot], cnt, status.data_ptr(),
+                current_stream_handle())
+        else:
+            if self.counters is not None and self.counters.numel() < 5:
+                raise ValueError("Sakura options need 5 counters")
+            path = {None: 0, "one-pass": 1, "two-pass": 2}[self.path]
+            net = self.guess.packed.data_ptr() if self.guess is not None \
+                else None
+            log = self.solve_log.struct() if self.solve_log is not None \
+                else None
+            near = ctypes.c_int(-1)
+            rc = self.lib.tupan_sakura_ex(
+                ni, _native.ptr_array([t.data_ptr() for t in it]), nj,
+                _native.ptr_array([t.data_ptr() for t in jt]), dt, flag,
+                _native.ptr_array([t.data_ptr() for t in ot]), path, net,
+                ctypes.byref(log) if log is not None else None, cnt,
+                ctypes.byref(near), status.data_ptr(),
+                current_stream_handle())
+            self.near_pairs = near.value
+            if self.solve_log is not None:
+                self.solve_log.count = max(0, min(near.value,
+                                                  self.solve_log.capacity))
         _native.check(rc, self.name)
         if int(status.item()) != 0:
             raise _native.TupanHipError(
