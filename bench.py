@@ -58,6 +58,9 @@ def parse():
     ap.add_argument("--energy-steps", type=int, default=100)
     ap.add_argument("--sakura-n", type=int, default=16384)
     ap.add_argument("--ensemble-systems", type=int, default=4096)
+    ap.add_argument("--learned-net",
+                    default=os.path.join(HERE, "tupan_amd", "learned",
+                                         "kepler_net.npz"))
     ap.add_argument("--sia-n", type=int, default=262144,
                     help="block-step workload size (0 skips it)")
     ap.add_argument("--sia-tau", type=float, default=0.02 / 64,
@@ -150,6 +153,70 @@ def sakura_rate(n):
             "leapfrog_pairs": leap, "kernel_s": kms / 1e3, "call_s": wall,
             "solves_per_s": solves / (kms / 1e3),
             "pairs_per_s": n * n / (kms / 1e3)}
+
+
+def sakura_forms(n, net_path, reps=3):
+    """The learned-guess workload: Plummer N = n, equal masses, eps = 0,
+    seed 1; one Sakura pair-drift call (flag -1, dt = 1/64) in the one-pass
+    form, the two-pass form with the plain guess, and the two-pass form
+    with the trained network (if `net_path` exists).  Best of `reps` calls;
+    kernel time from HIP events around the whole call (both passes, the
+    near-pair compaction and the host read of the near-pair count)."""
+    import torch
+    from tupan_amd import _native
+    from tupan_amd.ics.plummer import make_plummer
+    from tupan_amd.learned import KeplerNet, sakura_kernel
+    ps = make_plummer(n, 0.0, ("equalmass",), seed=1, device="cuda:0")
+    forms = [("one_pass", "one-pass", None), ("two_pass", "two-pass", None)]
+    if net_path and os.path.exists(net_path):
+        forms.append(("two_pass_learned", "two-pass",
+                      KeplerNet.load(net_path)))
+    out = {"n": n, "dt": 1.0 / 64, "flag": -1}
+    results = {}
+    for name, path, net in forms:
+        ext = sakura_kernel(path=path, guess=net)
+        ext.calc(ps, ps, 1.0 / 64, -1)
+        best = None
+        for _ in range(reps):
+            ext.kernel.counters = torch.zeros(5, dtype=torch.int64,
+                                              device="cuda:0")
+            torch.cuda.synchronize()
+            _native.timing(True)
+            t0 = time.perf_counter()
+            res = ext.calc(ps, ps, 1.0 / 64, -1)
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+            kms, _ = _native.timing_read()
+            _native.timing(False)
+            if best is None or kms < best[0]:
+                best = (kms, wall)
+        c = ext.kernel.counters.cpu().tolist()
+        results[name] = [r.detach().cpu().numpy().copy() for r in res]
+        out[name] = {"kernel_s": best[0] / 1e3, "call_s": best[1],
+                     "kepler_solves": c[0], "laguerre_iterations": c[1],
+                     "iterations_per_solve": c[1] / max(1, c[0]),
+                     "guess_fallbacks": c[4], "near_pairs":
+                         ext.kernel.near_pairs,
+                     "solves_per_s": c[0] / (best[0] / 1e3)}
+    ref = results["one_pass"]
+
+    def rel(a):
+        num = sum(np.max(np.sqrt(sum((a[g + k] - ref[g + k])**2
+                                     for k in range(3))) /
+                         np.maximum(np.sqrt(sum(ref[g + k]**2
+                                                for k in range(3))), 1e-300))
+                  for g in (0, 3))
+        return float(num)
+
+    for name in results:
+        if name != "one_pass":
+            out[name]["max_rel_diff_vs_one_pass"] = rel(results[name])
+    out["two_pass_speedup"] = out["one_pass"]["kernel_s"] / \
+        out["two_pass"]["kernel_s"]
+    if "two_pass_learned" in out:
+        out["learned_speedup_vs_two_pass"] = out["two_pass"]["kernel_s"] / \
+            out["two_pass_learned"]["kernel_s"]
+    return out
 
 
 def ensemble_rate(nsys, lanes):
@@ -393,6 +460,10 @@ def main():
     if args.sakura_n > 0:
         log("bench: Sakura N=%d" % args.sakura_n)
         out["sakura"] = sakura_rate(args.sakura_n)
+
+    if args.sakura_n > 0:
+        log("bench: Sakura forms and learned guess N=%d" % args.sakura_n)
+        out["sakura_forms"] = sakura_forms(args.sakura_n, args.learned_net)
 
     if args.ensemble_systems > 0:
         log("bench: Sakura ensemble of %d systems" % args.ensemble_systems)
