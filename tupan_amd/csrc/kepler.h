@@ -382,11 +382,43 @@ __device__ inline void drift_twobody(double dt, long flag, double m, double e2, 
     }
 }
 
+// The reference's DKD leapfrog with 1/R^3 from the refined hardware
+// reciprocal square root (within an ulp of the IEEE division and square
+// root of leapfrog()).  Not used by the Sakura kernels: the increment
+// p1 - p0 of a far pair cancels its O(v dt) terms, so last-ulp changes in
+// 1/R^3 show up amplified by v / (a dt) in the summed drifts.
+__device__ inline void leapfrog_fast(double dt, double m, double e2, double* p, double* v,
+                                     Counters& cnt)
+{
+    const double h = dt / 2;
+    double x = p[0], y = p[1], z = p[2];
+    double u = v[0], w = v[1], t = v[2];
+    x += u * h;
+    y += w * h;
+    z += t * h;
+    const double r2 = x * x + y * y + z * z;
+    double q1 = rsqrt_refined(r2 + e2);
+    q1 = r2 > 0 ? q1 : 0.0;
+    const double mq = m * (q1 * q1 * q1);
+    u += (-mq * x) * dt;
+    w += (-mq * y) * dt;
+    t += (-mq * z) * dt;
+    x += u * h;
+    y += w * h;
+    z += t * h;
+    p[0] = x; p[1] = y; p[2] = z;
+    v[0] = u; v[1] = w; v[2] = t;
+    cnt.leapfrog++;
+}
+
 // Far-pair part of drift_twobody: runs the flag's stages as long as every
 // twobody stage is a far pair (leapfrog) and returns true with p, v
 // advanced; returns false (p, v partly advanced, to be discarded) at the
 // first stage that needs a Kepler solve.  The leapfrogs of a pair that
-// returns false are not counted.
+// returns false are not counted.  The far/near test is the reference's
+// (IEEE division), so a single-stage flag classifies every pair exactly as
+// drift_twobody does.
+template <bool kFast>
 __device__ inline bool far_stage(double dt, double m, double e2, double* p, double* v,
                                  Counters& cnt)
 {
@@ -394,10 +426,15 @@ __device__ inline bool far_stage(double dt, double m, double e2, double* p, doub
     const double v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
     const double R = 64 * (m / v2);
     if (!(r2 > R * R)) return false;
-    leapfrog(dt, m, e2, p, v, cnt);
+    if (kFast) {
+        leapfrog_fast(dt, m, e2, p, v, cnt);
+    } else {
+        leapfrog(dt, m, e2, p, v, cnt);
+    }
     return true;
 }
 
+template <bool kFast = false>
 __device__ inline bool drift_twobody_far(double dt, long flag, double m, double e2, double* p,
                                          double* v, Counters& cnt)
 {
@@ -405,23 +442,34 @@ __device__ inline bool drift_twobody_far(double dt, long flag, double m, double 
     bool ok = true;
     if (flag == -1) {
         p[0] -= v[0] * dt; p[1] -= v[1] * dt; p[2] -= v[2] * dt;
-        ok = far_stage(dt, m, e2, p, v, c);
+        ok = far_stage<kFast>(dt, m, e2, p, v, c);
     } else if (flag == 1) {
-        ok = far_stage(dt, m, e2, p, v, c);
+        ok = far_stage<kFast>(dt, m, e2, p, v, c);
         if (ok) { p[0] -= v[0] * dt; p[1] -= v[1] * dt; p[2] -= v[2] * dt; }
     } else if (flag == -2) {
         p[0] -= v[0] * dt / 2; p[1] -= v[1] * dt / 2; p[2] -= v[2] * dt / 2;
-        ok = far_stage(dt, m, e2, p, v, c);
+        ok = far_stage<kFast>(dt, m, e2, p, v, c);
         if (ok) { p[0] -= v[0] * dt / 2; p[1] -= v[1] * dt / 2; p[2] -= v[2] * dt / 2; }
     } else if (flag == 2) {
-        ok = far_stage(dt / 2, m, e2, p, v, c);
+        ok = far_stage<kFast>(dt / 2, m, e2, p, v, c);
         if (ok) {
             p[0] -= v[0] * dt; p[1] -= v[1] * dt; p[2] -= v[2] * dt;
-            ok = far_stage(dt / 2, m, e2, p, v, c);
+            ok = far_stage<kFast>(dt / 2, m, e2, p, v, c);
         }
     }
     if (ok) cnt.leapfrog += c.leapfrog;
     return ok;
+}
+
+// 1/x from the hardware estimate refined by two Newton steps (within an ulp
+// of IEEE division for normal x).
+__device__ __forceinline__ double recip_refined(double x)
+{
+    double y = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-x, y, 1.0);
+    return __builtin_fma(y, e, y);
 }
 
 // State of a pair's first twobody stage (the one a learned guess serves):

@@ -36,7 +36,7 @@ struct SakuraArgs {
 
 // Near pairs listed per (j-split, i); further ones are solved by
 // near_overflow_kernel.
-constexpr int kNearSlots = 16;
+constexpr int kNearSlots = 32;
 // Calls with at least this many pairs take the two-pass form by default.
 constexpr double kTwoPassPairs = 4194304.0;
 
@@ -188,8 +188,12 @@ __device__ __forceinline__ void flush_counters(unsigned long long* counters, boo
 // pass 3 adds each (j-split, i)'s near contributions, in j order, to its far
 // partial sum, and the split reduction finishes as in the one-pass form.
 // Per (split, i) the far pairs are summed first, then the near pairs, each
-// in j order: deterministic, and equal to the one-pass sums up to that
-// reassociation.
+// in j order: deterministic.  Each pair's drift is the one-pass arithmetic
+// (the leapfrog keeps IEEE division and square root: a pair's increment
+// p1 - p0 cancels its O(v dt) terms down to O(a dt^2), so it amplifies any
+// change in their rounding by v / (a dt), up to 1e8 here); only the far
+// pairs' mass ratio m_j / m comes from the refined hardware reciprocal, a
+// factor on the increment, within an ulp of the division.
 // ---------------------------------------------------------------------------
 template <int BLOCK, int TILE>
 __global__ __launch_bounds__(BLOCK) void sakura_far_kernel(SakuraArgs a)
@@ -230,7 +234,7 @@ __global__ __launch_bounds__(BLOCK) void sakura_far_kernel(SakuraArgs a)
                 ++ndef;
                 continue;
             }
-            const double mu = sj.v[0] / m;
+            const double mu = sj.v[0] * kep::recip_refined(m);
             acc.d[0] += mu * (p[0] - p0[0]);
             acc.d[1] += mu * (p[1] - p0[1]);
             acc.d[2] += mu * (p[2] - p0[2]);
@@ -249,45 +253,101 @@ __global__ __launch_bounds__(BLOCK) void sakura_far_kernel(SakuraArgs a)
     if (a.status && valid && acc.c.fail) atomicOr(a.status, 1);
 }
 
-// Exclusive scan of the listed counts min(count, kNearSlots) by one
-// workgroup: each thread sums a contiguous chunk, the chunk totals are
-// scanned in LDS, and each thread writes its chunk's offsets.  total[0]
-// receives the listed sum, total[1] the number of entries with more near
-// pairs than slots.
-__global__ __launch_bounds__(1024) void near_scan_kernel(int n, const int* count, int* off,
-                                                         int* total)
+// Exclusive scan of the listed counts min(count, kNearSlots) in three
+// kernels: per-block sums (kScanPer entries per block, each thread four
+// consecutive ones), a one-workgroup scan of the block sums, and the
+// per-block scan that writes the offsets.  total[0] receives the listed
+// sum, total[1] the number of entries with more near pairs than slots, and
+// those entries are appended to `over_list` (in no particular order; each
+// is processed independently).
+constexpr int kScanThreads = 256;
+constexpr int kScanPer = 4 * kScanThreads;
+
+__device__ __forceinline__ int block_exclusive_scan(int v, int* lds, int& block_total)
 {
-    __shared__ int part[1024];
-    __shared__ int over[1024];
     const int t = static_cast<int>(threadIdx.x);
-    const int chunk = (n + 1023) / 1024;
-    const int b = min(n, t * chunk);
-    const int e = min(n, b + chunk);
-    int sum = 0, ov = 0;
-    for (int k = b; k < e; ++k) {
-        const int c = count[k];
-        sum += min(c, kNearSlots);
-        ov += c > kNearSlots;
-    }
-    part[t] = sum;
-    over[t] = ov;
+    lds[t] = v;
     __syncthreads();
-    for (int d = 1; d < 1024; d <<= 1) {
-        const int v = t >= d ? part[t - d] : 0;
-        const int w = t >= d ? over[t - d] : 0;
+    for (int d = 1; d < kScanThreads; d <<= 1) {
+        const int w = t >= d ? lds[t - d] : 0;
         __syncthreads();
-        part[t] += v;
-        over[t] += w;
+        lds[t] += w;
         __syncthreads();
     }
-    int run = part[t] - sum;
-    for (int k = b; k < e; ++k) {
-        off[k] = run;
-        run += min(count[k], kNearSlots);
+    const int incl = lds[t];
+    block_total = lds[kScanThreads - 1];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ __launch_bounds__(kScanThreads) void near_block_sums_kernel(int n, const int* count,
+                                                                       int* bsum, int* bover)
+{
+    __shared__ int lds[kScanThreads];
+    const int base = static_cast<int>(blockIdx.x) * kScanPer + 4 * static_cast<int>(threadIdx.x);
+    int sum = 0, ov = 0;
+    for (int q = 0; q < 4; ++q) {
+        const int e = base + q;
+        if (e < n) {
+            const int c = count[e];
+            sum += min(c, kNearSlots);
+            ov += c > kNearSlots;
+        }
     }
-    if (t == 1023) {
-        total[0] = part[1023];
-        total[1] = over[1023];
+    int tot_s, tot_o;
+    block_exclusive_scan(sum, lds, tot_s);
+    block_exclusive_scan(ov, lds, tot_o);
+    if (threadIdx.x == 0) {
+        bsum[blockIdx.x] = tot_s;
+        bover[blockIdx.x] = tot_o;
+    }
+}
+
+__global__ __launch_bounds__(kScanThreads) void near_block_offsets_kernel(int nb, int* bsum,
+                                                                          const int* bover,
+                                                                          int* total)
+{
+    __shared__ int lds[kScanThreads];
+    int carry = 0, over = 0;
+    for (int b0 = 0; b0 < nb; b0 += kScanThreads) {
+        const int b = b0 + static_cast<int>(threadIdx.x);
+        const int v = b < nb ? bsum[b] : 0;
+        const int o = b < nb ? bover[b] : 0;
+        int tot, tot_o;
+        const int ex = block_exclusive_scan(v, lds, tot);
+        block_exclusive_scan(o, lds, tot_o);
+        if (b < nb) bsum[b] = carry + ex;
+        carry += tot;
+        over += tot_o;
+    }
+    if (threadIdx.x == 0) {
+        total[0] = carry;
+        total[1] = over;
+        total[2] = 0;  // overflow list fill counter
+    }
+}
+
+__global__ __launch_bounds__(kScanThreads) void near_offsets_kernel(int n, const int* count,
+                                                                    const int* boff, int* off,
+                                                                    int* total, int* over_list)
+{
+    __shared__ int lds[kScanThreads];
+    const int base = static_cast<int>(blockIdx.x) * kScanPer + 4 * static_cast<int>(threadIdx.x);
+    int c[4];
+    int sum = 0;
+    for (int q = 0; q < 4; ++q) {
+        const int e = base + q;
+        c[q] = e < n ? count[e] : 0;
+        sum += min(c[q], kNearSlots);
+    }
+    int tot;
+    int run = boff[blockIdx.x] + block_exclusive_scan(sum, lds, tot);
+    for (int q = 0; q < 4; ++q) {
+        const int e = base + q;
+        if (e >= n) break;
+        off[e] = run;
+        run += min(c[q], kNearSlots);
+        if (c[q] > kNearSlots) over_list[atomicAdd(total + 2, 1)] = e;
     }
 }
 
@@ -420,51 +480,71 @@ __global__ __launch_bounds__(256) void near_add_kernel(int n, int ni, int nitems
     }
 }
 
-// Entries with more near pairs than list slots: one thread per (split, i)
-// walks its j range again and solves, in place and in j order, the near
-// pairs after the first kNearSlots, adding them to the partial sum (after
-// the listed ones).
-__global__ __launch_bounds__(64) void near_overflow_kernel(SakuraArgs a, int n)
+// Entries with more near pairs than list slots, one wave per entry: the
+// wave walks its j range in chunks of 64 (one j per lane), ranks the near
+// pairs of the chunk in j order with a ballot, solves those ranked after the
+// first kNearSlots (in parallel, one per lane), and adds the chunk's
+// contributions to the partial sum through a fixed shuffle tree
+// (deterministic).
+__global__ __launch_bounds__(64) void near_overflow_kernel(SakuraArgs a, const int* over_list,
+                                                           int nover)
 {
-    const int e = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const int w = static_cast<int>(blockIdx.x);
+    const int lane = static_cast<int>(threadIdx.x);
+    if (w >= nover) return;
+    const int e = over_list[w];
+    const int split = e / a.ni;
+    const int i = e % a.ni;
+    const int j0 = split * a.jchunk;
+    const int j1 = min(a.nj, j0 + a.jchunk);
     kep::Counters c;
-    const bool valid = e < n && a.near_count[e] > kNearSlots;
-    if (valid) {
-        const int split = e / a.ni;
-        const int i = e % a.ni;
-        const int j0 = split * a.jchunk;
-        const int j1 = min(a.nj, j0 + a.jchunk);
-        double d[6] = {0, 0, 0, 0, 0, 0};
-        int seen = 0;
-        for (int j = j0; j < j1; ++j) {
-            const double p0[3] = {a.ip[1][i] - a.jp[1][j], a.ip[2][i] - a.jp[2][j],
-                                  a.ip[3][i] - a.jp[3][j]};
-            const double e2 = a.ip[4][i] + a.jp[4][j];
-            const double v0[3] = {a.ip[5][i] - a.jp[5][j], a.ip[6][i] - a.jp[6][j],
-                                  a.ip[7][i] - a.jp[7][j]};
-            const double mj = a.jp[0][j];
-            const double m = a.ip[0][i] + mj;
-            double p[3] = {p0[0], p0[1], p0[2]};
-            double v[3] = {v0[0], v0[1], v0[2]};
-            kep::Counters scratch;
-            if (kep::drift_twobody_far(a.dt, a.flag, m, e2, p, v, scratch)) continue;
-            if (++seen <= kNearSlots) continue;
+    double d[6] = {0, 0, 0, 0, 0, 0};
+    int seen = 0;  // near pairs before this chunk
+    for (int jb = j0; jb < j1; jb += kWave) {
+        const int j = jb + lane;
+        const bool inr = j < j1;
+        const int jj = inr ? j : j0;
+        const double p0[3] = {a.ip[1][i] - a.jp[1][jj], a.ip[2][i] - a.jp[2][jj],
+                              a.ip[3][i] - a.jp[3][jj]};
+        const double e2 = a.ip[4][i] + a.jp[4][jj];
+        const double v0[3] = {a.ip[5][i] - a.jp[5][jj], a.ip[6][i] - a.jp[6][jj],
+                              a.ip[7][i] - a.jp[7][jj]};
+        const double mj = a.jp[0][jj];
+        const double m = a.ip[0][i] + mj;
+        double p[3] = {p0[0], p0[1], p0[2]};
+        double v[3] = {v0[0], v0[1], v0[2]};
+        kep::Counters scratch;
+        const bool near = inr && !kep::drift_twobody_far(a.dt, a.flag, m, e2, p, v, scratch);
+        const unsigned long long mask = __ballot(near);
+        const int rank = seen + __popcll(mask & ((1ull << lane) - 1ull));
+        seen += __popcll(mask);
+        double cd[6] = {0, 0, 0, 0, 0, 0};
+        if (near && rank >= kNearSlots) {
             p[0] = p0[0]; p[1] = p0[1]; p[2] = p0[2];
             v[0] = v0[0]; v[1] = v0[1]; v[2] = v0[2];
             kep::drift_twobody(a.dt, a.flag, m, e2, p, v, c);
             const double mu = mj / m;
-            d[0] += mu * (p[0] - p0[0]);
-            d[1] += mu * (p[1] - p0[1]);
-            d[2] += mu * (p[2] - p0[2]);
-            d[3] += mu * (v[0] - v0[0]);
-            d[4] += mu * (v[1] - v0[1]);
-            d[5] += mu * (v[2] - v0[2]);
+            cd[0] = mu * (p[0] - p0[0]);
+            cd[1] = mu * (p[1] - p0[1]);
+            cd[2] = mu * (p[2] - p0[2]);
+            cd[3] = mu * (v[0] - v0[0]);
+            cd[4] = mu * (v[1] - v0[1]);
+            cd[5] = mu * (v[2] - v0[2]);
         }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            double x = cd[k];
+#pragma unroll
+            for (int off = kWave / 2; off > 0; off >>= 1) x += __shfl_xor(x, off, kWave);
+            d[k] += x;
+        }
+    }
+    if (lane == 0) {
         double* part = a.partial + static_cast<size_t>(split) * 6 * a.ni;
         for (int k = 0; k < 6; ++k) part[static_cast<size_t>(k) * a.ni + i] += d[k];
     }
-    flush_counters(a.counters, valid, c, static_cast<int>(threadIdx.x));
-    if (a.status && valid && c.fail) atomicOr(a.status, 1);
+    flush_counters(a.counters, true, c, lane);
+    if (a.status && c.fail) atomicOr(a.status, 1);
 }
 
 int sakura_call(size_t ni, const double* const* ip, size_t nj, const double* const* jp,
@@ -531,26 +611,44 @@ int sakura_call_ex(size_t ni, const double* const* ip, size_t nj, const double* 
         return check_launch("sakura");
     }
 
-    // workspace slot 1: partials [nsplit][6][ni], counts, offsets, total,
-    // near j slots
+    // workspace slot 1: partials [nsplit][6][ni], counts, offsets, the
+    // overflow list, block sums, totals and the near j slots
     auto up = [](size_t b) { return (b + 255) & ~static_cast<size_t>(255); };
+    const int n = static_cast<int>(nent);
+    const int nb = (n + kScanPer - 1) / kScanPer;
     const size_t b_part = up(nent * 6 * sizeof(double));
     const size_t b_cnt = up(nent * sizeof(int));
+    const size_t b_blk = up(static_cast<size_t>(nb) * sizeof(int));
     const size_t b_tot = 256;
     const size_t b_slots = up(nent * kNearSlots * sizeof(int));
-    char* w1 = static_cast<char*>(workspace(b_part + 2 * b_cnt + b_tot + b_slots, s, 1));
+    char* w1 = static_cast<char*>(
+        workspace(b_part + 3 * b_cnt + 2 * b_blk + b_tot + b_slots, s, 1));
     if (!w1) return set_error(TUPAN_ERR_ALLOC, "workspace allocation failed");
-    a.partial = reinterpret_cast<double*>(w1);
-    a.near_count = reinterpret_cast<int*>(w1 + b_part);
-    int* near_off = reinterpret_cast<int*>(w1 + b_part + b_cnt);
-    int* total_dev = reinterpret_cast<int*>(w1 + b_part + 2 * b_cnt);
-    a.near_j = reinterpret_cast<int*>(w1 + b_part + 2 * b_cnt + b_tot);
-    const int n = static_cast<int>(nent);
+    char* q = w1;
+    a.partial = reinterpret_cast<double*>(q);
+    q += b_part;
+    a.near_count = reinterpret_cast<int*>(q);
+    q += b_cnt;
+    int* near_off = reinterpret_cast<int*>(q);
+    q += b_cnt;
+    int* over_list = reinterpret_cast<int*>(q);
+    q += b_cnt;
+    int* bsum = reinterpret_cast<int*>(q);
+    q += b_blk;
+    int* bover = reinterpret_cast<int*>(q);
+    q += b_blk;
+    int* total_dev = reinterpret_cast<int*>(q);
+    q += b_tot;
+    a.near_j = reinterpret_cast<int*>(q);
 
     timing_begin(s);
     hipLaunchKernelGGL((sakura_far_kernel<BLOCK, TILE>), dim3(grid), dim3(BLOCK), 0, s, a);
-    hipLaunchKernelGGL(near_scan_kernel, dim3(1), dim3(1024), 0, s, n, a.near_count, near_off,
-                       total_dev);
+    hipLaunchKernelGGL(near_block_sums_kernel, dim3(nb), dim3(kScanThreads), 0, s, n,
+                       a.near_count, bsum, bover);
+    hipLaunchKernelGGL(near_block_offsets_kernel, dim3(1), dim3(kScanThreads), 0, s, nb, bsum,
+                       bover, total_dev);
+    hipLaunchKernelGGL(near_offsets_kernel, dim3(nb), dim3(kScanThreads), 0, s, n,
+                       a.near_count, bsum, near_off, total_dev, over_list);
     int counts[2] = {0, 0};
     int rc = check_hip(hipMemcpyAsync(counts, total_dev, 2 * sizeof(int), hipMemcpyDeviceToHost,
                                       s),
@@ -588,7 +686,8 @@ int sakura_call_ex(size_t ni, const double* const* ip, size_t nj, const double* 
                            a.near_count, near_off, contrib, a.partial);
     }
     if (counts[1] > 0) {
-        hipLaunchKernelGGL(near_overflow_kernel, dim3((n + 63) / 64), dim3(64), 0, s, a, n);
+        hipLaunchKernelGGL(near_overflow_kernel, dim3(counts[1]), dim3(kWave), 0, s, a,
+                           over_list, counts[1]);
     }
     hipLaunchKernelGGL((reduce_kernel<OpSakura>), dim3(rg), dim3(256), 0, s, a);
     timing_end(s);
