@@ -39,6 +39,12 @@ struct SakuraArgs {
 constexpr int kNearSlots = 32;
 // Calls with at least this many pairs take the two-pass form by default.
 constexpr double kTwoPassPairs = 4194304.0;
+// i-particles per thread in the far-pair pass (independent dependency
+// chains per thread).
+#ifndef TUPAN_SAKURA_FAR_WPT
+#define TUPAN_SAKURA_FAR_WPT 1
+#endif
+constexpr int kFarWpt = TUPAN_SAKURA_FAR_WPT;
 
 // Wave-level sum so that one lane per wave issues the counter atomics.
 __device__ __forceinline__ unsigned int wave_sum(unsigned int v)
@@ -195,22 +201,29 @@ __device__ __forceinline__ void flush_counters(unsigned long long* counters, boo
 // pairs' mass ratio m_j / m comes from the refined hardware reciprocal, a
 // factor on the increment, within an ulp of the division.
 // ---------------------------------------------------------------------------
-template <int BLOCK, int TILE>
+template <int BLOCK, int TILE, int WPT>
 __global__ __launch_bounds__(BLOCK) void sakura_far_kernel(SakuraArgs a)
 {
     using Op = OpSakura;
     __shared__ Op::Slot tile[TILE];
     const GridPos g = grid_pos(a.nsplit);
     const int tid = static_cast<int>(threadIdx.x);
-    const int i = g.iblock * BLOCK + tid;
-    const bool valid = i < a.ni;
-    Op::IReg r;
-    Op::Acc acc;
-    Op::load_i(a, {}, valid ? i : a.ni - 1, r);
-    Op::zero(acc);
-    const size_t entry = static_cast<size_t>(g.jsplit) * a.ni + (valid ? i : 0);
-    int* slots = a.near_j + entry * kNearSlots;
-    int ndef = 0;
+    const int ibase = g.iblock * (BLOCK * WPT);
+    Op::IReg r[WPT];
+    Op::Acc acc[WPT];
+    int* slots[WPT];
+    int ndef[WPT];
+    bool valid[WPT];
+#pragma unroll
+    for (int w = 0; w < WPT; ++w) {
+        const int i = ibase + w * BLOCK + tid;
+        valid[w] = i < a.ni;
+        Op::load_i(a, {}, valid[w] ? i : a.ni - 1, r[w]);
+        Op::zero(acc[w]);
+        const size_t entry = static_cast<size_t>(g.jsplit) * a.ni + (valid[w] ? i : 0);
+        slots[w] = a.near_j + entry * kNearSlots;
+        ndef[w] = 0;
+    }
     const int j0 = g.jsplit * a.jchunk;
     const int j1 = min(a.nj, j0 + a.jchunk);
     for (int jt = j0; jt < j1; jt += TILE) {
@@ -218,39 +231,49 @@ __global__ __launch_bounds__(BLOCK) void sakura_far_kernel(SakuraArgs a)
         __syncthreads();
         for (int k = tid; k < cnt; k += BLOCK) Op::load_j(a, {}, jt + k, tile[k]);
         __syncthreads();
-        if (!valid) continue;
         for (int k = 0; k < cnt; ++k) {
-            const Op::Slot& sj = tile[k];
-            const double p0[3] = {r.x - sj.v[1], r.y - sj.v[2], r.z - sj.v[3]};
-            const double e2 = r.e2 + sj.v[4];
-            const double v0[3] = {r.vx - sj.v[5], r.vy - sj.v[6], r.vz - sj.v[7]};
-            const double m = r.m + sj.v[0];
-            double p[3] = {p0[0], p0[1], p0[2]};
-            double v[3] = {v0[0], v0[1], v0[2]};
-            if (!kep::drift_twobody_far(r.dt, r.flag, m, e2, p, v, acc.c)) {
-                // near pair: the first kNearSlots go to the list, the rest
-                // to near_overflow_kernel
-                if (ndef < kNearSlots) slots[ndef] = jt + k;
-                ++ndef;
-                continue;
+            const Op::Slot sj = tile[k];
+#pragma unroll
+            for (int w = 0; w < WPT; ++w) {
+                if (!valid[w]) continue;
+                const double p0[3] = {r[w].x - sj.v[1], r[w].y - sj.v[2], r[w].z - sj.v[3]};
+                const double e2 = r[w].e2 + sj.v[4];
+                const double v0[3] = {r[w].vx - sj.v[5], r[w].vy - sj.v[6], r[w].vz - sj.v[7]};
+                const double m = r[w].m + sj.v[0];
+                double p[3] = {p0[0], p0[1], p0[2]};
+                double v[3] = {v0[0], v0[1], v0[2]};
+                if (!kep::drift_twobody_far(r[w].dt, r[w].flag, m, e2, p, v, acc[w].c)) {
+                    // near pair: the first kNearSlots go to the list, the
+                    // rest to near_overflow_kernel
+                    if (ndef[w] < kNearSlots) slots[w][ndef[w]] = jt + k;
+                    ++ndef[w];
+                    continue;
+                }
+                const double mu = sj.v[0] * kep::recip_refined(m);
+                acc[w].d[0] += mu * (p[0] - p0[0]);
+                acc[w].d[1] += mu * (p[1] - p0[1]);
+                acc[w].d[2] += mu * (p[2] - p0[2]);
+                acc[w].d[3] += mu * (v[0] - v0[0]);
+                acc[w].d[4] += mu * (v[1] - v0[1]);
+                acc[w].d[5] += mu * (v[2] - v0[2]);
             }
-            const double mu = sj.v[0] * kep::recip_refined(m);
-            acc.d[0] += mu * (p[0] - p0[0]);
-            acc.d[1] += mu * (p[1] - p0[1]);
-            acc.d[2] += mu * (p[2] - p0[2]);
-            acc.d[3] += mu * (v[0] - v0[0]);
-            acc.d[4] += mu * (v[1] - v0[1]);
-            acc.d[5] += mu * (v[2] - v0[2]);
         }
     }
-    if (valid) {
-        double* part = a.partial + static_cast<size_t>(g.jsplit) * 6 * a.ni;
+    kep::Counters tot;
 #pragma unroll
-        for (int k = 0; k < 6; ++k) part[static_cast<size_t>(k) * a.ni + i] = acc.d[k];
-        a.near_count[entry] = ndef;
+    for (int w = 0; w < WPT; ++w) {
+        const int i = ibase + w * BLOCK + tid;
+        if (valid[w]) {
+            double* part = a.partial + static_cast<size_t>(g.jsplit) * 6 * a.ni;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) part[static_cast<size_t>(k) * a.ni + i] = acc[w].d[k];
+            a.near_count[static_cast<size_t>(g.jsplit) * a.ni + i] = ndef[w];
+            tot.leapfrog += acc[w].c.leapfrog;
+            tot.fail |= acc[w].c.fail;
+        }
     }
-    flush_counters(a.counters, valid, acc.c, tid);
-    if (a.status && valid && acc.c.fail) atomicOr(a.status, 1);
+    flush_counters(a.counters, true, tot, tid);
+    if (a.status && tot.fail) atomicOr(a.status, 1);
 }
 
 // Exclusive scan of the listed counts min(count, kNearSlots) in three
@@ -580,20 +603,22 @@ int sakura_call_ex(size_t ni, const double* const* ip, size_t nj, const double* 
     a.status = status;
     constexpr int BLOCK = 128, TILE = 64;
     const Device& dev = current_device();
-    Launch L = plan_launch(a.ni, a.nj, BLOCK, 1, TILE, 2 * dev.target_wg, 64);
-    a.nsplit = L.nsplit;
-    a.jchunk = L.jchunk;
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const size_t nent = static_cast<size_t>(L.nsplit) * ni;
     bool two_pass = opt.path == 2 ||
                     (opt.path == 0 && static_cast<double>(ni) * static_cast<double>(nj) >=
                                           static_cast<double>(kTwoPassPairs));
     if (opt.net || opt.log.target) two_pass = true;
-    if (nent * kNearSlots > static_cast<size_t>(INT32_MAX)) {
+    // the far-pair pass has its own geometry (kFarWpt i per thread)
+    Launch L = plan_launch(a.ni, a.nj, BLOCK, two_pass ? kFarWpt : 1, TILE, 2 * dev.target_wg, 64);
+    size_t nent = static_cast<size_t>(L.nsplit) * ni;
+    if (two_pass && nent * kNearSlots > static_cast<size_t>(INT32_MAX)) {
         if (opt.path == 2 || opt.net || opt.log.target)
             return set_error(TUPAN_ERR_ARG, "too many particles for the two-pass Sakura form");
         two_pass = false;
+        L = plan_launch(a.ni, a.nj, BLOCK, 1, TILE, 2 * dev.target_wg, 64);
     }
+    a.nsplit = L.nsplit;
+    a.jchunk = L.jchunk;
     opt.near_pairs = -1;
     const unsigned grid = static_cast<unsigned>(L.nblock_i) * static_cast<unsigned>(L.nsplit);
     const unsigned rg = static_cast<unsigned>((a.ni + 255) / 256);
@@ -610,6 +635,7 @@ int sakura_call_ex(size_t ni, const double* const* ip, size_t nj, const double* 
             hipLaunchKernelGGL((reduce_kernel<OpSakura>), dim3(rg), dim3(256), 0, s, a);
         return check_launch("sakura");
     }
+    const unsigned fgrid = grid;
 
     // workspace slot 1: partials [nsplit][6][ni], counts, offsets, the
     // overflow list, block sums, totals and the near j slots
@@ -642,7 +668,8 @@ int sakura_call_ex(size_t ni, const double* const* ip, size_t nj, const double* 
     a.near_j = reinterpret_cast<int*>(q);
 
     timing_begin(s);
-    hipLaunchKernelGGL((sakura_far_kernel<BLOCK, TILE>), dim3(grid), dim3(BLOCK), 0, s, a);
+    hipLaunchKernelGGL((sakura_far_kernel<BLOCK, TILE, kFarWpt>), dim3(fgrid), dim3(BLOCK), 0, s,
+                       a);
     hipLaunchKernelGGL(near_block_sums_kernel, dim3(nb), dim3(kScanThreads), 0, s, n,
                        a.near_count, bsum, bover);
     hipLaunchKernelGGL(near_block_offsets_kernel, dim3(1), dim3(kScanThreads), 0, s, nb, bsum,
