@@ -222,7 +222,10 @@ def sakura_forms(n, net_path, reps=3):
 def ensemble_rate(nsys, lanes):
     """The ensemble workload: nsys Plummer systems with sizes U[3,64] and
     log-uniform masses in [0.1, 1], Sakura with dt = 1/64 to t = 1, every
-    pair-drift stage one launch over all systems."""
+    pair-drift stage one launch over all systems.  Timed twice from the same
+    initial state: eager launches (kernel time from HIP events, counters)
+    and replays of one captured 8-step HIP graph (wall time; the final
+    states are compared)."""
     import torch
     from tupan_amd import _native
     from tupan_amd.ics.ensemble import make_ensemble
@@ -234,8 +237,10 @@ def ensemble_rate(nsys, lanes):
     ps, ens = make_ensemble(nsys=nsys, nmin=3, nmax=64, seed=1,
                             device="cuda:0", min_lanes=lanes)
     t_ic = time.perf_counter() - t0
+    ps_init = ps.copy()
     drv = SakuraEnsemble(ps, ens)
     e0 = drv.energies()
+    # eager launches: kernel time (HIP events) and work counters
     ens.counters = torch.zeros(4, dtype=torch.int64, device="cuda:0")
     torch.cuda.synchronize()
     _native.timing(True)
@@ -245,8 +250,22 @@ def ensemble_rate(nsys, lanes):
     wall = time.perf_counter() - t0
     kms, launches = _native.timing_read()
     _native.timing(False)
-    solves, iters, retries, leap = [int(x) for x in
-                                    ens.counters.cpu().tolist()]
+    counts = ens.counters.cpu().tolist()
+    ens.counters = None
+    # the same 64 steps as replays of one captured 8-step HIP graph
+    gdrv = SakuraEnsemble(ps_init, ens)
+    graph = gdrv.capture(1.0 / 64, 8)
+    ens.reset_status()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(drv.nsteps // 8):
+        graph.replay()
+    torch.cuda.synchronize()
+    wall_graph = time.perf_counter() - t0
+    ens.check_status()
+    same = all(torch.equal(getattr(ps, k), getattr(ps_init, k))
+               for k in ("rx", "ry", "rz", "vx", "vy", "vz"))
+    solves, iters, retries, leap = [int(x) for x in counts]
     sizes = np.diff(ens.offsets)
     pairs = int(np.sum(sizes.astype(np.int64)**2))
     return {"systems": nsys, "particles": int(ens.ntot),
@@ -254,7 +273,9 @@ def ensemble_rate(nsys, lanes):
             "lanes_per_system_min": lanes, "launches": launches,
             "kepler_solves": solves, "laguerre_iterations": iters,
             "subdivisions": retries, "leapfrog_pairs": leap,
-            "kernel_s": kms / 1e3, "wall_s": wall, "ic_s": t_ic,
+            "kernel_s": kms / 1e3, "wall_s": wall,
+            "wall_s_graph": wall_graph, "graph_equals_eager": same,
+            "ic_s": t_ic,
             "solves_per_s": solves / (kms / 1e3),
             "pair_drifts_per_s": 2 * drv.nsteps * pairs / (kms / 1e3),
             "iterations_per_solve": iters / max(solves, 1),

@@ -130,3 +130,43 @@ def test_ensemble_driver_vs_single_system_sakura():
             np.testing.assert_allclose(getattr(ps, k)[a:b].cpu().numpy(),
                                        getattr(q, k).cpu().numpy(),
                                        rtol=0, atol=1e-12)
+
+
+def test_ensemble_graph_replay_equals_eager():
+    """A captured block of Sakura ensemble steps, replayed several times with
+    eager work and synchronisations in between, gives the eager steps' numbers
+    after every replay; evolve_graph equals evolve."""
+    from tupan_amd.ics.ensemble import make_ensemble
+    from tupan_amd.integrator.ensemble import SakuraEnsemble
+    from tupan_amd.lib.ensemble import Ensemble
+    from tupan_amd.particles import ParticleSystem
+    ps1, ens1 = make_ensemble(nsys=64, nmin=3, nmax=40, seed=3,
+                              device="cuda:0")
+    host = ps1.to_numpy()
+    ps2 = ParticleSystem.from_numpy(host, device="cuda:0")
+    ens2 = Ensemble(ens1.offsets)
+    eager = SakuraEnsemble(ps1, ens1)
+    graphed = SakuraEnsemble(ps2, ens2)
+    tau = 1.0 / 64
+    g = graphed.capture(tau, 4)
+    keys = ("rx", "ry", "rz", "vx", "vy", "vz")
+    for rep in range(3):
+        g.replay()
+        for _ in range(4):
+            eager.step(tau)
+        phi = ens1.phi(ps1).sum().item()  # eager work between replays
+        torch.cuda.synchronize()
+        assert phi == phi
+        for k in keys:
+            assert torch.equal(getattr(ps1, k), getattr(ps2, k)), (rep, k)
+    ens2.check_status()
+
+    ps3 = ParticleSystem.from_numpy(host, device="cuda:0")
+    ps4 = ParticleSystem.from_numpy(host, device="cuda:0")
+    a = SakuraEnsemble(ps3, Ensemble(ens1.offsets))
+    b = SakuraEnsemble(ps4, Ensemble(ens1.offsets))
+    a.evolve(0.25, tau)
+    b.evolve_graph(0.25, tau, steps_per_graph=6)
+    assert a.nsteps == b.nsteps == 16 and a.time == b.time
+    for k in keys:
+        assert torch.equal(getattr(ps3, k), getattr(ps4, k)), k

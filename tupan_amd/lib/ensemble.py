@@ -68,14 +68,18 @@ class Ensemble(object):
                 raise TypeError("ensemble arrays must be contiguous float64 "
                                 "device tensors")
 
-    def sakura(self, ps, dt, flag):
+    def sakura(self, ps, dt, flag, check=True):
         """dr, dv of every particle from the pair drifts inside its system
-        (sakura_kernel.c:5-63); registers drx..dvz on ps."""
+        (sakura_kernel.c:5-63); registers drx..dvz on ps.  With
+        check=False the solver status is left on the device (no host
+        synchronisation; read it with check_status): the form used inside a
+        captured graph."""
         self._check_ps(ps)
         for k in ("drx", "dry", "drz", "dvx", "dvy", "dvz"):
             if k not in ps.__dict__:
                 ps.register_auxiliary_attribute(k, "real")
-        self._status.zero_()
+        if check:
+            self._status.zero_()
         cnt = self.counters.data_ptr() if self.counters is not None else None
         p = [getattr(ps, k).data_ptr() for k in
              ("mass", "rx", "ry", "rz", "eps2", "vx", "vy", "vz")]
@@ -85,11 +89,20 @@ class Ensemble(object):
             self._plan, *p, float(dt), int(flag), *o, cnt,
             self._status.data_ptr(), current_stream_handle())
         _native.check(rc, "tupan_ensemble_sakura")
+        if check:
+            self.check_status()
+        return ps.drx, ps.dry, ps.drz, ps.dvx, ps.dvy, ps.dvz
+
+    def reset_status(self):
+        self._status.zero_()
+
+    def check_status(self):
+        """Raise if any pair drift since the last reset exhausted the
+        solver's bounded step subdivision (one host synchronisation)."""
         if int(self._status.item()) != 0:
             raise _native.TupanHipError(
                 "ensemble Sakura: Kepler solver exhausted its bounded step "
                 "subdivision for at least one pair")
-        return ps.drx, ps.dry, ps.drz, ps.dvx, ps.dvy, ps.dvz
 
     def phi(self, ps):
         """Potential of every particle due to its own system."""
