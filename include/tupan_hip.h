@@ -310,6 +310,34 @@ int tupan_sakura_ex(size_t ni, const double *const *ifields, size_t nj,
                     const tupan_solve_log *log, unsigned long long *counters,
                     int *near_pairs, int *status, void *stream);
 
+/* Hierarchical SIA step of a small subset on the device (no counterpart in
+ * the reference, whose driver visits every level from the host:
+ * tupan/integrator/sia.py:1089-1123, 'h' variants).  Runs recurse(S, tau)
+ * for the n <= 64 particles of S in one launch of one wave, with the host
+ * driver's operations in the host driver's order, so the results are the
+ * host driver's bit for bit.
+ *   arrays[narrays]  every 8-byte per-particle array of S (device; they are
+ *                    permuted together at each level's [slow | fast] split)
+ *   roles[15]        index in arrays of mass, eps2, rx, ry, rz, vx, vy, vz,
+ *                    time, nstep (int64), tstep, tstepij, ax, ay, az
+ *   acc_mode, tstep_mode  TUPAN_MODE_FP64 or TUPAN_MODE_SPLIT32
+ *   bridge_op/c[nbridge]  the scheme's slow/fast sequence: 0 = evolve slow
+ *                    and recurse on fast over c tau, 1 = slow<->fast kicks
+ *   evolve_op/c[nevolve]  the scheme's sequence on a set: 0 = drift,
+ *                    1 = kick, over c dt
+ *   t_curr           device double, advanced as the host driver advances
+ *                    its time (by tau at each level whose fast set is empty)
+ *   pairs            device uint64[2], incremented by the Tstep and Acc
+ *                    interactions evaluated
+ *   status           device int: 1 Kepler failure, 2 more than 64 levels,
+ *                    3 more than 65536 level visits */
+int tupan_sia_subtree(int n, void *const *arrays, int narrays, const int *roles,
+                      int acc_mode, int tstep_mode, double eta, double tau,
+                      int nbridge, const int *bridge_op, const double *bridge_c,
+                      int nevolve, const int *evolve_op, const double *evolve_c,
+                      double *t_curr, unsigned long long *pairs, int *status,
+                      void *stream);
+
 /* ---------------------------------------------------------------------------
  * Kepler-guess network (no counterpart in the reference): 4 -> 64 -> 64 -> 1
  * perceptron with SiLU, inputs x = (log tau, sigma, kappa, eps) of

@@ -87,6 +87,12 @@ def _declare(lib):
                                     ctypes.c_long, _PP, ctypes.c_int, _P,
                                     ctypes.POINTER(SolveLog), _P,
                                     ctypes.POINTER(ctypes.c_int), _P, _P]
+    lib.tupan_sia_subtree.restype = ctypes.c_int
+    lib.tupan_sia_subtree.argtypes = ([ctypes.c_int, _PP, ctypes.c_int, _P,
+                                       ctypes.c_int, ctypes.c_int,
+                                       ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_int, _P, _P, ctypes.c_int, _P,
+                                       _P, _P, _P, _P, _P])
     lib.tupan_mlp_sizes.restype = ctypes.c_int
     lib.tupan_mlp_sizes.argtypes = [ctypes.POINTER(ctypes.c_int)] * 3
     lib.tupan_mlp_grad.restype = ctypes.c_int
@@ -169,4 +175,4 @@ DEVICE_ABI = ("tupan_last_error", "tupan_version", "tupan_device_info",
               "tupan_ensemble_info", "tupan_ensemble_sakura",
               "tupan_ensemble_phi", "tupan_sakura_ex", "tupan_mlp_sizes",
               "tupan_mlp_grad", "tupan_mlp_forward", "tupan_mlp_adam",
-              "tupan_mlp_pack", "tupan_mlp_infer")
+              "tupan_mlp_pack", "tupan_mlp_infer", "tupan_sia_subtree")

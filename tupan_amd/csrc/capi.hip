@@ -612,6 +612,17 @@ int tupan_sakura_ex(size_t ni, const double* const* ifields, size_t nj,
     return rc;
 }
 
+int tupan_sia_subtree(int n, void* const* arrays, int narrays, const int* roles, int acc_mode,
+                      int tstep_mode, double eta, double tau, int nbridge, const int* bridge_op,
+                      const double* bridge_c, int nevolve, const int* evolve_op,
+                      const double* evolve_c, double* t_curr, unsigned long long* pairs,
+                      int* status, void* stream)
+{
+    return sia_subtree_call(n, arrays, narrays, roles, acc_mode, tstep_mode, eta, tau, nbridge,
+                            bridge_op, bridge_c, nevolve, evolve_op, evolve_c, t_curr, pairs,
+                            status, stream);
+}
+
 int tupan_kepler_pairs(size_t npairs, const int* idx, const double* m, double* rx, double* ry,
                        double* rz, const double* e2, double* vx, double* vy, double* vz,
                        double dt, int* status, void* stream)

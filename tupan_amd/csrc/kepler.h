@@ -411,6 +411,44 @@ __device__ inline void leapfrog_fast(double dt, double m, double e2, double* p, 
     cnt.leapfrog++;
 }
 
+// Exact two-body advance of particles i and j in place, in their
+// centre-of-mass frame (kepler_solver_kernel_common.h:8-81): the SIA n = 2
+// leaves.
+__device__ inline void advance_pair(int i, int j, const double* m, double* rx, double* ry,
+                                    double* rz, const double* e2, double* vx, double* vy,
+                                    double* vz, double dt, Counters& cnt)
+{
+    const double r0[3] = {rx[i] - rx[j], ry[i] - ry[j], rz[i] - rz[j]};
+    const double ee = e2[i] + e2[j];
+    const double v0[3] = {vx[i] - vx[j], vy[i] - vy[j], vz[i] - vz[j]};
+    const double mt = m[i] + m[j];
+    const double mu0 = m[i] / mt;
+    const double mu1 = m[j] / mt;
+    double cx = mu0 * rx[i] + mu1 * rx[j];
+    double cy = mu0 * ry[i] + mu1 * ry[j];
+    double cz = mu0 * rz[i] + mu1 * rz[j];
+    const double wx = mu0 * vx[i] + mu1 * vx[j];
+    const double wy = mu0 * vy[i] + mu1 * vy[j];
+    const double wz = mu0 * vz[i] + mu1 * vz[j];
+    cx += wx * dt;
+    cy += wy * dt;
+    cz += wz * dt;
+    double pr[3], pv[3];
+    kepler_solve(dt, mt, ee, r0, v0, pr, pv, cnt);
+    rx[i] = cx + mu1 * pr[0];
+    ry[i] = cy + mu1 * pr[1];
+    rz[i] = cz + mu1 * pr[2];
+    vx[i] = wx + mu1 * pv[0];
+    vy[i] = wy + mu1 * pv[1];
+    vz[i] = wz + mu1 * pv[2];
+    rx[j] = cx - mu0 * pr[0];
+    ry[j] = cy - mu0 * pr[1];
+    rz[j] = cz - mu0 * pr[2];
+    vx[j] = wx - mu0 * pv[0];
+    vy[j] = wy - mu0 * pv[1];
+    vz[j] = wz - mu0 * pv[2];
+}
+
 // Far-pair part of drift_twobody: runs the flag's stages as long as every
 // twobody stage is a far pair (leapfrog) and returns true with p, v
 // advanced; returns false (p, v partly advanced, to be discarded) at the

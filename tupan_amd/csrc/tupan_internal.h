@@ -104,6 +104,13 @@ int sakura_call_ex(size_t ni, const double* const* ip, size_t nj, const double* 
                    double dt, long flag, double* const* out, SakuraOptions& opt,
                    unsigned long long* counters, int* status, void* stream);
 
+// Device SIA recursion for sets of at most 64 particles (sia.hip).
+int sia_subtree_call(int n, void* const* arrays, int narrays, const int* roles, int acc_mode,
+                     int tstep_mode, double eta, double tau, int nbridge, const int* bridge_op,
+                     const double* bridge_c, int nevolve, const int* evolve_op,
+                     const double* evolve_c, double* t_curr, unsigned long long* pairs,
+                     int* status, void* stream);
+
 // Kepler-guess network (mlp.hip).  mlp_infer: delta[k] for the features
 // feat[4][n] (SoA) with the packed BF16 network `net`.
 int mlp_infer(int n, const float* feat, const void* net, float* delta, hipStream_t stream);

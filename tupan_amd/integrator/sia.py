@@ -25,13 +25,26 @@ arrays to [slow | fast] and hands out slice views of the two parts, so every
 kernel call sees the same particles in the same order as the reference's
 copies, no subset is ever copied out, and the join is a no-op.
 ``SIA(..., inplace=False)`` runs the reference's copy/append form.
+
+Small subsets.  With the HIP backend in FP64 or split-FP32 mode, the 'h'
+variants hand every subset of at most SUBTREE_MAX particles to the device
+(libtupan_hip: tupan_sia_subtree), which runs that subset's whole recursion
+in one launch with the same operations in the same order, so the deep
+levels -- most of the visits, each a few tiny kernels -- cost one launch per
+subset instead of host work per visit.  ``SIA(..., device_subtree=False)``
+keeps every level on the host.
 """
+import ctypes
 import logging
 
 import torch
 
 from . import Base
 from .fewbody import FewBody
+
+SUBTREE_MAX = 64
+SUBTREE_ROLES = ("mass", "eps2", "rx", "ry", "rz", "vx", "vy", "vz", "time",
+                 "nstep", "tstep", "tstepij", "ax", "ay", "az")
 
 logger = logging.getLogger(__name__)
 
@@ -189,9 +202,11 @@ class SIA(Base):
     # system up front so that slice views share them
     AUX = ("tstepij", "ax", "ay", "az")
 
-    def __init__(self, eta, time, ps, method, inplace=True, **kwargs):
+    def __init__(self, eta, time, ps, method, inplace=True,
+                 device_subtree=True, **kwargs):
         super(SIA, self).__init__(eta, time, ps, **kwargs)
         self.inplace = inplace
+        self.device_subtree = device_subtree
         self.method = method
         self.scheme = Scheme(method[:5])
         variant = method[5]
@@ -205,12 +220,16 @@ class SIA(Base):
             for name in self.AUX:
                 if name not in ps.__dict__:
                     ps.register_auxiliary_attribute(name, "real")
+        self._subtree = _DeviceSubtree.make(self, ps)
         return self.recurse(ps, tau, self.evolve, self.scheme.bridge_sf)
 
     def recurse(self, ps, tau, evolve, bridge_sf):
         """sia.py:1089-1123."""
         if not ps.n:
             return ps
+        sub = getattr(self, "_subtree", None)
+        if sub is not None and ps.n <= SUBTREE_MAX and evolve == self.evolve:
+            return sub.run(ps, tau)
         flag = -1
         if self.update_tstep:
             flag = 1
@@ -227,3 +246,82 @@ class SIA(Base):
             slow.time += tau
             slow.nstep += 1
         return ps if self.inplace else join(slow, fast)
+
+
+class _DeviceSubtree(object):
+    """The host side of tupan_sia_subtree for one SIA object: the scheme's
+    operator sequences, the arithmetic modes of the installed Acc and Tstep
+    kernels, and small device buffers for the time, counters and status."""
+
+    @classmethod
+    def make(cls, sia, ps):
+        from ..lib import extensions
+        from ..lib.hip_backend import HIPKernel
+        if not (sia.device_subtree and sia.inplace and sia.update_tstep
+                and not sia.shared_tstep):
+            return None
+        kernels = []
+        for ext in (extensions.acc, extensions.tstep):
+            k = getattr(ext, "kernel", None)
+            if not isinstance(k, HIPKernel) or k.prec != "float64" or \
+                    k.mode not in ("fp64", "split32"):
+                return None
+            kernels.append(k)
+        arrays = ps.arrays()
+        if not all(t.is_cuda and t.element_size() == 8
+                   for t in arrays.values()):
+            return None
+        if any(r not in arrays for r in SUBTREE_ROLES) or len(arrays) > 32:
+            return None
+        return cls(sia, kernels[0].mode, kernels[1].mode, ps.device)
+
+    def __init__(self, sia, acc_mode, tstep_mode, device):
+        from .. import _native
+        self.lib = _native.lib()
+        self.sia = sia
+        self.modes = (_native.MODES[acc_mode], _native.MODES[tstep_mode])
+        seq = palindrome(sia.scheme.d, sia.scheme.k)
+        self.bridge_op = (ctypes.c_int * len(seq))(*[w for w, _ in seq])
+        self.bridge_c = (ctypes.c_double * len(seq))(*[c for _, c in seq])
+        if sia.method.endswith("dkd"):
+            ev = [(0 if w == 0 else 1, c) for w, c in seq]
+        else:
+            ev = [(1 if w == 0 else 0, c) for w, c in seq]
+        self.evolve_op = (ctypes.c_int * len(ev))(*[w for w, _ in ev])
+        self.evolve_c = (ctypes.c_double * len(ev))(*[c for _, c in ev])
+        self.t_curr = torch.zeros(1, dtype=torch.float64, device=device)
+        self.pairs = torch.zeros(2, dtype=torch.int64, device=device)
+        self.status = torch.zeros(1, dtype=torch.int32, device=device)
+
+    def run(self, ps, tau):
+        from .. import _native
+        from ..lib import extensions
+        from ..lib.hip_backend import current_stream_handle
+        arrays = ps.arrays()
+        names = list(arrays)
+        ptrs = _native.ptr_array([t.data_ptr() for t in arrays.values()])
+        roles = (ctypes.c_int * len(SUBTREE_ROLES))(
+            *[names.index(r) for r in SUBTREE_ROLES])
+        self.t_curr.fill_(type(ps).t_curr)
+        self.pairs.zero_()
+        self.status.zero_()
+        rc = self.lib.tupan_sia_subtree(
+            ps.n, ptrs, len(names), ctypes.addressof(roles), self.modes[0],
+            self.modes[1], self.sia.eta, tau, len(self.bridge_c),
+            ctypes.addressof(self.bridge_op), ctypes.addressof(self.bridge_c),
+            len(self.evolve_c), ctypes.addressof(self.evolve_op),
+            ctypes.addressof(self.evolve_c), self.t_curr.data_ptr(),
+            self.pairs.data_ptr(), self.status.data_ptr(),
+            current_stream_handle())
+        _native.check(rc, "tupan_sia_subtree")
+        status = int(self.status.item())
+        if status:
+            raise _native.TupanHipError(
+                "SIA subtree failed (status %d: 1 Kepler solver, 2 more than "
+                "64 levels, 3 more than 65536 level visits)" % status)
+        type(ps).t_curr = float(self.t_curr.item())
+        if extensions.pair_counts is not None:
+            tp, ap = self.pairs.tolist()
+            extensions.pair_counts["Tstep"] += tp
+            extensions.pair_counts["Acc"] += ap
+        return ps
