@@ -353,6 +353,82 @@ def block_step_rate(n, tau, steps, modes):
     return out
 
 
+def _best(fn, reps):
+    best = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    return best
+
+
+def cpu_sakura(n, rows):
+    """Serial oracle Sakura (flag -1, dt = 1/64) on `rows` i-rows of the
+    Plummer N = n state against all n j; Kepler solves/s and pairs/s."""
+    from oracle import Oracle
+    from tupan_amd.ics.plummer import make_plummer
+    host = make_plummer(n, 0.0, ("equalmass",), seed=1,
+                        device="cuda:0").to_numpy()
+    sub = {k: v[:rows] for k, v in host.items()}
+    orc = Oracle("float64")
+    orc.counters_reset()
+    t = _best(lambda: orc.sakura(sub, host, 1.0 / 64, -1), 1)
+    solves = orc.counters()[0]
+    return {"value": solves / t / 1e6, "unit": "M Kepler solves/s",
+            "pairs_per_s": rows * n / t, "cores": 1, "kind": "port",
+            "sample": "%d i-rows x %d j, Sakura flag -1, dt 1/64 (%.2f s, "
+                      "%d solves)" % (rows, n, t, solves)}
+
+
+def cpu_block_steps(n, rows, mix):
+    """Serial oracle Tstep and Acc on `rows` i-rows of the block-step
+    workload's initial state against all n j; the step's interaction mix
+    `mix` ({"Tstep": pairs, "Acc": pairs}) at those rates, in Gint/s."""
+    from oracle import Oracle
+    from tupan_amd.ics.imf import IMF
+    from tupan_amd.ics.plummer import make_plummer
+    host = make_plummer(n, 1e-4, IMF.power_law(-2.35, 0.1, 10.0), seed=1,
+                        device="cuda:0").to_numpy()
+    sub = {k: v[:rows] for k, v in host.items()}
+    orc = Oracle("float64")
+    tt = _best(lambda: orc.tstep(sub, host, 0.02), 1)
+    ta = _best(lambda: orc.acc(sub, host), 1)
+    rt, ra = rows * n / tt, rows * n / ta
+    pt, pa = mix.get("Tstep", 0), mix.get("Acc", 0)
+    step_s = pt / rt + pa / ra
+    return {"value": (pt + pa) / step_s / 1e9, "unit": "Gint/s", "cores": 1,
+            "kind": "port", "tstep_gint_s": rt / 1e9, "acc_gint_s": ra / 1e9,
+            "sample": "%d i-rows x %d j of Tstep (%.2f s) and Acc (%.2f s), "
+                      "weighted by the step's interaction counts"
+                      % (rows, n, tt, ta)}
+
+
+def cpu_ensemble(nsys):
+    """Serial oracle Sakura pair drifts (flags -1 and +1, dt 1/128) on the
+    first `nsys` systems of the ensemble workload's initial state, system by
+    system; Kepler solves/s."""
+    from oracle import Oracle
+    from tupan_amd.ics.ensemble import make_ensemble
+    ps, ens = make_ensemble(nsys=4096, nmin=3, nmax=64, seed=1,
+                            device="cuda:0")
+    host = ps.to_numpy()
+    off = ens.offsets
+    orc = Oracle("float64")
+    orc.counters_reset()
+    t0 = time.perf_counter()
+    for s in range(nsys):
+        sub = {k: v[off[s]:off[s + 1]] for k, v in host.items()}
+        for flag in (-1, 1):
+            orc.sakura(sub, sub, 1.0 / 128, flag)
+    t = time.perf_counter() - t0
+    solves = orc.counters()[0]
+    return {"value": solves / t / 1e6, "unit": "M Kepler solves/s",
+            "cores": 1, "kind": "port",
+            "sample": "first %d of 4096 systems, 2 pair-drift calls each "
+                      "(%.2f s, %d solves)" % (nsys, t, solves)}
+
+
 def cpu_baseline(ps, rows):
     """Serial oracle on `rows` i-rows against all j; pairs/s -> Gint/s."""
     import numpy as np
@@ -503,6 +579,17 @@ def main():
         one, allc = cpu_baseline(ps, rows)
         out["cpu_baseline"] = one
         out["cpu_baseline_all_cores"] = allc
+        if args.sakura_n > 0:
+            log("bench: CPU baseline, Sakura")
+            out["sakura"]["cpu_baseline"] = cpu_sakura(args.sakura_n, 256)
+        if args.ensemble_systems > 0:
+            log("bench: CPU baseline, ensemble")
+            out["sakura_ensemble"]["cpu_baseline"] = cpu_ensemble(256)
+        if args.sia_n > 0:
+            log("bench: CPU baseline, block steps")
+            mode = [m for m in args.sia_modes.split(",") if m][-1]
+            out["block_steps"]["cpu_baseline"] = cpu_block_steps(
+                args.sia_n, 64, out["block_steps"][mode]["pairs_by_kernel"])
     print(json.dumps(out), flush=True)
     return 0
 

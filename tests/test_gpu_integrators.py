@@ -131,3 +131,28 @@ def test_sia_device_subtree_equals_host_levels(method, n, mode):
         assert torch.equal(getattr(a, k), getattr(b, k)), k
     levels = torch.unique(a.tstep).numel()
     assert levels >= 3, levels
+
+
+def test_plummer_1024_ahermite4_to_t1():
+    """The 1024-particle workload in full: Plummer, eps = 4/N, ahermite4
+    with eta = 0.01 to t = 1; |dE/E| at t = 1 stays below 1e-6 and every
+    particle ends at t = 1."""
+    from tupan_amd.diagnostics import energy
+    from tupan_amd.ics.plummer import make_plummer
+    from tupan_amd.integrator import Integrator
+    install_hip("float64")
+    ps = make_plummer(1024, 4.0 / 1024, ("equalmass",), seed=1,
+                      device="cuda:0")
+    type(ps).t_curr = 0.0
+    e0 = energy(ps)
+    it = Integrator(0.01, 0.0, ps, method="ahermite4")
+    nstep = 0
+    while it.time < 1.0:
+        it.evolve_step(1.0)
+        nstep += 1
+    q = it.particle_system
+    err = abs((energy(q) - e0) / e0)
+    print("ahermite4 N=1024 to t=1: %d steps, |dE/E| = %.3e" % (nstep, err))
+    assert it.time == 1.0
+    assert torch.all(q.time == 1.0)
+    assert err < 1e-6
