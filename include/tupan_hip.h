@@ -155,6 +155,31 @@ void tstep_kernel(const unsigned long ni,
                   const double eta,
                   double *_idt_a, double *_idt_b);
 
+/* libtupan.h:152-178 */
+void nreg_Xkernel(const unsigned long ni,
+                  const double *_im, const double *_irx, const double *_iry,
+                  const double *_irz, const double *_ie2,
+                  const double *_ivx, const double *_ivy, const double *_ivz,
+                  const unsigned long nj,
+                  const double *_jm, const double *_jrx, const double *_jry,
+                  const double *_jrz, const double *_je2,
+                  const double *_jvx, const double *_jvy, const double *_jvz,
+                  const double dt,
+                  double *_idrx, double *_idry, double *_idrz,
+                  double *_iax, double *_iay, double *_iaz, double *_iu);
+
+/* libtupan.h:180-201 */
+void nreg_Vkernel(const unsigned long ni,
+                  const double *_im, const double *_ivx, const double *_ivy,
+                  const double *_ivz, const double *_iax, const double *_iay,
+                  const double *_iaz,
+                  const unsigned long nj,
+                  const double *_jm, const double *_jvx, const double *_jvy,
+                  const double *_jvz, const double *_jax, const double *_jay,
+                  const double *_jaz,
+                  const double dt,
+                  double *_idvx, double *_idvy, double *_idvz, double *_ik);
+
 /* libtupan.h:203-229 */
 void sakura_kernel(const unsigned long ni,
                    const double *_im, const double *_irx, const double *_iry,
@@ -240,6 +265,8 @@ int tupan_tstep(int mode, size_t ni,
 #define TUPAN_K_ACC_JERK 2
 #define TUPAN_K_SNAP_CRACKLE 3
 #define TUPAN_K_TSTEP 4
+#define TUPAN_K_NREG_X 5  /* nreg_Xkernel: 8 + 8 fields, dt; 7 outputs */
+#define TUPAN_K_NREG_V 6  /* nreg_Vkernel: 7 + 7 fields, dt; 4 outputs */
 int tupan_pairwise(int kernel, int mode, size_t ni,
                    const double *const *ifields, size_t nj,
                    const double *const *jfields, double *const *out,

@@ -35,6 +35,12 @@ SPECS = {
     "tstep": (("mass", "rx", "ry", "rz", "eps2", "vx", "vy", "vz"),
               ("mass", "rx", "ry", "rz", "eps2", "vx", "vy", "vz"),
               ("eta",), ("tstep", "tstepij")),
+    "nreg_x": (("mass", "rx", "ry", "rz", "eps2", "vx", "vy", "vz"),
+               ("mass", "rx", "ry", "rz", "eps2", "vx", "vy", "vz"),
+               ("dt",), ("mrx", "mry", "mrz", "ax", "ay", "az", "u")),
+    "nreg_v": (("mass", "vx", "vy", "vz", "ax", "ay", "az"),
+               ("mass", "vx", "vy", "vz", "ax", "ay", "az"),
+               ("dt",), ("mvx", "mvy", "mvz", "mk")),
     "sakura": (("mass", "rx", "ry", "rz", "eps2", "vx", "vy", "vz"),
                ("mass", "rx", "ry", "rz", "eps2", "vx", "vy", "vz"),
                ("dt", "flag"), ("drx", "dry", "drz", "dvx", "dvy", "dvz")),
@@ -193,6 +199,7 @@ class Reference(_Lib):
                  "acc_jerk": "acc_jerk_kernel",
                  "snap_crackle": "snap_crackle_kernel",
                  "tstep": "tstep_kernel", "sakura": "sakura_kernel",
+                 "nreg_x": "nreg_Xkernel", "nreg_v": "nreg_Vkernel",
                  "kepler": "kepler_solver_kernel"}
         super().__init__(path, prec, lambda n: names[n])
 

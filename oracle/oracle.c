@@ -330,6 +330,95 @@ void SYM(tstep)(size_t ni, const real *im, const real *irx, const real *iry,
 }
 
 /* ---------------------------------------------------------------------------
+ * NREG X step: nreg_kernels.c:5-65, nreg_kernels_common.h:7-61.  The pair's
+ * separation is drifted by dt with the relative velocity before the
+ * softened 1/R and 1/R^3; u is m_i times the sum of m_j / R.
+ * ------------------------------------------------------------------------- */
+void SYM(nreg_x)(size_t ni, const real *im, const real *irx, const real *iry,
+                 const real *irz, const real *ie2,
+                 const real *ivx, const real *ivy, const real *ivz,
+                 size_t nj, const real *jm, const real *jrx, const real *jry,
+                 const real *jrz, const real *je2,
+                 const real *jvx, const real *jvy, const real *jvz,
+                 real dt, real *idrx, real *idry, real *idrz,
+                 real *iax, real *iay, real *iaz, real *iu)
+{
+    for (size_t i = 0; i < ni; ++i) {
+        real sx = 0, sy = 0, sz = 0, ax = 0, ay = 0, az = 0, u = 0;
+        for (size_t j = 0; j < nj; ++j) {
+            real dx = irx[i] - jrx[j];
+            real dy = iry[i] - jry[j];
+            real dz = irz[i] - jrz[j];
+            real e2 = ie2[i] + je2[j];
+            real ux = ivx[i] - jvx[j];
+            real uy = ivy[i] - jvy[j];
+            real uz = ivz[i] - jvz[j];
+            dx += ux * dt;
+            dy += uy * dt;
+            dz += uz * dt;
+            real r2 = dx * dx + dy * dy + dz * dz;
+            int mask = r2 > 0;
+            real q2 = soft_inv_r2(r2, e2, mask);
+            real q1 = R_SQRT(q2);
+            real q3 = q2 * q1;
+            q3 *= jm[j];
+            sx += jm[j] * dx;
+            sy += jm[j] * dy;
+            sz += jm[j] * dz;
+            ax -= q3 * dx;
+            ay -= q3 * dy;
+            az -= q3 * dz;
+            u += jm[j] * q1;
+        }
+        idrx[i] = sx;
+        idry[i] = sy;
+        idrz[i] = sz;
+        iax[i] = ax;
+        iay[i] = ay;
+        iaz[i] = az;
+        iu[i] = im[i] * u;
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * NREG V step: nreg_kernels.c:68-116, nreg_kernels_common.h:63-100.  The
+ * relative velocity is kicked by dt with the relative acceleration; k is m_i
+ * times the sum of m_j v^2.
+ * ------------------------------------------------------------------------- */
+void SYM(nreg_v)(size_t ni, const real *im, const real *ivx, const real *ivy,
+                 const real *ivz, const real *iax, const real *iay,
+                 const real *iaz,
+                 size_t nj, const real *jm, const real *jvx, const real *jvy,
+                 const real *jvz, const real *jax, const real *jay,
+                 const real *jaz,
+                 real dt, real *idvx, real *idvy, real *idvz, real *ik)
+{
+    for (size_t i = 0; i < ni; ++i) {
+        real sx = 0, sy = 0, sz = 0, k = 0;
+        for (size_t j = 0; j < nj; ++j) {
+            real ux = ivx[i] - jvx[j];
+            real uy = ivy[i] - jvy[j];
+            real uz = ivz[i] - jvz[j];
+            real bx = iax[i] - jax[j];
+            real by = iay[i] - jay[j];
+            real bz = iaz[i] - jaz[j];
+            ux += bx * dt;
+            uy += by * dt;
+            uz += bz * dt;
+            real v2 = ux * ux + uy * uy + uz * uz;
+            sx += jm[j] * ux;
+            sy += jm[j] * uy;
+            sz += jm[j] * uz;
+            k += jm[j] * v2;
+        }
+        idvx[i] = sx;
+        idvy[i] = sy;
+        idvz[i] = sz;
+        ik[i] = im[i] * k;
+    }
+}
+
+/* ---------------------------------------------------------------------------
  * Universal-variable Kepler solver: universal_kepler_solver.h:17-614.
  * ------------------------------------------------------------------------- */
 static inline real stumpff_c0(real z)

@@ -2,7 +2,8 @@
 
 Runs only where the reference checkout is present: `make -C oracle ref`
 compiles tupan/lib/src/{phi,acc,acc_jerk,snap_crackle,tstep,sakura,
-kepler_solver}_kernel.c (FP64 and FP32 builds, the reference's cffi flags)
+kepler_solver}_kernel.c and nreg_kernels.c (FP64 and FP32 builds, the
+reference's cffi flags)
 into oracle/_ref, and this script calls those libraries through ctypes on
 seeded inputs.  The fixtures are inputs and outputs only.
 
@@ -14,9 +15,9 @@ Fixtures (numpy .npz, float64 unless noted):
       U[0,1), positions and velocities U[0,10), accelerations and jerks
       U[-1,1) for SnapCrackle, eps2 either 0 or U[0,0.01); B shares one
       particle position with A (a coincident pair) and A contains a
-      duplicated particle; outputs of Phi, Acc, AccJerk, SnapCrackle and
-      Tstep(eta=1/64) for (A,B), (B,A) and (A,A), and for A against every
-      prefix B[:k], k in PREFIXES.
+      duplicated particle; outputs of Phi, Acc, AccJerk, SnapCrackle,
+      Tstep(eta=1/64), NREG X and NREG V (dt=1/64) for (A,B), (B,A) and
+      (A,A), and for A against every prefix B[:k], k in PREFIXES.
   sakura_f64.npz
       Sakura dr, dv for flags -2, -1, 0, 1, 2 and dt in {1/64, 0.5} on a
       random 24-particle set (eps2 = 0 and eps2 > 0) and on a bound 8-body
@@ -37,6 +38,7 @@ sys.path.insert(0, ROOT)
 from oracle import Reference, SPECS  # noqa: E402
 
 PREFIXES = (1, 2, 3, 7, 16, 33, 61)
+SCALAR_KERNELS = ("tstep", "nreg_x", "nreg_v")   # eta or dt = 1/64
 FIELDS = ("mass", "rx", "ry", "rz", "eps2", "vx", "vy", "vz",
           "ax", "ay", "az", "jx", "jy", "jz")
 
@@ -73,7 +75,7 @@ def pairwise_fixture(prec):
         for name in SPECS:
             if name == "sakura":
                 continue
-            sc = (1.0 / 64,) if name == "tstep" else ()
+            sc = (1.0 / 64,) if name in SCALAR_KERNELS else ()
             for lab, ii, jj in (("AB", a, b), ("BA", b, a), ("AA", a, a)):
                 res = ref.pairwise(name, ii, jj, *sc)
                 for o, r in zip(SPECS[name][3], res):

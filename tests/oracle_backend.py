@@ -10,18 +10,14 @@ from collections import namedtuple
 import numpy as np
 import torch
 
-from oracle import Oracle
+from oracle import SPECS, Oracle
 
 _NAMES = {"phi_kernel": "phi", "acc_kernel": "acc",
           "acc_jerk_kernel": "acc_jerk", "snap_crackle_kernel": "snap_crackle",
           "tstep_kernel": "tstep", "sakura_kernel": "sakura",
-          "kepler_solver_kernel": "kepler"}
-# name -> (i fields, j fields, scalars, outputs)
-_SPECS = {"phi": (5, 5, 0, 1), "acc": (5, 5, 0, 3), "acc_jerk": (8, 8, 0, 6),
-          "snap_crackle": (14, 14, 0, 6), "tstep": (8, 8, 1, 2),
-          "sakura": (8, 8, 2, 6)}
-_FIELDS = ("mass", "rx", "ry", "rz", "eps2", "vx", "vy", "vz",
-           "ax", "ay", "az", "jx", "jy", "jz")
+          "kepler_solver_kernel": "kepler", "nreg_Xkernel": "nreg_x",
+          "nreg_Vkernel": "nreg_v"}
+_FIELDS = ("mass", "rx", "ry", "rz", "eps2", "vx", "vy", "vz")
 
 
 def _np(x):
@@ -63,17 +59,17 @@ class OracleKernel(object):
                     out[:2] = r[:2]
             OracleKernel.calls.append((self.name, 2, 2))
             return
-        ni_f, nj_f, nsc, nout = _SPECS[self.name]
+        ifl, jfl, sfl, ofl = SPECS[self.name]
         k = 0
         ni = int(a[k]); k += 1
-        ips = dict(zip(_FIELDS[:ni_f], [_np(x)[:ni] for x in a[k:k + ni_f]]))
-        k += ni_f
+        ips = dict(zip(ifl, [_np(x)[:ni] for x in a[k:k + len(ifl)]]))
+        k += len(ifl)
         nj = int(a[k]); k += 1
-        jps = dict(zip(_FIELDS[:nj_f], [_np(x)[:nj] for x in a[k:k + nj_f]]))
-        k += nj_f
-        sc = a[k:k + nsc]
-        k += nsc
-        outs = a[k:k + nout]
+        jps = dict(zip(jfl, [_np(x)[:nj] for x in a[k:k + len(jfl)]]))
+        k += len(jfl)
+        sc = a[k:k + len(sfl)]
+        k += len(sfl)
+        outs = a[k:k + len(ofl)]
         res = self.orc.pairwise(self.name, ips, jps, *sc)
         for out, r in zip(outs, res):
             if isinstance(out, torch.Tensor):

@@ -17,7 +17,7 @@ HEADER = os.path.join(ROOT, "include", "tupan_hip.h")
 def declared_functions():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    names = re.findall(r"^[A-Za-z_][\w\s\*]*?\b([a-z_][a-z0-9_]*)\s*\(",
+    names = re.findall(r"^[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\(",
                        text, flags=re.M)
     return sorted(set(n for n in names if n not in ("defined",)))
 

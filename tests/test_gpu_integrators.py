@@ -52,12 +52,18 @@ def run(method, eta, host, steps, backend, t_end=1.0):
     ("hermite6", 1.0 / 256, 3, 128),
     ("asakura", 0.05, 3, 96),
     ("sia21h.dkd", 0.02, 2, 128),
+    ("nreg", 0.02, 4, 96),
+    ("anreg", 0.02, 4, 96),
 ])
 def test_driver_gpu_matches_oracle_backend(method, eta, steps, n):
     host = cpu_plummer(n, 4.0 / n if "sakura" not in method else 0.0, 11)
     g, tg = run(method, eta, host, steps, "hip")
     c, tc = run(method, eta, host, steps, "oracle")
-    assert tg == tc
+    if "nreg" in method:
+        # NREG's physical time comes from its energy sums (1e-13 apart)
+        assert abs(tg - tc) <= 1e-12 * abs(tc)
+    else:
+        assert tg == tc
     for k in ("rx", "ry", "rz"):
         np.testing.assert_allclose(g[k], c[k], rtol=0, atol=1e-11)
     for k in ("vx", "vy", "vz"):

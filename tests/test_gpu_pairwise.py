@@ -34,6 +34,10 @@ KERNELS = {
     "snap_crackle": ("SnapCrackle", (), ("sx", "sy", "sz", "cx", "cy", "cz"),
                      [(0, 1, 2), (3, 4, 5)]),
     "tstep": ("Tstep", (1.0 / 64,), ("tstep", "tstepij"), [(0,), (1,)]),
+    "nreg_x": ("NREG_X", (1.0 / 64,), ("mrx", "mry", "mrz", "ax", "ay", "az",
+                                        "u"), [(0, 1, 2), (3, 4, 5), (6,)]),
+    "nreg_v": ("NREG_V", (1.0 / 64,), ("mvx", "mvy", "mvz", "mk"),
+               [(0, 1, 2), (3,)]),
 }
 FIELDS = ("mass", "rx", "ry", "rz", "eps2", "vx", "vy", "vz",
           "ax", "ay", "az", "jx", "jy", "jz")

@@ -250,3 +250,54 @@ def test_particle_system_structure():
         ps.register_auxiliary_attribute("phi", "real")
     st = ps.get_state()
     assert st.dtype.names[:3] == ("id", "mass", "eps2")
+
+
+@pytest.mark.parametrize("method", ["nreg", "anreg"])
+def test_nreg_two_body_conserves_energy(method):
+    """NREG's time-transformed leapfrog is exact in shape for a Kepler pair
+    (the logarithmic-Hamiltonian leapfrog): after many steps of an
+    eccentric orbit the energy and angular momentum stay at round-off and
+    the separation stays on the orbit's [pericentre, apocentre]."""
+    d = kepler_pair(0.6)
+    ps = make_ps(d)
+    from tupan_amd.integrator import Integrator
+    type(ps).t_curr = 0.0
+    it = Integrator(0.1, 0.0, ps, method=method)
+    e0 = ps.kinetic_energy + ps.potential_energy
+    L0 = float(ps.mass[0] * (ps.rx[0] * ps.vy[0] - ps.ry[0] * ps.vx[0])
+               + ps.mass[1] * (ps.rx[1] * ps.vy[1] - ps.ry[1] * ps.vx[1]))
+    seps = []
+    for k in range(200):
+        it.evolve_step(1e9)
+        q = it.particle_system
+        seps.append(float(((q.rx[0] - q.rx[1])**2
+                           + (q.ry[0] - q.ry[1])**2)**0.5))
+    q = it.particle_system
+    e1 = q.kinetic_energy + q.potential_energy
+    L1 = float(q.mass[0] * (q.rx[0] * q.vy[0] - q.ry[0] * q.vx[0])
+               + q.mass[1] * (q.rx[1] * q.vy[1] - q.ry[1] * q.vx[1]))
+    assert abs((e1 - e0) / e0) < 1e-12
+    assert abs((L1 - L0) / L0) < 1e-12
+    # orbit: a = 1 / (2 |E| / m) with total mass 1, e = 0.6 -> r in [a(1-e), a(1+e)]
+    a = 1.0 / (-2 * e0 / (0.5 * 0.5))
+    assert min(seps) > a * 0.4 * (1 - 1e-9) and max(seps) < a * 1.6 * (1 + 1e-9)
+    assert it.time > 0
+
+
+def test_nreg_plummer_second_order():
+    """On a 24-body Plummer sphere the energy error over a fixed span of
+    regularised time drops about fourfold when the step is halved."""
+    from tupan_amd.ics.plummer import make_plummer
+    from tupan_amd.integrator import Integrator
+    errs = []
+    for h, n in ((0.02, 10), (0.01, 20)):
+        p = make_plummer(24, 0.02, ("equalmass",), seed=5, device="cpu")
+        type(p).t_curr = 0.0
+        e0 = p.kinetic_energy + p.potential_energy
+        it = Integrator(h, 0.0, p, method="nreg")
+        for _ in range(n):
+            it.evolve_step(1e9)
+        q = it.particle_system
+        errs.append(abs((q.kinetic_energy + q.potential_energy - e0) / e0))
+    assert errs[0] < 1e-3
+    assert 2.5 < errs[0] / errs[1] < 6.5, errs

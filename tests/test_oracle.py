@@ -30,11 +30,11 @@ def sets(g, tag, n=None):
 
 @pytest.mark.parametrize("prec", ["float64", "float32"])
 @pytest.mark.parametrize("name", ["phi", "acc", "acc_jerk", "snap_crackle",
-                                  "tstep"])
+                                  "tstep", "nreg_x", "nreg_v"])
 def test_pairwise_bitexact_vs_reference(golden, prec, name):
     g = golden("pairwise_%s.npz" % ("f64" if prec == "float64" else "f32"))
     orc = Oracle(prec)
-    sc = (1.0 / 64,) if name == "tstep" else ()
+    sc = (1.0 / 64,) if name in ("tstep", "nreg_x", "nreg_v") else ()
     for soft in (0, 1):
         tag = "s%d" % soft
         a, b = sets(g, tag)
@@ -261,3 +261,38 @@ def test_counters_count_solves_and_leapfrogs():
     solves, iters, retries, leap, fails = orc.counters()
     assert solves + leap == n * (n - 1)
     assert iters >= solves and fails == 0
+
+
+def test_nreg_two_body_closed_form():
+    """NREG X and V for one pair against their closed forms: the separation
+    drifted by dt, the softened 1/R and 1/R^3, m_i m_j / R and
+    m_i m_j |v + a dt|^2; self pairs contribute m_j * 0 and nothing else."""
+    orc = Oracle("float64")
+    d = {"mass": np.array([0.3, 0.7]), "eps2": np.array([0.01, 0.02]),
+         "rx": np.array([0.5, -0.2]), "ry": np.array([0.1, 0.4]),
+         "rz": np.array([-0.3, 0.2]), "vx": np.array([0.2, -0.1]),
+         "vy": np.array([0.0, 0.3]), "vz": np.array([0.1, 0.1]),
+         "ax": np.array([0.05, -0.02]), "ay": np.array([0.0, 0.01]),
+         "az": np.array([-0.03, 0.04])}
+    dt = 0.125
+    mrx, mry, mrz, ax, ay, az, u = orc.nreg_x(d, d, dt)
+    r = np.array([d["rx"][0] - d["rx"][1], d["ry"][0] - d["ry"][1],
+                  d["rz"][0] - d["rz"][1]])
+    v = np.array([d["vx"][0] - d["vx"][1], d["vy"][0] - d["vy"][1],
+                  d["vz"][0] - d["vz"][1]])
+    rd = r + v * dt
+    R = np.sqrt(rd @ rd + 0.03)
+    m0, m1 = d["mass"]
+    np.testing.assert_allclose([mrx[0], mry[0], mrz[0]], m1 * rd, rtol=1e-14)
+    np.testing.assert_allclose([ax[0], ay[0], az[0]], -m1 * rd / R**3,
+                               rtol=1e-13)
+    np.testing.assert_allclose(u[0], m0 * m1 / R, rtol=1e-14)
+    np.testing.assert_allclose([mrx[1], mry[1], mrz[1]], -m0 * rd,
+                               rtol=1e-14)
+    mvx, mvy, mvz, k = orc.nreg_v(d, d, dt)
+    a = np.array([d["ax"][0] - d["ax"][1], d["ay"][0] - d["ay"][1],
+                  d["az"][0] - d["az"][1]])
+    w = v + a * dt
+    np.testing.assert_allclose([mvx[0], mvy[0], mvz[0]], m1 * w, rtol=1e-14)
+    np.testing.assert_allclose(k[0], m0 * m1 * (w @ w), rtol=1e-14)
+    np.testing.assert_allclose(k[1], m1 * m0 * (w @ w), rtol=1e-14)

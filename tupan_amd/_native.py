@@ -18,7 +18,7 @@ MODE_SPLIT32 = 1
 MODE_IEEE64 = 2
 MODES = {"fp64": MODE_FP64, "split32": MODE_SPLIT32, "ieee64": MODE_IEEE64}
 
-K_PHI, K_ACC, K_ACC_JERK, K_SNAP_CRACKLE, K_TSTEP = range(5)
+K_PHI, K_ACC, K_ACC_JERK, K_SNAP_CRACKLE, K_TSTEP, K_NREG_X, K_NREG_V = range(7)
 
 _lib = None
 
@@ -163,7 +163,7 @@ def device_info():
 # Names exported by the shared libraries (used by tests that check the ABI).
 REFERENCE_ABI = ("phi_kernel", "acc_kernel", "acc_jerk_kernel",
                  "snap_crackle_kernel", "tstep_kernel", "sakura_kernel",
-                 "kepler_solver_kernel")
+                 "kepler_solver_kernel", "nreg_Xkernel", "nreg_Vkernel")
 DEVICE_ABI = ("tupan_last_error", "tupan_version", "tupan_device_info",
               "tupan_tune_set", "tupan_tune_get", "tupan_tune_shape",
               "tupan_timing", "tupan_timing_read",

@@ -155,7 +155,7 @@ static void host_pairwise(const char* fn, int kernel, size_t ni, const T* const*
 {
     const int nin = kernel_in_fields(kernel);
     const int nout = kernel_out_fields(kernel);
-    size_t lens[6];
+    size_t lens[8];
     for (int k = 0; k < nout; ++k) lens[k] = ni;
     if (ni == 0) return;
     const size_t bytes = sizeof(T) * (nin * (ni + nj) + nout * ni);
@@ -167,7 +167,7 @@ static void host_pairwise(const char* fn, int kernel, size_t ni, const T* const*
     }
     const T* ip[14];
     const T* jp[14];
-    T* op[6];
+    T* op[8];
     int rc = TUPAN_OK;
     T* cur = base;
     for (int f = 0; f < nin && rc == TUPAN_OK; ++f) {
@@ -373,6 +373,34 @@ void tstep_kernel(const unsigned long ni, const double* _im, const double* _irx,
     host_pairwise<double>("tstep_kernel", K_TSTEP, ni, ih, nj, jh, oh, eta);
 }
 
+void nreg_Xkernel(const unsigned long ni, const double* _im, const double* _irx,
+                  const double* _iry, const double* _irz, const double* _ie2,
+                  const double* _ivx, const double* _ivy, const double* _ivz,
+                  const unsigned long nj, const double* _jm, const double* _jrx,
+                  const double* _jry, const double* _jrz, const double* _je2,
+                  const double* _jvx, const double* _jvy, const double* _jvz, const double dt,
+                  double* _idrx, double* _idry, double* _idrz, double* _iax, double* _iay,
+                  double* _iaz, double* _iu)
+{
+    const double* ih[8] = {_im, _irx, _iry, _irz, _ie2, _ivx, _ivy, _ivz};
+    const double* jh[8] = {_jm, _jrx, _jry, _jrz, _je2, _jvx, _jvy, _jvz};
+    double* oh[7] = {_idrx, _idry, _idrz, _iax, _iay, _iaz, _iu};
+    host_pairwise<double>("nreg_Xkernel", K_NREG_X, ni, ih, nj, jh, oh, dt);
+}
+
+void nreg_Vkernel(const unsigned long ni, const double* _im, const double* _ivx,
+                  const double* _ivy, const double* _ivz, const double* _iax,
+                  const double* _iay, const double* _iaz, const unsigned long nj,
+                  const double* _jm, const double* _jvx, const double* _jvy, const double* _jvz,
+                  const double* _jax, const double* _jay, const double* _jaz, const double dt,
+                  double* _idvx, double* _idvy, double* _idvz, double* _ik)
+{
+    const double* ih[7] = {_im, _ivx, _ivy, _ivz, _iax, _iay, _iaz};
+    const double* jh[7] = {_jm, _jvx, _jvy, _jvz, _jax, _jay, _jaz};
+    double* oh[4] = {_idvx, _idvy, _idvz, _ik};
+    host_pairwise<double>("nreg_Vkernel", K_NREG_V, ni, ih, nj, jh, oh, dt);
+}
+
 void sakura_kernel(const unsigned long ni, const double* _im, const double* _irx,
                    const double* _iry, const double* _irz, const double* _ie2,
                    const double* _ivx, const double* _ivy, const double* _ivz,
@@ -439,9 +467,12 @@ void kepler_solver_kernel(const double* _im, const double* _irx, const double* _
 int tupan_host_pairwise_f32(int kernel, size_t ni, const float* const* ih, size_t nj,
                             const float* const* jh, float* const* oh, float eta)
 {
-    static const char* names[5] = {"phi_kernel", "acc_kernel", "acc_jerk_kernel",
-                                   "snap_crackle_kernel", "tstep_kernel"};
-    if (kernel < 0 || kernel > 4) return set_error(TUPAN_ERR_ARG, "unknown pairwise kernel");
+    static const char* names[kNumKernels] = {"phi_kernel",          "acc_kernel",
+                                             "acc_jerk_kernel",     "snap_crackle_kernel",
+                                             "tstep_kernel",        "nreg_Xkernel",
+                                             "nreg_Vkernel"};
+    if (kernel < 0 || kernel >= kNumKernels)
+        return set_error(TUPAN_ERR_ARG, "unknown pairwise kernel");
     g_error.clear();
     host_pairwise<float>(names[kernel], kernel, ni, ih, nj, jh, oh, eta);
     return g_error.empty() ? TUPAN_OK : TUPAN_ERR_HIP;

@@ -79,6 +79,31 @@ void tstep_kernel(const unsigned int ni, const float* _im, const float* _irx, co
     tupan_host_pairwise_f32(TUPAN_K_TSTEP, ni, ih, nj, jh, oh, eta);
 }
 
+void nreg_Xkernel(const unsigned int ni, const float* _im, const float* _irx, const float* _iry,
+                  const float* _irz, const float* _ie2, const float* _ivx, const float* _ivy,
+                  const float* _ivz, const unsigned int nj, const float* _jm, const float* _jrx,
+                  const float* _jry, const float* _jrz, const float* _je2, const float* _jvx,
+                  const float* _jvy, const float* _jvz, const float dt, float* _idrx,
+                  float* _idry, float* _idrz, float* _iax, float* _iay, float* _iaz, float* _iu)
+{
+    const float* ih[8] = {_im, _irx, _iry, _irz, _ie2, _ivx, _ivy, _ivz};
+    const float* jh[8] = {_jm, _jrx, _jry, _jrz, _je2, _jvx, _jvy, _jvz};
+    float* oh[7] = {_idrx, _idry, _idrz, _iax, _iay, _iaz, _iu};
+    tupan_host_pairwise_f32(TUPAN_K_NREG_X, ni, ih, nj, jh, oh, dt);
+}
+
+void nreg_Vkernel(const unsigned int ni, const float* _im, const float* _ivx, const float* _ivy,
+                  const float* _ivz, const float* _iax, const float* _iay, const float* _iaz,
+                  const unsigned int nj, const float* _jm, const float* _jvx, const float* _jvy,
+                  const float* _jvz, const float* _jax, const float* _jay, const float* _jaz,
+                  const float dt, float* _idvx, float* _idvy, float* _idvz, float* _ik)
+{
+    const float* ih[7] = {_im, _ivx, _ivy, _ivz, _iax, _iay, _iaz};
+    const float* jh[7] = {_jm, _jvx, _jvy, _jvz, _jax, _jay, _jaz};
+    float* oh[4] = {_idvx, _idvy, _idvz, _ik};
+    tupan_host_pairwise_f32(TUPAN_K_NREG_V, ni, ih, nj, jh, oh, dt);
+}
+
 void sakura_kernel(const unsigned int ni, const float* _im, const float* _irx,
                    const float* _iry, const float* _irz, const float* _ie2, const float* _ivx,
                    const float* _ivy, const float* _ivz, const unsigned int nj, const float* _jm,

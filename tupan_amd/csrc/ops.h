@@ -126,7 +126,7 @@ template <class T>
 struct PairArgs {
     const T* ip[14];
     const T* jp[14];
-    T* out[6];
+    T* out[8];
     int ni, nj;
     int nsplit, jchunk;
     double* partial;
@@ -736,6 +736,147 @@ struct OpSplitTstep {
     __device__ static void store(const Args& a, int i, const double* t)
     {
         OpTstep<Fast64>::store(a, i, t);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// NREG X step (nreg_kernels_common.h:7-61, nreg_kernels.c:5-65): relative
+// positions drifted by dt, then sum_j m_j r, the acceleration and
+// sum_j m_j / R (times m_i at the end).
+// i, j: m rx ry rz e2 vx vy vz; scalar dt; out: mrx mry mrz ax ay az u.
+// ---------------------------------------------------------------------------
+template <class P>
+struct OpNregX {
+    using T = typename P::T;
+    using Args = PairArgs<T>;
+    using Frame = NoFrame;
+    using Slot = Slots<T, 8>;
+    struct IReg { T m, x, y, z, e2, vx, vy, vz, dt; };
+    struct Acc { T d[7]; };
+    static constexpr int kNacc = 7;
+    static constexpr int kFlush = 1 << 30;
+
+    __device__ static Frame frame(const Args&, int) { return {}; }
+    __device__ static void load_i(const Args& a, const Frame&, int i, IReg& r)
+    {
+        r.m = a.ip[0][i]; r.x = a.ip[1][i]; r.y = a.ip[2][i]; r.z = a.ip[3][i];
+        r.e2 = a.ip[4][i]; r.vx = a.ip[5][i]; r.vy = a.ip[6][i]; r.vz = a.ip[7][i];
+        r.dt = a.eta;
+    }
+    __device__ static void load_j(const Args& a, const Frame&, int j, Slot& s)
+    {
+#pragma unroll
+        for (int f = 0; f < 8; ++f) s.v[f] = a.jp[f][j];
+    }
+    __device__ static void zero(Acc& c)
+    {
+#pragma unroll
+        for (int k = 0; k < 7; ++k) c.d[k] = 0;
+    }
+    __device__ static __forceinline__ void interact(const IReg& r, const Slot& s, Acc& c)
+    {
+        T dx = r.x - s.v[1];
+        T dy = r.y - s.v[2];
+        T dz = r.z - s.v[3];
+        const T e2 = r.e2 + s.v[4];
+        const T ux = r.vx - s.v[5];
+        const T uy = r.vy - s.v[6];
+        const T uz = r.vz - s.v[7];
+        dx = mad<P>(ux, r.dt, dx);
+        dy = mad<P>(uy, r.dt, dy);
+        dz = mad<P>(uz, r.dt, dz);
+        const T r2 = dot3<P>(dx, dy, dz, dx, dy, dz);
+        T q1, q2;
+        P::q12(r2, e2, r2 > T(0), q1, q2);
+        T q3 = q2 * q1;
+        q3 *= s.v[0];
+        c.d[0] = mad<P>(s.v[0], dx, c.d[0]);
+        c.d[1] = mad<P>(s.v[0], dy, c.d[1]);
+        c.d[2] = mad<P>(s.v[0], dz, c.d[2]);
+        c.d[3] = nmad<P>(q3, dx, c.d[3]);
+        c.d[4] = nmad<P>(q3, dy, c.d[4]);
+        c.d[5] = nmad<P>(q3, dz, c.d[5]);
+        c.d[6] = mad<P>(s.v[0], q1, c.d[6]);
+    }
+    __device__ static void flush(Acc&) {}
+    __device__ static double get(const Acc& c, int k) { return static_cast<double>(c.d[k]); }
+    __device__ static void combine(double* t, const double* p)
+    {
+#pragma unroll
+        for (int k = 0; k < 7; ++k) t[k] += p[k];
+    }
+    __device__ static void store(const Args& a, int i, const double* t)
+    {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) a.out[k][i] = static_cast<T>(t[k]);
+        a.out[6][i] = a.ip[0][i] * static_cast<T>(t[6]);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// NREG V step (nreg_kernels_common.h:63-100, nreg_kernels.c:68-116):
+// relative velocities kicked by dt, then sum_j m_j v and sum_j m_j v^2
+// (times m_i at the end).
+// i, j: m vx vy vz ax ay az; scalar dt; out: mvx mvy mvz k.
+// ---------------------------------------------------------------------------
+template <class P>
+struct OpNregV {
+    using T = typename P::T;
+    using Args = PairArgs<T>;
+    using Frame = NoFrame;
+    using Slot = Slots<T, 8>;
+    struct IReg { T vx, vy, vz, ax, ay, az, dt; };
+    struct Acc { T d[4]; };
+    static constexpr int kNacc = 4;
+    static constexpr int kFlush = 1 << 30;
+
+    __device__ static Frame frame(const Args&, int) { return {}; }
+    __device__ static void load_i(const Args& a, const Frame&, int i, IReg& r)
+    {
+        r.vx = a.ip[1][i]; r.vy = a.ip[2][i]; r.vz = a.ip[3][i];
+        r.ax = a.ip[4][i]; r.ay = a.ip[5][i]; r.az = a.ip[6][i];
+        r.dt = a.eta;
+    }
+    __device__ static void load_j(const Args& a, const Frame&, int j, Slot& s)
+    {
+#pragma unroll
+        for (int f = 0; f < 7; ++f) s.v[f] = a.jp[f][j];
+        s.v[7] = 0;
+    }
+    __device__ static void zero(Acc& c)
+    {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) c.d[k] = 0;
+    }
+    __device__ static __forceinline__ void interact(const IReg& r, const Slot& s, Acc& c)
+    {
+        T ux = r.vx - s.v[1];
+        T uy = r.vy - s.v[2];
+        T uz = r.vz - s.v[3];
+        const T bx = r.ax - s.v[4];
+        const T by = r.ay - s.v[5];
+        const T bz = r.az - s.v[6];
+        ux = mad<P>(bx, r.dt, ux);
+        uy = mad<P>(by, r.dt, uy);
+        uz = mad<P>(bz, r.dt, uz);
+        const T v2 = dot3<P>(ux, uy, uz, ux, uy, uz);
+        c.d[0] = mad<P>(s.v[0], ux, c.d[0]);
+        c.d[1] = mad<P>(s.v[0], uy, c.d[1]);
+        c.d[2] = mad<P>(s.v[0], uz, c.d[2]);
+        c.d[3] = mad<P>(s.v[0], v2, c.d[3]);
+    }
+    __device__ static void flush(Acc&) {}
+    __device__ static double get(const Acc& c, int k) { return static_cast<double>(c.d[k]); }
+    __device__ static void combine(double* t, const double* p)
+    {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) t[k] += p[k];
+    }
+    __device__ static void store(const Args& a, int i, const double* t)
+    {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) a.out[k][i] = static_cast<T>(t[k]);
+        a.out[3][i] = a.ip[0][i] * static_cast<T>(t[3]);
     }
 };
 

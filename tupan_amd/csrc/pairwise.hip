@@ -82,7 +82,7 @@ struct TuneEntry {
     int shape = 0;
     int wg_per_cu = 4;
 };
-static TuneEntry g_tune[5][4][kBuckets];
+static TuneEntry g_tune[kNumKernels][4][kBuckets];
 
 static int mode_slot(int mode, bool f32)
 {
@@ -137,6 +137,12 @@ int dispatch<double>(int kernel, int mode, PairArgs<double>& a, hipStream_t s)
         if (mode == TUPAN_MODE_SPLIT32) TUPAN_RUN(OpSplitTstep, TUPAN_LIGHT_SHAPES, K_TSTEP, slot);
         if (mode == TUPAN_MODE_IEEE64) TUPAN_RUN(OpTstep<Ieee64>, TUPAN_LIGHT_SHAPES, K_TSTEP, slot);
         TUPAN_RUN(OpTstep<Fast64>, TUPAN_LIGHT_SHAPES, K_TSTEP, slot);
+    case K_NREG_X:
+        if (mode == TUPAN_MODE_IEEE64) TUPAN_RUN(OpNregX<Ieee64>, TUPAN_LIGHT_SHAPES, K_NREG_X, slot);
+        TUPAN_RUN(OpNregX<Fast64>, TUPAN_LIGHT_SHAPES, K_NREG_X, slot);
+    case K_NREG_V:
+        if (mode == TUPAN_MODE_IEEE64) TUPAN_RUN(OpNregV<Ieee64>, TUPAN_LIGHT_SHAPES, K_NREG_V, slot);
+        TUPAN_RUN(OpNregV<Fast64>, TUPAN_LIGHT_SHAPES, K_NREG_V, slot);
     default:
         return set_error(TUPAN_ERR_ARG, "unknown pairwise kernel");
     }
@@ -153,14 +159,16 @@ int dispatch<float>(int kernel, int, PairArgs<float>& a, hipStream_t s)
     case K_SNAP_CRACKLE:
         TUPAN_RUN(OpSnapCrackle<Fast32>, TUPAN_HEAVY_SHAPES, K_SNAP_CRACKLE, slot);
     case K_TSTEP: TUPAN_RUN(OpTstep<Fast32>, TUPAN_LIGHT_SHAPES, K_TSTEP, slot);
+    case K_NREG_X: TUPAN_RUN(OpNregX<Fast32>, TUPAN_LIGHT_SHAPES, K_NREG_X, slot);
+    case K_NREG_V: TUPAN_RUN(OpNregV<Fast32>, TUPAN_LIGHT_SHAPES, K_NREG_V, slot);
     default: return set_error(TUPAN_ERR_ARG, "unknown pairwise kernel");
     }
 }
 
 int tune_set(int kernel, int slot, int bucket, int shape, int wg_per_cu)
 {
-    if (kernel < 0 || kernel > 4 || slot < 0 || slot > 3 || bucket < 0 || bucket >= kBuckets ||
-        shape < 0 || shape >= kShapes || wg_per_cu < 1 || wg_per_cu > 64)
+    if (kernel < 0 || kernel >= kNumKernels || slot < 0 || slot > 3 || bucket < 0 ||
+        bucket >= kBuckets || shape < 0 || shape >= kShapes || wg_per_cu < 1 || wg_per_cu > 64)
         return set_error(TUPAN_ERR_ARG, "tuning entry out of range");
     g_tune[kernel][slot][bucket].shape = shape;
     g_tune[kernel][slot][bucket].wg_per_cu = wg_per_cu;
@@ -169,7 +177,8 @@ int tune_set(int kernel, int slot, int bucket, int shape, int wg_per_cu)
 
 int tune_get(int kernel, int slot, int bucket, int* shape, int* wg_per_cu)
 {
-    if (kernel < 0 || kernel > 4 || slot < 0 || slot > 3 || bucket < 0 || bucket >= kBuckets)
+    if (kernel < 0 || kernel >= kNumKernels || slot < 0 || slot > 3 || bucket < 0 ||
+        bucket >= kBuckets)
         return set_error(TUPAN_ERR_ARG, "tuning entry out of range");
     if (shape) *shape = g_tune[kernel][slot][bucket].shape;
     if (wg_per_cu) *wg_per_cu = g_tune[kernel][slot][bucket].wg_per_cu;
@@ -178,7 +187,7 @@ int tune_get(int kernel, int slot, int bucket, int* shape, int* wg_per_cu)
 
 int tune_shape(int kernel, int shape, int* block, int* wpt, int* tile, int* unroll)
 {
-    if (kernel < 0 || kernel > 4 || shape < 0 || shape >= kShapes)
+    if (kernel < 0 || kernel >= kNumKernels || shape < 0 || shape >= kShapes)
         return set_error(TUPAN_ERR_ARG, "shape out of range");
     const int* d = kernel == K_SNAP_CRACKLE ? kHeavyShape[shape] : kLightShape[shape];
     if (block) *block = d[0];

@@ -14,10 +14,13 @@ enum KernelId {
     K_ACC_JERK = TUPAN_K_ACC_JERK,
     K_SNAP_CRACKLE = TUPAN_K_SNAP_CRACKLE,
     K_TSTEP = TUPAN_K_TSTEP,
+    K_NREG_X = TUPAN_K_NREG_X,
+    K_NREG_V = TUPAN_K_NREG_V,
 };
+constexpr int kNumKernels = 7;
 
 // Number of i/j input fields and of outputs of each pairwise kernel, in the
-// reference's argument order (tupan/lib/extensions.py:100-387).
+// reference's argument order (tupan/lib/extensions.py:100-387, 511-615).
 inline int kernel_in_fields(int k)
 {
     switch (k) {
@@ -26,6 +29,8 @@ inline int kernel_in_fields(int k)
     case K_ACC_JERK: return 8;
     case K_SNAP_CRACKLE: return 14;
     case K_TSTEP: return 8;
+    case K_NREG_X: return 8;
+    case K_NREG_V: return 7;
     default: return -1;
     }
 }
@@ -38,6 +43,8 @@ inline int kernel_out_fields(int k)
     case K_ACC_JERK: return 6;
     case K_SNAP_CRACKLE: return 6;
     case K_TSTEP: return 2;
+    case K_NREG_X: return 7;
+    case K_NREG_V: return 4;
     default: return -1;
     }
 }

@@ -81,6 +81,7 @@ class Base(object):
 
 
 from . import hermite  # noqa: E402
+from . import nreg  # noqa: E402
 from . import sakura  # noqa: E402
 from . import sia  # noqa: E402
 
@@ -91,6 +92,7 @@ class Integrator(object):
     PROVIDED_METHODS.extend(sia.SIA.PROVIDED_METHODS)
     PROVIDED_METHODS.extend(sakura.Sakura.PROVIDED_METHODS)
     PROVIDED_METHODS.extend(hermite.Hermite.PROVIDED_METHODS)
+    PROVIDED_METHODS.extend(nreg.NREG.PROVIDED_METHODS)
 
     def __init__(self, eta, time, ps, **kwargs):
         logger = logging.getLogger(__name__)
@@ -102,6 +104,8 @@ class Integrator(object):
             self.integrator = sakura.Sakura(eta, time, ps, method, **kwargs)
         elif method in hermite.Hermite.PROVIDED_METHODS:
             self.integrator = hermite.Hermite(eta, time, ps, method, **kwargs)
+        elif method in nreg.NREG.PROVIDED_METHODS:
+            self.integrator = nreg.NREG(eta, time, ps, method, **kwargs)
         else:
             logger.critical("Unexpected integration method: '%s'. Provided "
                             "methods: %s", method, str(self.PROVIDED_METHODS))

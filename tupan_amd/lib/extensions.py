@@ -13,8 +13,8 @@ product backend and the default of the singletons.  As in the reference
 backend name; a backend may also be given as a class with the reference
 kernel-object protocol, constructed as ``cls(prec, name)``.
 
-PNAcc, NREG_X and NREG_V (post-Newtonian and NREG kernels) are not provided
-by this build; their singletons raise ``NotImplementedError`` when used.
+PNAcc (the post-Newtonian kernel) is not provided by this build; its
+singleton raises ``NotImplementedError`` when used.
 """
 import collections
 import logging
@@ -29,6 +29,8 @@ __all__ = ["Phi", "phi",
            "Tstep", "tstep",
            "Sakura", "sakura",
            "Kepler", "kepler",
+           "NREG_X", "nreg_x",
+           "NREG_V", "nreg_v",
            "get_kernel", "install"]
 
 logger = logging.getLogger(__name__)
@@ -284,6 +286,57 @@ class Kepler(AbstractExtension):
                    (ips.rx, ips.ry, ips.rz, ips.vx, ips.vy, ips.vz))
 
 
+class NREG_X(AbstractExtension):
+    """NREG position step: sum_j m_j r_ij(t + dt), the acceleration at the
+    drifted separations and m_i sum_j m_j / R (extensions.py:511-565)."""
+
+    def __init__(self, backend, prec, **kw):
+        self.kernel = get_kernel("nreg_Xkernel", backend, prec, **kw)
+        cty = self.kernel.cty
+        self.argtypes = ((cty.c_uint,) + (cty.c_real_p,) * 8
+                         + (cty.c_uint,) + (cty.c_real_p,) * 8
+                         + (cty.c_real,))
+        self.restypes = (cty.c_real_p,) * 7
+
+    def set_args(self, ips, jps, dt):
+        ni, nj = ips.n, jps.n
+        self.kernel.set_gsize(ni, nj)
+        _register(ips, "mrx", "mry", "mrz", "ax", "ay", "az", "u")
+        self._bind(ni, nj,
+                   (ni, ips.mass, ips.rx, ips.ry, ips.rz,
+                    ips.eps2, ips.vx, ips.vy, ips.vz,
+                    nj, jps.mass, jps.rx, jps.ry, jps.rz,
+                    jps.eps2, jps.vx, jps.vy, jps.vz,
+                    dt),
+                   (ips.mrx, ips.mry, ips.mrz, ips.ax, ips.ay, ips.az,
+                    ips.u))
+
+
+class NREG_V(AbstractExtension):
+    """NREG velocity step: sum_j m_j v_ij(t + dt) and m_i sum_j m_j v_ij^2
+    (extensions.py:568-615)."""
+
+    def __init__(self, backend, prec, **kw):
+        self.kernel = get_kernel("nreg_Vkernel", backend, prec, **kw)
+        cty = self.kernel.cty
+        self.argtypes = ((cty.c_uint,) + (cty.c_real_p,) * 7
+                         + (cty.c_uint,) + (cty.c_real_p,) * 7
+                         + (cty.c_real,))
+        self.restypes = (cty.c_real_p,) * 4
+
+    def set_args(self, ips, jps, dt):
+        ni, nj = ips.n, jps.n
+        self.kernel.set_gsize(ni, nj)
+        _register(ips, "mvx", "mvy", "mvz", "mk")
+        self._bind(ni, nj,
+                   (ni, ips.mass, ips.vx, ips.vy, ips.vz,
+                    ips.ax, ips.ay, ips.az,
+                    nj, jps.mass, jps.vx, jps.vy, jps.vz,
+                    jps.ax, jps.ay, jps.az,
+                    dt),
+                   (ips.mvx, ips.mvy, ips.mvz, ips.mk))
+
+
 class _NotProvided(object):
     """Singleton placeholder for a reference kernel outside this build."""
 
@@ -292,7 +345,7 @@ class _NotProvided(object):
 
     def calc(self, *args, **kwargs):
         raise NotImplementedError(
-            "%s (post-Newtonian / NREG kernels) is not provided by tupan_amd"
+            "%s (the post-Newtonian kernel) is not provided by tupan_amd"
             % self.name)
 
     set_args = run = get_result = calc
@@ -302,9 +355,8 @@ backend = "HIP"
 
 clight = Clight()
 phi = acc = acc_jerk = snap_crackle = tstep = sakura = kepler = None
+nreg_x = nreg_v = None
 pnacc = _NotProvided("pnacc")
-nreg_x = _NotProvided("nreg_x")
-nreg_v = _NotProvided("nreg_v")
 
 
 def install(backend_=None, prec=None, **kw):
@@ -312,6 +364,7 @@ def install(backend_=None, prec=None, **kw):
     and precision.  Called at import with the HIP backend and
     ``ctype.prec``."""
     global backend, phi, acc, acc_jerk, snap_crackle, tstep, sakura, kepler
+    global nreg_x, nreg_v
     backend = "HIP" if backend_ is None else backend_
     prec = ctype.prec if prec is None else prec
     phi = Phi(backend, prec, **kw)
@@ -321,6 +374,8 @@ def install(backend_=None, prec=None, **kw):
     tstep = Tstep(backend, prec, **kw)
     sakura = Sakura(backend, prec, **kw)
     kepler = Kepler(backend, prec, **kw)
+    nreg_x = NREG_X(backend, prec, **kw)
+    nreg_v = NREG_V(backend, prec, **kw)
 
 
 install()

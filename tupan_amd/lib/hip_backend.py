@@ -41,6 +41,8 @@ KERNEL_SPECS = {
     "tstep_kernel": (8, 8, ("real",), 2),
     "sakura_kernel": (8, 8, ("real", "int"), 6),
     "kepler_solver_kernel": (8, 0, ("real",), 6),
+    "nreg_Xkernel": (8, 8, ("real",), 7),
+    "nreg_Vkernel": (7, 7, ("real",), 4),
 }
 PAIRWISE_IDS = {
     "phi_kernel": _native.K_PHI,
@@ -48,6 +50,8 @@ PAIRWISE_IDS = {
     "acc_jerk_kernel": _native.K_ACC_JERK,
     "snap_crackle_kernel": _native.K_SNAP_CRACKLE,
     "tstep_kernel": _native.K_TSTEP,
+    "nreg_Xkernel": _native.K_NREG_X,
+    "nreg_Vkernel": _native.K_NREG_V,
 }
 
 
