@@ -103,6 +103,24 @@ def time_accjerk(ps, mode, steps, warmup):
     return wall / steps, kms / 1e3 / max(cnt, 1)
 
 
+def time_kernel(ps, cls_name, mode, steps, *scalars):
+    """(kernel seconds per call, Gint/s) of an extension kernel on ps x ps."""
+    import torch
+    from tupan_amd import _native
+    from tupan_amd.lib import extensions
+    ext = getattr(extensions, cls_name)("HIP", "float64", mode=mode)
+    ext.calc(ps, ps, *scalars)
+    torch.cuda.synchronize()
+    _native.timing(True)
+    for _ in range(steps):
+        ext.calc(ps, ps, *scalars)
+    torch.cuda.synchronize()
+    kms, cnt = _native.timing_read()
+    _native.timing(False)
+    t = kms / 1e3 / max(cnt, 1)
+    return {"kernel_ms": t * 1e3, "gint_s": ps.n * ps.n / t / 1e9}
+
+
 def energy_run(ps0, mode, steps):
     """ahermite4 (eta 0.01) for `steps` steps from a copy of ps0; returns
     |dE/E| with E from FP64 Phi and exactly rounded sums."""
@@ -544,6 +562,16 @@ def main():
                         "peak": PEAK_FP64 / 1e12,
                         "frac": flops / k_fp64 / PEAK_FP64},
     }
+
+    out["other_kernels"] = {
+        "%s_%s" % (name.lower(), mode): time_kernel(ps, name, mode, 5, *sc)
+        for name, sc in (("Acc", ()), ("Tstep", (0.01,)))
+        for mode in ("split32", "fp64")}
+    from tupan_amd.lib import extensions as _ext
+    _ext.clight.pn_order, _ext.clight.clight = 7, 128.0
+    for mode in ("split32", "fp64"):
+        out["other_kernels"]["pnacc_order7_%s" % mode] = time_kernel(
+            ps, "PNAcc", mode, 3)
 
     if args.energy_steps > 0:
         log("bench: energy check, %d ahermite4 steps per mode"

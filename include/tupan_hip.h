@@ -46,8 +46,8 @@ extern "C" {
 /* Arithmetic modes of the FP64 device entry points. */
 #define TUPAN_MODE_FP64 0     /* FP64, refined hardware rsqrt, fused ops      */
 #define TUPAN_MODE_SPLIT32 1  /* split-FP32 pair arithmetic, FP64 sums (Acc,
-                                 AccJerk, Tstep; Phi and SnapCrackle fall
-                                 back to FP64)                               */
+                                 AccJerk, Tstep, PNAcc; Phi, SnapCrackle and
+                                 NREG fall back to FP64)                     */
 #define TUPAN_MODE_IEEE64 2   /* FP64 in the reference's operation order with
                                  IEEE division and square root               */
 
@@ -180,6 +180,23 @@ void nreg_Vkernel(const unsigned long ni,
                   const double dt,
                   double *_idvx, double *_idvy, double *_idvz, double *_ik);
 
+/* libtupan.h:121-149.  Post-Newtonian pair accelerations: order 2 (or 3)
+ * gives 1PN, 4 adds 2PN, 5 adds 2.5PN, 6 adds 3PN, 7 (or more) adds 3.5PN;
+ * order 0 or 1 gives zeros.  inv1..inv7 are 1/c^k (inv1 and inv3 are not
+ * read, as in the reference). */
+void pnacc_kernel(const unsigned long ni,
+                  const double *_im, const double *_irx, const double *_iry,
+                  const double *_irz, const double *_ie2,
+                  const double *_ivx, const double *_ivy, const double *_ivz,
+                  const unsigned long nj,
+                  const double *_jm, const double *_jrx, const double *_jry,
+                  const double *_jrz, const double *_je2,
+                  const double *_jvx, const double *_jvy, const double *_jvz,
+                  const unsigned long order, const double inv1,
+                  const double inv2, const double inv3, const double inv4,
+                  const double inv5, const double inv6, const double inv7,
+                  double *_ipnax, double *_ipnay, double *_ipnaz);
+
 /* libtupan.h:203-229 */
 void sakura_kernel(const unsigned long ni,
                    const double *_im, const double *_irx, const double *_iry,
@@ -267,6 +284,7 @@ int tupan_tstep(int mode, size_t ni,
 #define TUPAN_K_TSTEP 4
 #define TUPAN_K_NREG_X 5  /* nreg_Xkernel: 8 + 8 fields, dt; 7 outputs */
 #define TUPAN_K_NREG_V 6  /* nreg_Vkernel: 7 + 7 fields, dt; 4 outputs */
+#define TUPAN_K_PNACC 7   /* pnacc_kernel: 8 + 8 fields; through tupan_pnacc */
 int tupan_pairwise(int kernel, int mode, size_t ni,
                    const double *const *ifields, size_t nj,
                    const double *const *jfields, double *const *out,
@@ -276,6 +294,22 @@ int tupan_pairwise_f32(int kernel, int mode, size_t ni,
                        const float *const *ifields, size_t nj,
                        const float *const *jfields, float *const *out,
                        float eta, void *stream);
+
+/* PN pair accelerations (pnacc_kernel.c:5-58, libtupan.h:121-149) on device
+ * arrays.  ifields/jfields: m rx ry rz e2 vx vy vz; out: pnax pnay pnaz;
+ * inv7[k-1] = 1/c^k for k = 1..7 (a host array).  Modes TUPAN_MODE_FP64 and
+ * TUPAN_MODE_IEEE64 (bit-exact with the reference C backend for one j
+ * split) and TUPAN_MODE_SPLIT32 (pair terms in FP32 on hi+lo separations,
+ * FP64 sums; about 1e-6 of the PN acceleration, which is itself (v/c)^2 of
+ * the Newtonian one).  tupan_pairwise with TUPAN_K_PNACC
+ * returns TUPAN_ERR_ARG: the PN parameters only travel through these. */
+int tupan_pnacc(int mode, size_t ni, const double *const *ifields, size_t nj,
+                const double *const *jfields, unsigned long order,
+                const double *inv7, double *const *out, void *stream);
+int tupan_pnacc_f32(int mode, size_t ni, const float *const *ifields,
+                    size_t nj, const float *const *jfields,
+                    unsigned long order, const float *inv7,
+                    float *const *out, void *stream);
 
 /* Sakura pair drift (sakura_kernel.c:5-63, libtupan.h:203-229) on device
  * arrays.  flag: -1, +1, -2, +2 (any other value leaves every pair
@@ -434,6 +468,9 @@ int tupan_ensemble_phi(const tupan_ensemble *plan, const double *m,
 int tupan_host_pairwise_f32(int kernel, size_t ni, const float *const *ifields,
                             size_t nj, const float *const *jfields,
                             float *const *out, float eta);
+int tupan_host_pnacc_f32(size_t ni, const float *const *ifields, size_t nj,
+                         const float *const *jfields, unsigned long order,
+                         const float *inv7, float *const *out);
 int tupan_host_sakura_f32(size_t ni, const float *const *ifields, size_t nj,
                           const float *const *jfields, float dt, int flag,
                           float *const *out);

@@ -41,6 +41,10 @@ SPECS = {
     "nreg_v": (("mass", "vx", "vy", "vz", "ax", "ay", "az"),
                ("mass", "vx", "vy", "vz", "ax", "ay", "az"),
                ("dt",), ("mvx", "mvy", "mvz", "mk")),
+    "pnacc": (("mass", "rx", "ry", "rz", "eps2", "vx", "vy", "vz"),
+              ("mass", "rx", "ry", "rz", "eps2", "vx", "vy", "vz"),
+              ("order", "inv1", "inv2", "inv3", "inv4", "inv5", "inv6",
+               "inv7"), ("pnax", "pnay", "pnaz")),
     "sakura": (("mass", "rx", "ry", "rz", "eps2", "vx", "vy", "vz"),
                ("mass", "rx", "ry", "rz", "eps2", "vx", "vy", "vz"),
                ("dt", "flag"), ("drx", "dry", "drz", "dvx", "dvy", "dvz")),
@@ -99,7 +103,12 @@ class _Lib:
         nj = len(keep[-1])
         sargs = []
         for f, v in zip(sfl, scalars):
-            sargs.append(self.c_int(int(v)) if f == "flag" else self.c_real(v))
+            if f == "flag":
+                sargs.append(self.c_int(int(v)))
+            elif f == "order":
+                sargs.append(self.c_uint(int(v)))
+            else:
+                sargs.append(self.c_real(v))
         outs = [np.zeros(ni, self.dtype) for _ in ofl]
         oargs = [o.ctypes.data_as(ctypes.c_void_p) for o in outs]
         self._fn(name)(self.c_uint(ni), *iargs, self.c_uint(nj), *jargs,
@@ -200,7 +209,7 @@ class Reference(_Lib):
                  "snap_crackle": "snap_crackle_kernel",
                  "tstep": "tstep_kernel", "sakura": "sakura_kernel",
                  "nreg_x": "nreg_Xkernel", "nreg_v": "nreg_Vkernel",
-                 "kepler": "kepler_solver_kernel"}
+                 "pnacc": "pnacc_kernel", "kepler": "kepler_solver_kernel"}
         super().__init__(path, prec, lambda n: names[n])
 
 

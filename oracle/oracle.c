@@ -419,6 +419,309 @@ void SYM(nreg_v)(size_t ni, const real *im, const real *ivx, const real *ivy,
 }
 
 /* ---------------------------------------------------------------------------
+ * Post-Newtonian pair acceleration: pnacc_kernel.c:5-58,
+ * pnacc_kernel_common.h:8-70, pn_terms.h:126-566.
+ *
+ * The acceleration of i due to j is A n R + B v with n = r/R,
+ *   A = (jm/R^3) * (a7 + a6 + a5 + a4 + a2),  B = (jm/R^2) * (b7 + ... + b2),
+ * where a_k, b_k are the 1/c^k terms of the two-body harmonic-gauge
+ * equations of motion (1PN, 2PN, 2.5PN, 3PN, 3.5PN); `order` selects them
+ * cumulatively (2 -> 1PN; 4 -> + 2PN; 5 -> + 2.5PN; 6 -> + 3PN; 7 -> + 3.5PN;
+ * 3 adds nothing to 2, and 0 or 1 give zero).  Every sum is evaluated in the
+ * reference's grouping; the named partial terms (A1..A8 of the 3PN a-part
+ * and so on) are one line each of the reference's expressions.
+ *
+ * The reference's 2.5PN b-term carries -v^2 where the published harmonic-gauge
+ * equations (Blanchet, Living Rev. Relativ. 17 (2014) 2, eq. 203) have
+ * -4/5 v^2; the restatement keeps the reference's value.
+ * ------------------------------------------------------------------------- */
+#define Q(p, q) (((real)(p)) / (q))
+#define PN_PI2 ((real)(9.869604401089358))
+
+struct pn_inv {
+    real c2, c4, c5, c6, c7;
+};
+
+struct pn_pair {
+    real im, jm, im2, jm2, imjm;
+    real ir, ir2;                  /* 1/R, 1/R^2 */
+    real v2, iv2, jv2, iv4, jv4;
+    real ivjv, ivjv2, nv, nv2, niv, njv, niv2, njv2, nivnjv;
+};
+
+/* pn_terms.h:126-151 */
+static void pn_1pn(const struct pn_pair *p, real *a, real *b)
+{
+    real s = -p->iv2 - 2 * p->jv2;
+    s = s + 4 * p->ivjv;
+    s = s + Q(3, 2) * p->njv2;
+    s = s + p->ir * (5 * p->im + 4 * p->jm);
+    *a = s;
+    *b = 4 * p->niv - 3 * p->njv;
+}
+
+/* pn_terms.h:154-219 */
+static void pn_2pn(const struct pn_pair *p, real *a, real *b)
+{
+    real t1 = -2 * (p->jv4 + p->ivjv2);
+    real t2 = 4 * p->jv2 * p->ivjv;
+    real t3 = p->njv2 * (Q(3, 2) * p->iv2 + Q(9, 2) * p->jv2 - 6 * p->ivjv
+                         - Q(15, 8) * p->njv2);
+    real t4 = p->ir2 * (Q(57, 4) * p->im2 + 9 * p->jm2 + Q(69, 2) * p->imjm);
+    real ci = -Q(15, 4) * p->iv2 + Q(5, 4) * p->jv2 - Q(5, 2) * p->ivjv
+              + Q(39, 2) * p->niv2 - 39 * p->nivnjv + Q(17, 2) * p->njv2;
+    real cj = 4 * p->jv2 - 8 * p->ivjv + 2 * p->niv2 - 4 * p->nivnjv
+              - 6 * p->njv2;
+    real t5 = p->ir * (p->im * ci + p->jm * cj);
+    *a = t1 + t2 + t3 - t4 + t5;
+
+    real u1 = p->ir * (p->im * (-Q(63, 4) * p->niv + Q(55, 4) * p->njv)
+                       - p->jm * 2 * (p->niv + p->njv));
+    real u2 = p->iv2 * p->njv;
+    real u3 = p->ivjv * p->nv * 4;
+    real u4 = p->jv2 * (4 * p->niv - 5 * p->njv);
+    real u5 = p->njv2 * (-6 * p->niv + Q(9, 2) * p->njv);
+    *b = u1 + u2 - u3 + u4 + u5;
+}
+
+/* pn_terms.h:222-244, without the final m_i/(R c^5) factor */
+static void pn_25pn(const struct pn_pair *p, real *a, real *b)
+{
+    *a = p->nv * (p->ir * (-Q(24, 5) * p->im + Q(208, 15) * p->jm)
+                  + Q(12, 5) * p->v2);
+    *b = -p->v2 + p->ir * (Q(8, 5) * p->im - Q(32, 5) * p->jm);
+}
+
+/* pn_terms.h:247-389 */
+static void pn_3pn(const struct pn_pair *p, real *a, real *b)
+{
+    const real niv = p->niv, njv = p->njv, iv2 = p->iv2, jv2 = p->jv2;
+    const real ivjv = p->ivjv, njv2 = p->njv2, niv2 = p->niv2;
+    const real C41 = Q(41, 16) * PN_PI2;
+
+    real a1 = njv2 * (3 * p->ivjv2 + Q(3, 2) * iv2 * jv2 - 12 * ivjv * jv2
+                      + Q(15, 2) * p->jv4
+                      + njv2 * (Q(15, 2) * (ivjv - jv2 - Q(1, 4) * iv2)
+                                + Q(35, 16) * njv2));
+    real a2 = 2 * jv2 * (-p->ivjv2 + jv2 * (2 * ivjv - jv2));
+    real a3i = niv * (njv * (244 * ivjv - Q(205, 2) * iv2 - Q(283, 2) * jv2
+                             + Q(383, 2) * njv2)
+                      + niv * (Q(229, 4) * (iv2 + jv2 - 2 * ivjv)
+                               - Q(723, 4) * njv2
+                               + niv * (Q(171, 2) * (njv - Q(1, 4) * niv))));
+    real a3 = p->im * p->ir * (a3i
+                               + njv2 * (Q(191, 4) * iv2 + Q(259, 4) * jv2
+                                         - Q(225, 2) * ivjv - Q(455, 8) * njv2)
+                               + ivjv * (Q(91, 2) * iv2 + 43 * jv2
+                                         - Q(177, 4) * ivjv)
+                               - Q(91, 8) * iv2 * (iv2 + 2 * jv2)
+                               - Q(81, 8) * p->jv4);
+    real a4 = p->jm * p->ir * 4
+              * (p->jv4
+                 + njv * (niv * (ivjv - jv2)
+                          + njv * (3 * (ivjv - jv2) - Q(3, 2) * niv2
+                                   + njv * (3 * niv + Q(3, 2) * njv)))
+                 + ivjv * (ivjv - 2 * jv2));
+    real a5 = p->jm2 * p->ir2 * (-niv2 + 2 * p->nivnjv + Q(43, 2) * njv2
+                                 + 18 * ivjv - 9 * jv2);
+    real a6 = p->imjm * p->ir2 * (Q(415, 8) * niv2 - Q(375, 4) * p->nivnjv
+                                  + Q(1113, 8) * njv2 + 18 * iv2
+                                  + PN_PI2 * (Q(123, 64) * p->v2
+                                              - Q(615, 64) * p->nv2)
+                                  + 33 * (ivjv - Q(1, 2) * jv2));
+    real a7 = p->im2 * p->ir2 * (-Q(2069, 8) * niv2 + 543 * p->nivnjv
+                                 - Q(939, 4) * njv2 + Q(471, 8) * iv2
+                                 + Q(357, 8) * (jv2 - 2 * ivjv));
+    real a8 = p->ir * p->ir2 * (16 * p->jm * p->jm2
+                                + p->im2 * p->jm * (Q(547, 3) - C41)
+                                - Q(13, 12) * p->im * p->im2
+                                + p->im * p->jm2 * (Q(545, 3) - C41));
+    *a = a1 + a2 + a3 + a4 + a5 + a6 + a7 + a8;
+
+    real b1 = njv * (jv2 * (iv2 + 8 * ivjv - 7 * jv2) - 2 * p->ivjv2
+                     + njv * (6 * niv * (ivjv - 2 * jv2)
+                              + njv * (6 * (2 * jv2 - ivjv - Q(1, 4) * iv2)
+                                       + njv * (Q(15, 2) * (niv - Q(3, 4) * njv)))));
+    real b2 = 4 * niv * (p->jv4 - ivjv * jv2);
+    real b3 = p->jm * p->ir * (njv * (4 * (ivjv - jv2 - Q(1, 2) * niv2)
+                                      + njv * (2 * (4 * niv + njv)))
+                               + 2 * niv * (ivjv - jv2));
+    real b4 = p->im * p->ir * (niv * (Q(207, 8) * iv2 + Q(81, 8) * jv2 - 36 * ivjv
+                                      - Q(269, 4) * njv2
+                                      + niv * (Q(565, 4) * njv - Q(243, 4) * niv))
+                               + njv * (Q(83, 8) * jv2 + Q(27, 4) * ivjv
+                                        - Q(137, 8) * iv2 - Q(95, 12) * njv2));
+    real b5 = p->ir2 * (p->jm2 * (4 * niv + 5 * njv)
+                        + p->im2 * (Q(311, 4) * niv - Q(357, 4) * njv)
+                        + p->imjm * (Q(479, 8) * njv - Q(307, 8) * niv
+                                     + Q(123, 32) * PN_PI2 * p->nv));
+    *b = b1 + b2 + b3 + b4 + b5;
+}
+
+/* pn_terms.h:392-489, without the final m_i/(R c^7) factor */
+static void pn_35pn(const struct pn_pair *p, real *a, real *b)
+{
+    const real niv = p->niv, njv = p->njv, iv2 = p->iv2, jv2 = p->jv2;
+    const real ivjv = p->ivjv, njv2 = p->njv2, niv2 = p->niv2;
+    const real nivnjv = p->nivnjv, nv = p->nv, nv2 = p->nv2, v2 = p->v2;
+
+    real c1 = p->im2 * p->ir2 * (Q(3992, 105) * niv - Q(4328, 105) * njv);
+    real c2 = p->imjm * p->ir * p->ir2 * (-Q(13576, 105) * niv + Q(2872, 21) * njv);
+    real c3 = p->jm2 * p->ir * p->ir2 * (-Q(3172, 21) * nv);
+    real c4 = p->im * p->ir
+              * (niv * (48 * niv2 - Q(4888, 105) * iv2 + Q(2056, 21) * ivjv
+                        - Q(1028, 21) * jv2)
+                 + nivnjv * (-Q(696, 5) * niv + Q(744, 5) * njv)
+                 + njv * (-Q(288, 5) * njv2 + Q(5056, 105) * iv2
+                          - Q(2224, 21) * ivjv + Q(5812, 105) * jv2));
+    real c5 = p->jm * p->ir
+              * (niv * (-Q(582, 5) * niv2 - Q(2864, 35) * ivjv + Q(1432, 35) * jv2)
+                 + nivnjv * (Q(1746, 5) * niv - Q(1954, 5) * njv)
+                 + Q(3568, 105) * nv * iv2
+                 + njv * (158 * njv2 - Q(5752, 105) * jv2 + Q(10048, 105) * ivjv));
+    real c6 = nv * (-56 * nv2 * nv2 - Q(246, 35) * p->iv4)
+              + niv * (v2 * (60 * niv2 - 180 * nivnjv + 174 * njv2)
+                       + ivjv * (Q(1068, 35) * (iv2 - ivjv) + Q(984, 35) * jv2)
+                       - Q(534, 35) * iv2 * jv2
+                       - Q(204, 35) * p->jv4)
+              + njv * (-54 * njv2 * v2
+                       + ivjv * (-Q(984, 35) * iv2 + Q(180, 7) * ivjv
+                                 - Q(732, 35) * jv2)
+                       + Q(90, 7) * iv2 * jv2
+                       + Q(24, 7) * p->jv4);
+    *a = c1 + c2 + c3 + c4 + c5 + c6;
+
+    real d1 = -p->im2 * p->ir2 * Q(184, 21);
+    real d2 = p->imjm * p->ir2 * Q(6224, 105);
+    real d3 = p->jm2 * p->ir2 * Q(6388, 105);
+    real d4 = p->im * p->ir * (Q(52, 15) * niv2 - Q(56, 15) * nivnjv
+                               - Q(44, 15) * njv2 - Q(132, 35) * iv2
+                               + Q(152, 35) * ivjv - Q(48, 35) * jv2);
+    real d5 = p->jm * p->ir * (Q(454, 15) * niv2 - Q(372, 5) * nivnjv
+                               + Q(854, 15) * njv2 - Q(152, 21) * iv2
+                               + Q(2864, 105) * ivjv - Q(1768, 105) * jv2);
+    real d6 = 60 * nv2 * nv2
+              + v2 * (-Q(348, 5) * niv2 + Q(684, 5) * nivnjv - 66 * njv2)
+              + Q(334, 35) * p->iv4
+              + ivjv * (-Q(1336, 35) * iv2 + Q(1308, 35) * ivjv - Q(1252, 35) * jv2)
+              + Q(654, 35) * iv2 * jv2
+              + Q(292, 35) * p->jv4;
+    *b = d1 + d2 + d3 + d4 + d5 + d6;
+}
+
+/* pn_terms.h:492-566 and pnacc_kernel_common.h:29-69: the pair's (A, B). */
+static void pn_pair_terms(size_t order, const struct pn_inv *c, real im, real jm,
+                          real rx, real ry, real rz, real vx, real vy, real vz,
+                          real v2, real ivx, real ivy, real ivz, real jvx,
+                          real jvy, real jvz, real ir, real ir2, real ir3,
+                          real *A, real *B)
+{
+    struct pn_pair p;
+    real a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    real b[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const real nx = rx * ir, ny = ry * ir, nz = rz * ir;
+    p.im = im;
+    p.jm = jm;
+    p.ir = ir;
+    p.ir2 = ir2;
+    p.v2 = v2;
+    p.iv2 = ivx * ivx + ivy * ivy + ivz * ivz;
+    p.jv2 = jvx * jvx + jvy * jvy + jvz * jvz;
+    if (order > 1) {
+        p.niv = nx * ivx + ny * ivy + nz * ivz;
+        p.njv = nx * jvx + ny * jvy + nz * jvz;
+        p.njv2 = p.njv * p.njv;
+        p.ivjv = ivx * jvx + ivy * jvy + ivz * jvz;
+        pn_1pn(&p, &a[2], &b[2]);
+        a[2] *= c->c2;
+        b[2] *= c->c2;
+    }
+    if (order > 3) {
+        p.im2 = im * im;
+        p.jm2 = jm * jm;
+        p.imjm = im * jm;
+        p.jv4 = p.jv2 * p.jv2;
+        p.ivjv2 = p.ivjv * p.ivjv;
+        p.nv = nx * vx + ny * vy + nz * vz;
+        p.niv2 = p.niv * p.niv;
+        p.nivnjv = p.niv * p.njv;
+        pn_2pn(&p, &a[4], &b[4]);
+        a[4] *= c->c4;
+        b[4] *= c->c4;
+    }
+    if (order > 4) {
+        real f = (im * ir) * c->c5;
+        pn_25pn(&p, &a[5], &b[5]);
+        a[5] *= f;
+        b[5] *= f;
+    }
+    if (order > 5) {
+        p.nv2 = p.nv * p.nv;
+        pn_3pn(&p, &a[6], &b[6]);
+        a[6] *= c->c6;
+        b[6] *= c->c6;
+    }
+    if (order > 6) {
+        real f = (im * ir) * c->c7;
+        p.iv4 = p.iv2 * p.iv2;
+        pn_35pn(&p, &a[7], &b[7]);
+        a[7] *= f;
+        b[7] *= f;
+    }
+    real sa = a[7], sb = b[7];
+    for (int k = 6; k >= 1; --k) {
+        sa = sa + a[k];
+        sb = sb + b[k];
+    }
+    *A = (0 + sa) * (jm * ir3);
+    *B = (0 + sb) * (jm * ir2);
+}
+
+void SYM(pnacc)(size_t ni, const real *im, const real *irx, const real *iry,
+                const real *irz, const real *ie2, const real *ivx,
+                const real *ivy, const real *ivz,
+                size_t nj, const real *jm, const real *jrx, const real *jry,
+                const real *jrz, const real *je2, const real *jvx,
+                const real *jvy, const real *jvz,
+                size_t order, real inv1, real inv2, real inv3, real inv4,
+                real inv5, real inv6, real inv7,
+                real *ipnax, real *ipnay, real *ipnaz)
+{
+    const struct pn_inv c = {inv2, inv4, inv5, inv6, inv7};
+    (void)inv1;
+    (void)inv3;
+    for (size_t i = 0; i < ni; ++i) {
+        real sx = 0, sy = 0, sz = 0;
+        for (size_t j = 0; j < nj; ++j) {
+            real rx = irx[i] - jrx[j];
+            real ry = iry[i] - jry[j];
+            real rz = irz[i] - jrz[j];
+            real e2 = ie2[i] + je2[j];
+            real vx = ivx[i] - jvx[j];
+            real vy = ivy[i] - jvy[j];
+            real vz = ivz[i] - jvz[j];
+            real r2 = rx * rx + ry * ry + rz * rz;
+            real v2 = vx * vx + vy * vy + vz * vz;
+            real ir2 = r2 > 0 ? 1 / (r2 + e2) : 0;
+            real ir = R_SQRT(ir2);
+            real ir3 = ir2 * ir;
+            real A, B;
+            pn_pair_terms(order, &c, im[i], jm[j], rx, ry, rz, vx, vy, vz, v2,
+                          ivx[i], ivy[i], ivz[i], jvx[j], jvy[j], jvz[j],
+                          ir, ir2, ir3, &A, &B);
+            sx += A * rx + B * vx;
+            sy += A * ry + B * vy;
+            sz += A * rz + B * vz;
+        }
+        ipnax[i] = sx;
+        ipnay[i] = sy;
+        ipnaz[i] = sz;
+    }
+}
+#undef Q
+#undef PN_PI2
+
+/* ---------------------------------------------------------------------------
  * Universal-variable Kepler solver: universal_kepler_solver.h:17-614.
  * ------------------------------------------------------------------------- */
 static inline real stumpff_c0(real z)

@@ -2,7 +2,7 @@
 
 Runs only where the reference checkout is present: `make -C oracle ref`
 compiles tupan/lib/src/{phi,acc,acc_jerk,snap_crackle,tstep,sakura,
-kepler_solver}_kernel.c and nreg_kernels.c (FP64 and FP32 builds, the
+kepler_solver,pnacc}_kernel.c and nreg_kernels.c (FP64 and FP32 builds, the
 reference's cffi flags)
 into oracle/_ref, and this script calls those libraries through ctypes on
 seeded inputs.  The fixtures are inputs and outputs only.
@@ -16,8 +16,11 @@ Fixtures (numpy .npz, float64 unless noted):
       U[-1,1) for SnapCrackle, eps2 either 0 or U[0,0.01); B shares one
       particle position with A (a coincident pair) and A contains a
       duplicated particle; outputs of Phi, Acc, AccJerk, SnapCrackle,
-      Tstep(eta=1/64), NREG X and NREG V (dt=1/64) for (A,B), (B,A) and
-      (A,A), and for A against every prefix B[:k], k in PREFIXES.
+      Tstep(eta=1/64), NREG X and NREG V (dt=1/64) and PNAcc (order 7,
+      c = 128, the reference's own test values) for (A,B), (B,A) and (A,A),
+      and for A against every prefix B[:k], k in PREFIXES; PNAcc of every
+      order 0..8 for (A,B) and (A,A) at c = 128 (keys pnacc_o<order>) and of
+      order 7 at c = 16 (keys pnacc_c16).
   sakura_f64.npz
       Sakura dr, dv for flags -2, -1, 0, 1, 2 and dt in {1/64, 0.5} on a
       random 24-particle set (eps2 = 0 and eps2 > 0) and on a bound 8-body
@@ -39,6 +42,14 @@ from oracle import Reference, SPECS  # noqa: E402
 
 PREFIXES = (1, 2, 3, 7, 16, 33, 61)
 SCALAR_KERNELS = ("tstep", "nreg_x", "nreg_v")   # eta or dt = 1/64
+PN_ORDER, PN_CLIGHT = 7, 128.0   # tupan/tests/test_extensions.py:110-111
+
+
+def pn_scalars(order, c):
+    """(order, 1/c, ..., 1/c^7) as the reference's Clight forms them
+    (extensions.py:51-60)."""
+    inv1 = 1.0 / c
+    return (order,) + tuple(inv1**k for k in range(1, 8))
 FIELDS = ("mass", "rx", "ry", "rz", "eps2", "vx", "vy", "vz",
           "ax", "ay", "az", "jx", "jy", "jz")
 
@@ -75,7 +86,10 @@ def pairwise_fixture(prec):
         for name in SPECS:
             if name == "sakura":
                 continue
-            sc = (1.0 / 64,) if name in SCALAR_KERNELS else ()
+            if name == "pnacc":
+                sc = pn_scalars(PN_ORDER, PN_CLIGHT)
+            else:
+                sc = (1.0 / 64,) if name in SCALAR_KERNELS else ()
             for lab, ii, jj in (("AB", a, b), ("BA", b, a), ("AA", a, a)):
                 res = ref.pairwise(name, ii, jj, *sc)
                 for o, r in zip(SPECS[name][3], res):
@@ -85,6 +99,15 @@ def pairwise_fixture(prec):
                 res = ref.pairwise(name, a, bk, *sc)
                 for o, r in zip(SPECS[name][3], res):
                     out["%s_%s_pre%d_%s" % (tag, name, k, o)] = r
+        for lab, ii, jj in (("AB", a, b), ("AA", a, a)):
+            for order in range(9):
+                res = ref.pairwise("pnacc", ii, jj,
+                                   *pn_scalars(order, PN_CLIGHT))
+                for o, r in zip(SPECS["pnacc"][3], res):
+                    out["%s_pnacc_o%d_%s_%s" % (tag, order, lab, o)] = r
+            res = ref.pairwise("pnacc", ii, jj, *pn_scalars(7, 16.0))
+            for o, r in zip(SPECS["pnacc"][3], res):
+                out["%s_pnacc_c16_%s_%s" % (tag, lab, o)] = r
     return out
 
 

@@ -16,7 +16,7 @@ _NAMES = {"phi_kernel": "phi", "acc_kernel": "acc",
           "acc_jerk_kernel": "acc_jerk", "snap_crackle_kernel": "snap_crackle",
           "tstep_kernel": "tstep", "sakura_kernel": "sakura",
           "kepler_solver_kernel": "kepler", "nreg_Xkernel": "nreg_x",
-          "nreg_Vkernel": "nreg_v"}
+          "nreg_Vkernel": "nreg_v", "pnacc_kernel": "pnacc"}
 _FIELDS = ("mass", "rx", "ry", "rz", "eps2", "vx", "vy", "vz")
 
 

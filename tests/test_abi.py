@@ -105,5 +105,4 @@ def test_unknown_backend_and_precision():
         extensions.Phi("C", "float64")
     with pytest.raises(KeyError):
         extensions.Phi("HIP", "float16")
-    with pytest.raises(NotImplementedError):
-        extensions.pnacc.calc(None, None)
+    assert isinstance(extensions.pnacc, extensions.PNAcc)

@@ -13,6 +13,8 @@ scalar outputs relative):
                         first-order FP32 error model and 1e-6 relative (the
                         Plummer error budget is in test_gpu_precision.py)
   FP32 (float build):   <= 2e-5 against the reference's float build
+PNAcc runs at the reference's own test values (order 7, c = 128) in the
+generic tests, and at every order 0..8 and at c = 16 in test_pnacc_orders.
 """
 import ctypes
 
@@ -38,10 +40,26 @@ KERNELS = {
                                         "u"), [(0, 1, 2), (3, 4, 5), (6,)]),
     "nreg_v": ("NREG_V", (1.0 / 64,), ("mvx", "mvy", "mvz", "mk"),
                [(0, 1, 2), (3,)]),
+    "pnacc": ("PNAcc", (), ("pnax", "pnay", "pnaz"), [(0, 1, 2)]),
 }
 FIELDS = ("mass", "rx", "ry", "rz", "eps2", "vx", "vy", "vz",
           "ax", "ay", "az", "jx", "jy", "jz")
 PREFIXES = (1, 2, 3, 7, 16, 33, 61)  # as written by golden/make_golden.py
+
+
+def set_clight(order, c):
+    from tupan_amd.lib import extensions
+    extensions.clight.pn_order = order
+    extensions.clight.clight = c
+
+
+@pytest.fixture(autouse=True)
+def pn_parameters():
+    """PNAcc reads the module's clight: the reference's test values
+    (tupan/tests/test_extensions.py:110-111)."""
+    set_clight(7, 128.0)
+    yield
+    set_clight(0, 128.0)
 
 
 def device_ps(d, dtype=torch.float64):
@@ -240,6 +258,68 @@ def test_fp32_golden(golden, name):
             assert_close(name, got, g, "%s_%s_%s" % (tag, name, lab), 2e-5)
 
 
+@pytest.mark.parametrize("soft", [0, 1])
+def test_pnacc_orders(golden, soft):
+    """Every PN order 0..8 at c = 128 and order 7 at c = 16: FP64 within
+    1e-13, ieee64 bit-exact, the float build within 2e-5."""
+    g = golden("pairwise_f64.npz")
+    g32 = golden("pairwise_f32.npz")
+    tag = "s%d" % soft
+    A = device_ps(split_set(g, tag, "A"))
+    B = device_ps(split_set(g, tag, "B"))
+    A32 = device_ps(split_set(g32, tag, "A"), torch.float32)
+    B32 = device_ps(split_set(g32, tag, "B"), torch.float32)
+    cases = [("o%d" % o, o, 128.0) for o in range(9)] + [("c16", 7, 16.0)]
+    for key, order, c in cases:
+        set_clight(order, c)
+        for lab, ii, jj, i32, j32 in (("AB", A, B, A32, B32),
+                                      ("AA", A, A, A32, A32)):
+            full = "%s_pnacc_%s_%s" % (tag, key, lab)
+            ref = [g["%s_%s" % (full, o)] for o in ("pnax", "pnay", "pnaz")]
+            got = run_ext("pnacc", "float64", "ieee64", ii, jj, ())
+            for r, e in zip(got, ref):
+                assert np.array_equal(r, e), (key, lab)
+            got = run_ext("pnacc", "float64", "fp64", ii, jj, ())
+            if order < 2:
+                assert all(not np.any(r) for r in got), (key, lab)
+                continue
+            assert_close("pnacc", got, g, full, 1e-13)
+            got = run_ext("pnacc", "float32", "fp64", i32, j32, ())
+            assert_close("pnacc", got, g32, full, 2e-5)
+
+
+def test_split32_pnacc(golden):
+    """Split-FP32 PNAcc against the reference at orders 2, 4, 5, 6, 7
+    (c = 128) and 7 (c = 16): FP32 pair terms on FP64 inputs, within 2e-6
+    of the PN acceleration per particle (its size relative to the Newtonian
+    acceleration is printed; on these sets v/c is 0.04 to 0.08)."""
+    g = golden("pairwise_f64.npz")
+    errs = {}
+    for soft in (0, 1):
+        tag = "s%d" % soft
+        A = device_ps(split_set(g, tag, "A"))
+        B = device_ps(split_set(g, tag, "B"))
+        cases = [("o%d" % o, o, 128.0) for o in (2, 4, 5, 6, 7)]
+        cases.append(("c16", 7, 16.0))
+        for key, order, c in cases:
+            set_clight(order, c)
+            for lab, ii, jj in (("AB", A, B), ("AA", A, A)):
+                full = "%s_pnacc_%s_%s" % (tag, key, lab)
+                ref = [g["%s_%s" % (full, o)] for o in ("pnax", "pnay",
+                                                        "pnaz")]
+                got = run_ext("pnacc", "float64", "split32", ii, jj, ())
+                an = [g["%s_acc_%s_%s" % (tag, lab, o)]
+                      for o in ("ax", "ay", "az")]
+                d = np.sqrt(sum((x - y)**2 for x, y in zip(got, ref)))
+                en = float(np.max(d / np.sqrt(sum(x * x for x in an))))
+                errs[(tag, key, lab)] = (vec_rel_err(got, ref), en)
+    for k, (e, en) in sorted(errs.items()):
+        print("split32 pnacc %s: %.3e of PN acc, %.3e of Newtonian" % (
+            "/".join(k), e, en))
+    worst = max(errs.items(), key=lambda kv: kv[1][0])
+    assert worst[1][0] <= 2e-6, worst
+
+
 def test_host_pointer_abi(golden):
     """The reference-named host-pointer entry points (libtupan.h ABI) on
     numpy arrays, called through ctypes as cffi would call the reference."""
@@ -265,6 +345,13 @@ def test_host_pointer_abi(golden):
                      ctypes.c_ulong(nj), *[p(B[k]) for k in FIELDS[:8]],
                      ctypes.c_double(1.0 / 64), *[p(o) for o in out2])
     assert_close("tstep", out2, g, "s1_tstep_AB", 1e-13)
+    lib.pnacc_kernel.restype = None
+    inv = [ctypes.c_double(128.0**-k) for k in range(1, 8)]
+    out3 = [np.zeros(ni) for _ in range(3)]
+    lib.pnacc_kernel(ctypes.c_ulong(ni), *[p(A[k]) for k in FIELDS[:8]],
+                     ctypes.c_ulong(nj), *[p(B[k]) for k in FIELDS[:8]],
+                     ctypes.c_ulong(7), *inv, *[p(o) for o in out3])
+    assert_close("pnacc", out3, g, "s1_pnacc_AB", 1e-13)
 
 
 def test_host_pointer_abi_f32(golden):
@@ -284,6 +371,13 @@ def test_host_pointer_abi_f32(golden):
                    ctypes.c_uint(nj), *[p(B[k]) for k in FIELDS[:5]],
                    *[p(o) for o in out])
     assert_close("acc", out, g, "s0_acc_AB", 2e-5)
+    lib.pnacc_kernel.restype = None
+    inv = [ctypes.c_float(128.0**-k) for k in range(1, 8)]
+    out3 = [np.zeros(ni, np.float32) for _ in range(3)]
+    lib.pnacc_kernel(ctypes.c_uint(ni), *[p(A[k]) for k in FIELDS[:8]],
+                     ctypes.c_uint(nj), *[p(B[k]) for k in FIELDS[:8]],
+                     ctypes.c_uint(7), *inv, *[p(o) for o in out3])
+    assert_close("pnacc", out3, g, "s0_pnacc_AB", 2e-5)
 
 
 def test_numpy_staging_through_extension(golden):

@@ -18,7 +18,8 @@ MODE_SPLIT32 = 1
 MODE_IEEE64 = 2
 MODES = {"fp64": MODE_FP64, "split32": MODE_SPLIT32, "ieee64": MODE_IEEE64}
 
-K_PHI, K_ACC, K_ACC_JERK, K_SNAP_CRACKLE, K_TSTEP, K_NREG_X, K_NREG_V = range(7)
+(K_PHI, K_ACC, K_ACC_JERK, K_SNAP_CRACKLE, K_TSTEP, K_NREG_X, K_NREG_V,
+ K_PNACC) = range(8)
 
 _lib = None
 
@@ -62,6 +63,13 @@ def _declare(lib):
     lib.tupan_pairwise_f32.restype = ctypes.c_int
     lib.tupan_pairwise_f32.argtypes = [ctypes.c_int, ctypes.c_int, _S, _PP, _S,
                                        _PP, _PP, ctypes.c_float, _P]
+    lib.tupan_pnacc.restype = ctypes.c_int
+    lib.tupan_pnacc.argtypes = [ctypes.c_int, _S, _PP, _S, _PP, ctypes.c_ulong,
+                                ctypes.POINTER(ctypes.c_double), _PP, _P]
+    lib.tupan_pnacc_f32.restype = ctypes.c_int
+    lib.tupan_pnacc_f32.argtypes = [ctypes.c_int, _S, _PP, _S, _PP,
+                                    ctypes.c_ulong,
+                                    ctypes.POINTER(ctypes.c_float), _PP, _P]
     lib.tupan_sakura.restype = ctypes.c_int
     lib.tupan_sakura.argtypes = ([_S] + [_P] * 8 + [_S] + [_P] * 8
                                  + [ctypes.c_double, ctypes.c_long]
@@ -163,7 +171,8 @@ def device_info():
 # Names exported by the shared libraries (used by tests that check the ABI).
 REFERENCE_ABI = ("phi_kernel", "acc_kernel", "acc_jerk_kernel",
                  "snap_crackle_kernel", "tstep_kernel", "sakura_kernel",
-                 "kepler_solver_kernel", "nreg_Xkernel", "nreg_Vkernel")
+                 "kepler_solver_kernel", "nreg_Xkernel", "nreg_Vkernel",
+                 "pnacc_kernel")
 DEVICE_ABI = ("tupan_last_error", "tupan_version", "tupan_device_info",
               "tupan_tune_set", "tupan_tune_get", "tupan_tune_shape",
               "tupan_timing", "tupan_timing_read",
@@ -175,4 +184,5 @@ DEVICE_ABI = ("tupan_last_error", "tupan_version", "tupan_device_info",
               "tupan_ensemble_info", "tupan_ensemble_sakura",
               "tupan_ensemble_phi", "tupan_sakura_ex", "tupan_mlp_sizes",
               "tupan_mlp_grad", "tupan_mlp_forward", "tupan_mlp_adam",
-              "tupan_mlp_pack", "tupan_mlp_infer", "tupan_sia_subtree")
+              "tupan_mlp_pack", "tupan_mlp_infer", "tupan_sia_subtree",
+              "tupan_pnacc", "tupan_pnacc_f32", "tupan_host_pnacc_f32")

@@ -151,7 +151,8 @@ static void fail_outputs(const char* fn, T* const* out, const size_t* lens, int 
 
 template <class T>
 static void host_pairwise(const char* fn, int kernel, size_t ni, const T* const* ih, size_t nj,
-                          const T* const* jh, T* const* oh, T eta)
+                          const T* const* jh, T* const* oh, T eta,
+                          const PnParams* pn = nullptr)
 {
     const int nin = kernel_in_fields(kernel);
     const int nout = kernel_out_fields(kernel);
@@ -184,7 +185,8 @@ static void host_pairwise(const char* fn, int kernel, size_t ni, const T* const*
         op[k] = cur;
         cur += ni;
     }
-    if (rc == TUPAN_OK) rc = pairwise_call<T>(kernel, TUPAN_MODE_FP64, ni, ip, nj, jp, op, eta, nullptr);
+    if (rc == TUPAN_OK)
+        rc = pairwise_call<T>(kernel, TUPAN_MODE_FP64, ni, ip, nj, jp, op, eta, nullptr, pn);
     for (int k = 0; k < nout && rc == TUPAN_OK; ++k)
         rc = check_hip(hipMemcpy(oh[k], op[k], sizeof(T) * ni, hipMemcpyDeviceToHost), "copy out");
     if (rc != TUPAN_OK) fail_outputs(fn, oh, lens, nout);
@@ -401,6 +403,23 @@ void nreg_Vkernel(const unsigned long ni, const double* _im, const double* _ivx,
     host_pairwise<double>("nreg_Vkernel", K_NREG_V, ni, ih, nj, jh, oh, dt);
 }
 
+void pnacc_kernel(const unsigned long ni, const double* _im, const double* _irx,
+                  const double* _iry, const double* _irz, const double* _ie2,
+                  const double* _ivx, const double* _ivy, const double* _ivz,
+                  const unsigned long nj, const double* _jm, const double* _jrx,
+                  const double* _jry, const double* _jrz, const double* _je2,
+                  const double* _jvx, const double* _jvy, const double* _jvz,
+                  const unsigned long order, const double inv1, const double inv2,
+                  const double inv3, const double inv4, const double inv5, const double inv6,
+                  const double inv7, double* _ipnax, double* _ipnay, double* _ipnaz)
+{
+    const double* ih[8] = {_im, _irx, _iry, _irz, _ie2, _ivx, _ivy, _ivz};
+    const double* jh[8] = {_jm, _jrx, _jry, _jrz, _je2, _jvx, _jvy, _jvz};
+    double* oh[3] = {_ipnax, _ipnay, _ipnaz};
+    const PnParams pn = {order, {inv1, inv2, inv3, inv4, inv5, inv6, inv7}};
+    host_pairwise<double>("pnacc_kernel", K_PNACC, ni, ih, nj, jh, oh, 0.0, &pn);
+}
+
 void sakura_kernel(const unsigned long ni, const double* _im, const double* _irx,
                    const double* _iry, const double* _irz, const double* _ie2,
                    const double* _ivx, const double* _ivy, const double* _ivz,
@@ -470,8 +489,8 @@ int tupan_host_pairwise_f32(int kernel, size_t ni, const float* const* ih, size_
     static const char* names[kNumKernels] = {"phi_kernel",          "acc_kernel",
                                              "acc_jerk_kernel",     "snap_crackle_kernel",
                                              "tstep_kernel",        "nreg_Xkernel",
-                                             "nreg_Vkernel"};
-    if (kernel < 0 || kernel >= kNumKernels)
+                                             "nreg_Vkernel",        "pnacc_kernel"};
+    if (kernel < 0 || kernel >= kNumKernels || kernel == K_PNACC)
         return set_error(TUPAN_ERR_ARG, "unknown pairwise kernel");
     g_error.clear();
     host_pairwise<float>(names[kernel], kernel, ni, ih, nj, jh, oh, eta);
@@ -480,6 +499,17 @@ int tupan_host_pairwise_f32(int kernel, size_t ni, const float* const* ih, size_
 
 // Sakura and Kepler of the float build run in FP64 (the device solver is
 // FP64 only) and round the results to float.
+int tupan_host_pnacc_f32(size_t ni, const float* const* ih, size_t nj, const float* const* jh,
+                         unsigned long order, const float* inv7, float* const* oh)
+{
+    if (!inv7) return set_error(TUPAN_ERR_ARG, "null 1/c^k array");
+    PnParams pn = {order, {}};
+    for (int k = 0; k < 7; ++k) pn.inv[k] = inv7[k];
+    g_error.clear();
+    host_pairwise<float>("pnacc_kernel", K_PNACC, ni, ih, nj, jh, oh, 0.0f, &pn);
+    return g_error.empty() ? TUPAN_OK : TUPAN_ERR_HIP;
+}
+
 int tupan_host_sakura_f32(size_t ni, const float* const* ih, size_t nj, const float* const* jh,
                           float dt, int flag, float* const* oh)
 {
@@ -542,6 +572,26 @@ int tupan_pairwise_f32(int kernel, int mode, size_t ni, const float* const* ifie
                        const float* const* jfields, float* const* out, float eta, void* stream)
 {
     return pairwise_call<float>(kernel, mode, ni, ifields, nj, jfields, out, eta, stream);
+}
+
+int tupan_pnacc(int mode, size_t ni, const double* const* ifields, size_t nj,
+                const double* const* jfields, unsigned long order, const double* inv7,
+                double* const* out, void* stream)
+{
+    if (!inv7) return set_error(TUPAN_ERR_ARG, "null 1/c^k array");
+    PnParams pn = {order, {}};
+    for (int k = 0; k < 7; ++k) pn.inv[k] = inv7[k];
+    return pairwise_call<double>(K_PNACC, mode, ni, ifields, nj, jfields, out, 0.0, stream, &pn);
+}
+
+int tupan_pnacc_f32(int mode, size_t ni, const float* const* ifields, size_t nj,
+                    const float* const* jfields, unsigned long order, const float* inv7,
+                    float* const* out, void* stream)
+{
+    if (!inv7) return set_error(TUPAN_ERR_ARG, "null 1/c^k array");
+    PnParams pn = {order, {}};
+    for (int k = 0; k < 7; ++k) pn.inv[k] = inv7[k];
+    return pairwise_call<float>(K_PNACC, mode, ni, ifields, nj, jfields, out, 0.0f, stream, &pn);
 }
 
 int tupan_phi(int mode, size_t ni, const double* im, const double* irx, const double* iry,

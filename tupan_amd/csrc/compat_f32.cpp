@@ -104,6 +104,21 @@ void nreg_Vkernel(const unsigned int ni, const float* _im, const float* _ivx, co
     tupan_host_pairwise_f32(TUPAN_K_NREG_V, ni, ih, nj, jh, oh, dt);
 }
 
+void pnacc_kernel(const unsigned int ni, const float* _im, const float* _irx, const float* _iry,
+                  const float* _irz, const float* _ie2, const float* _ivx, const float* _ivy,
+                  const float* _ivz, const unsigned int nj, const float* _jm, const float* _jrx,
+                  const float* _jry, const float* _jrz, const float* _je2, const float* _jvx,
+                  const float* _jvy, const float* _jvz, const unsigned int order, const float inv1,
+                  const float inv2, const float inv3, const float inv4, const float inv5,
+                  const float inv6, const float inv7, float* _ipnax, float* _ipnay, float* _ipnaz)
+{
+    const float* ih[8] = {_im, _irx, _iry, _irz, _ie2, _ivx, _ivy, _ivz};
+    const float* jh[8] = {_jm, _jrx, _jry, _jrz, _je2, _jvx, _jvy, _jvz};
+    float* oh[3] = {_ipnax, _ipnay, _ipnaz};
+    const float inv[7] = {inv1, inv2, inv3, inv4, inv5, inv6, inv7};
+    tupan_host_pnacc_f32(ni, ih, nj, jh, order, inv, oh);
+}
+
 void sakura_kernel(const unsigned int ni, const float* _im, const float* _irx,
                    const float* _iry, const float* _irz, const float* _ie2, const float* _ivx,
                    const float* _ivy, const float* _ivz, const unsigned int nj, const float* _jm,

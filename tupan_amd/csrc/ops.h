@@ -1,6 +1,7 @@
 // Pair interactions of the all-pairs kernels (Phi, Acc, AccJerk,
-// SnapCrackle, Tstep and their split-FP32 forms), as policies for the tile
-// engine (engine.h).  Shared by pairwise.hip and sia.hip, which must
+// SnapCrackle, Tstep, the NREG sums and the split-FP32 forms), as policies
+// for the tile engine (engine.h); the post-Newtonian pair acceleration is in
+// pn.h.  Shared by pairwise.hip and sia.hip, which must
 // evaluate them with the same operation order.
 //
 // Per-pair mathematics follows the reference kernel cores
@@ -131,6 +132,8 @@ struct PairArgs {
     int nsplit, jchunk;
     double* partial;
     T eta;
+    T clight[5];  // PNAcc: 1/c^2, 1/c^4, 1/c^5, 1/c^6, 1/c^7
+    int order;    // PNAcc: post-Newtonian order
 };
 
 // ---------------------------------------------------------------------------

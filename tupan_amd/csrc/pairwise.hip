@@ -4,6 +4,7 @@
 // This file is compiled with -ffp-contract=off: every fused multiply-add in
 // ops.h is explicit.
 #include "ops.h"
+#include "pn.h"
 #include "tupan_internal.h"
 
 namespace tupan {
@@ -60,11 +61,20 @@ using Launcher = int (*)(typename Op::Args, hipStream_t, int);
             &run_pairwise<Op, 256, 2, 128, 1>, &run_pairwise<Op, 64, 1, 64, 2>,  \
             &run_pairwise<Op, 256, 1, 256, 4>                                    \
     }
+// Shapes for PNAcc (a few hundred operations and ~100 VGPRs per pair).
+#define TUPAN_PN_SHAPES(Op)                                                      \
+    {                                                                            \
+        &run_pairwise<Op, 256, 1, 128, 1>, &run_pairwise<Op, 128, 1, 128, 1>,    \
+            &run_pairwise<Op, 256, 1, 256, 2>, &run_pairwise<Op, 64, 1, 64, 1>,  \
+            &run_pairwise<Op, 512, 1, 256, 1>                                    \
+    }
 constexpr int kShapes = 5;
 static const int kLightShape[kShapes][4] = {
     {256, 2, 256, 4}, {256, 1, 256, 8}, {256, 4, 256, 2}, {128, 2, 128, 4}, {512, 1, 512, 4}};
 static const int kHeavyShape[kShapes][4] = {
     {256, 1, 128, 2}, {128, 1, 128, 2}, {256, 2, 128, 1}, {64, 1, 64, 2}, {256, 1, 256, 4}};
+static const int kPnShape[kShapes][4] = {
+    {256, 1, 128, 1}, {128, 1, 128, 1}, {256, 1, 256, 2}, {64, 1, 64, 1}, {512, 1, 256, 1}};
 
 // Size buckets by pair count ni * nj.
 constexpr int kBuckets = 3;
@@ -143,6 +153,10 @@ int dispatch<double>(int kernel, int mode, PairArgs<double>& a, hipStream_t s)
     case K_NREG_V:
         if (mode == TUPAN_MODE_IEEE64) TUPAN_RUN(OpNregV<Ieee64>, TUPAN_LIGHT_SHAPES, K_NREG_V, slot);
         TUPAN_RUN(OpNregV<Fast64>, TUPAN_LIGHT_SHAPES, K_NREG_V, slot);
+    case K_PNACC:
+        if (mode == TUPAN_MODE_SPLIT32) TUPAN_RUN(OpSplitPNAcc, TUPAN_PN_SHAPES, K_PNACC, slot);
+        if (mode == TUPAN_MODE_IEEE64) TUPAN_RUN(OpPNAcc<Ieee64>, TUPAN_PN_SHAPES, K_PNACC, slot);
+        TUPAN_RUN(OpPNAcc<Fast64>, TUPAN_PN_SHAPES, K_PNACC, slot);
     default:
         return set_error(TUPAN_ERR_ARG, "unknown pairwise kernel");
     }
@@ -161,6 +175,7 @@ int dispatch<float>(int kernel, int, PairArgs<float>& a, hipStream_t s)
     case K_TSTEP: TUPAN_RUN(OpTstep<Fast32>, TUPAN_LIGHT_SHAPES, K_TSTEP, slot);
     case K_NREG_X: TUPAN_RUN(OpNregX<Fast32>, TUPAN_LIGHT_SHAPES, K_NREG_X, slot);
     case K_NREG_V: TUPAN_RUN(OpNregV<Fast32>, TUPAN_LIGHT_SHAPES, K_NREG_V, slot);
+    case K_PNACC: TUPAN_RUN(OpPNAcc<Fast32>, TUPAN_PN_SHAPES, K_PNACC, slot);
     default: return set_error(TUPAN_ERR_ARG, "unknown pairwise kernel");
     }
 }
@@ -189,7 +204,9 @@ int tune_shape(int kernel, int shape, int* block, int* wpt, int* tile, int* unro
 {
     if (kernel < 0 || kernel >= kNumKernels || shape < 0 || shape >= kShapes)
         return set_error(TUPAN_ERR_ARG, "shape out of range");
-    const int* d = kernel == K_SNAP_CRACKLE ? kHeavyShape[shape] : kLightShape[shape];
+    const int* d = kernel == K_SNAP_CRACKLE ? kHeavyShape[shape]
+                   : kernel == K_PNACC      ? kPnShape[shape]
+                                            : kLightShape[shape];
     if (block) *block = d[0];
     if (wpt) *wpt = d[1];
     if (tile) *tile = d[2];
@@ -199,7 +216,7 @@ int tune_shape(int kernel, int shape, int* block, int* wpt, int* tile, int* unro
 
 template <class T>
 int pairwise_call(int kernel, int mode, size_t ni, const T* const* ip, size_t nj,
-                  const T* const* jp, T* const* out, T eta, void* stream)
+                  const T* const* jp, T* const* out, T eta, void* stream, const PnParams* pn)
 {
     const int nin = kernel_in_fields(kernel);
     const int nout = kernel_out_fields(kernel);
@@ -221,15 +238,30 @@ int pairwise_call(int kernel, int mode, size_t ni, const T* const* ip, size_t nj
     a.ni = static_cast<int>(ni);
     a.nj = static_cast<int>(nj);
     a.eta = eta;
+    if (kernel == K_PNACC) {
+        if (!pn) return set_error(TUPAN_ERR_ARG, "PNAcc needs the PN order and 1/c^k");
+        // pn_terms.h:503-548: order 2 and 3 give 1PN, 4 adds 2PN, 5 2.5PN,
+        // 6 3PN, 7 and above 3.5PN; below 2 every pair term is zero, so the
+        // call runs with no j-particles and stores zeros.
+        a.order = pn->order > 7 ? 7 : static_cast<int>(pn->order);
+        a.clight[0] = static_cast<T>(pn->inv[1]);
+        a.clight[1] = static_cast<T>(pn->inv[3]);
+        a.clight[2] = static_cast<T>(pn->inv[4]);
+        a.clight[3] = static_cast<T>(pn->inv[5]);
+        a.clight[4] = static_cast<T>(pn->inv[6]);
+        if (a.order < 2) a.nj = 0;
+    }
     if (mode == TUPAN_MODE_SPLIT32 && kernel != K_ACC && kernel != K_ACC_JERK &&
-        kernel != K_TSTEP)
-        mode = TUPAN_MODE_FP64;  // split FP32 exists for the forces and Tstep
+        kernel != K_TSTEP && kernel != K_PNACC)
+        mode = TUPAN_MODE_FP64;  // split FP32 exists for the forces, Tstep and PNAcc
     return dispatch<T>(kernel, mode, a, static_cast<hipStream_t>(stream));
 }
 
 template int pairwise_call<double>(int, int, size_t, const double* const*, size_t,
-                                   const double* const*, double* const*, double, void*);
+                                   const double* const*, double* const*, double, void*,
+                                   const PnParams*);
 template int pairwise_call<float>(int, int, size_t, const float* const*, size_t,
-                                  const float* const*, float* const*, float, void*);
+                                  const float* const*, float* const*, float, void*,
+                                  const PnParams*);
 
 }  // namespace tupan

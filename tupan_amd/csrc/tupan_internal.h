@@ -16,8 +16,9 @@ enum KernelId {
     K_TSTEP = TUPAN_K_TSTEP,
     K_NREG_X = TUPAN_K_NREG_X,
     K_NREG_V = TUPAN_K_NREG_V,
+    K_PNACC = TUPAN_K_PNACC,
 };
-constexpr int kNumKernels = 7;
+constexpr int kNumKernels = 8;
 
 // Number of i/j input fields and of outputs of each pairwise kernel, in the
 // reference's argument order (tupan/lib/extensions.py:100-387, 511-615).
@@ -31,6 +32,7 @@ inline int kernel_in_fields(int k)
     case K_TSTEP: return 8;
     case K_NREG_X: return 8;
     case K_NREG_V: return 7;
+    case K_PNACC: return 8;
     default: return -1;
     }
 }
@@ -45,6 +47,7 @@ inline int kernel_out_fields(int k)
     case K_TSTEP: return 2;
     case K_NREG_X: return 7;
     case K_NREG_V: return 4;
+    case K_PNACC: return 3;
     default: return -1;
     }
 }
@@ -78,9 +81,18 @@ int check_launch(const char* what);
 void timing_begin(hipStream_t stream);
 void timing_end(hipStream_t stream);
 
+// Post-Newtonian parameters of PNAcc: the order and 1/c^k, k = 1..7 (the
+// reference's CLIGHT, common.h:16-30).
+struct PnParams {
+    unsigned long order;
+    double inv[7];
+};
+
+// One all-pairs call.  `pn` is required by K_PNACC and ignored otherwise.
 template <class T>
 int pairwise_call(int kernel, int mode, size_t ni, const T* const* ip, size_t nj,
-                  const T* const* jp, T* const* out, T eta, void* stream);
+                  const T* const* jp, T* const* out, T eta, void* stream,
+                  const PnParams* pn = nullptr);
 
 // Tuning table of the pairwise kernels (pairwise.hip).
 int tune_set(int kernel, int slot, int bucket, int shape, int wg_per_cu);

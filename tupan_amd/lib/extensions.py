@@ -31,6 +31,7 @@ __all__ = ["Phi", "phi",
            "Kepler", "kepler",
            "NREG_X", "nreg_x",
            "NREG_V", "nreg_v",
+           "PNAcc", "pnacc", "Clight", "clight",
            "get_kernel", "install"]
 
 logger = logging.getLogger(__name__)
@@ -38,8 +39,7 @@ logger = logging.getLogger(__name__)
 
 class Clight(object):
     """PN order and inverse powers of the speed of light
-    (extensions.py:30-60).  Kept for interface compatibility; no kernel of
-    this build reads it."""
+    (extensions.py:30-60), read by PNAcc at each call."""
 
     def __init__(self):
         self._pn_order = 0
@@ -337,26 +337,40 @@ class NREG_V(AbstractExtension):
                    (ips.mvx, ips.mvy, ips.mvz, ips.mk))
 
 
-class _NotProvided(object):
-    """Singleton placeholder for a reference kernel outside this build."""
+class PNAcc(AbstractExtension):
+    """Post-Newtonian pair accelerations pnax, pnay, pnaz
+    (extensions.py:391-436).  The PN order and 1/c^k come from the module's
+    ``clight`` at each call, as in the reference."""
 
-    def __init__(self, name):
-        self.name = name
+    def __init__(self, backend, prec, **kw):
+        self.kernel = get_kernel("pnacc_kernel", backend, prec, **kw)
+        cty = self.kernel.cty
+        self.argtypes = ((cty.c_uint,) + (cty.c_real_p,) * 8
+                         + (cty.c_uint,) + (cty.c_real_p,) * 8
+                         + (cty.c_uint,) + (cty.c_real,) * 7)
+        self.restypes = (cty.c_real_p,) * 3
 
-    def calc(self, *args, **kwargs):
-        raise NotImplementedError(
-            "%s (the post-Newtonian kernel) is not provided by tupan_amd"
-            % self.name)
-
-    set_args = run = get_result = calc
+    def set_args(self, ips, jps):
+        if clight.pn_order > 0 and clight.clight is None:
+            raise TypeError("PNAcc: 'clight' is not set")
+        ni, nj = ips.n, jps.n
+        self.kernel.set_gsize(ni, nj)
+        _register(ips, "pnax", "pnay", "pnaz")
+        inv = [getattr(clight, "inv%d" % k, 0.0) for k in range(1, 8)]
+        self._bind(ni, nj,
+                   (ni, ips.mass, ips.rx, ips.ry, ips.rz,
+                    ips.eps2, ips.vx, ips.vy, ips.vz,
+                    nj, jps.mass, jps.rx, jps.ry, jps.rz,
+                    jps.eps2, jps.vx, jps.vy, jps.vz,
+                    clight.pn_order) + tuple(inv),
+                   (ips.pnax, ips.pnay, ips.pnaz))
 
 
 backend = "HIP"
 
 clight = Clight()
 phi = acc = acc_jerk = snap_crackle = tstep = sakura = kepler = None
-nreg_x = nreg_v = None
-pnacc = _NotProvided("pnacc")
+nreg_x = nreg_v = pnacc = None
 
 
 def install(backend_=None, prec=None, **kw):
@@ -364,7 +378,7 @@ def install(backend_=None, prec=None, **kw):
     and precision.  Called at import with the HIP backend and
     ``ctype.prec``."""
     global backend, phi, acc, acc_jerk, snap_crackle, tstep, sakura, kepler
-    global nreg_x, nreg_v
+    global nreg_x, nreg_v, pnacc
     backend = "HIP" if backend_ is None else backend_
     prec = ctype.prec if prec is None else prec
     phi = Phi(backend, prec, **kw)
@@ -376,6 +390,7 @@ def install(backend_=None, prec=None, **kw):
     kepler = Kepler(backend, prec, **kw)
     nreg_x = NREG_X(backend, prec, **kw)
     nreg_v = NREG_V(backend, prec, **kw)
+    pnacc = PNAcc(backend, prec, **kw)
 
 
 install()

@@ -43,6 +43,7 @@ KERNEL_SPECS = {
     "kepler_solver_kernel": (8, 0, ("real",), 6),
     "nreg_Xkernel": (8, 8, ("real",), 7),
     "nreg_Vkernel": (7, 7, ("real",), 4),
+    "pnacc_kernel": (8, 8, ("uint",) + ("real",) * 7, 3),
 }
 PAIRWISE_IDS = {
     "phi_kernel": _native.K_PHI,
@@ -52,6 +53,7 @@ PAIRWISE_IDS = {
     "tstep_kernel": _native.K_TSTEP,
     "nreg_Xkernel": _native.K_NREG_X,
     "nreg_Vkernel": _native.K_NREG_V,
+    "pnacc_kernel": _native.K_PNACC,
 }
 
 
@@ -210,6 +212,22 @@ class HIPKernel(object):
         op = _native.ptr_array([b.ptr for b in ob])
         kid = PAIRWISE_IDS[self.name]
         stream = current_stream_handle()
+        if kid == _native.K_PNACC:
+            # (order, 1/c, ..., 1/c^7) as the reference passes them
+            # (extensions.py:424-430)
+            order = int(sc[0])
+            if order < 0:
+                raise ValueError("pnacc_kernel: negative PN order")
+            if self.prec == "float64":
+                inv = (ctypes.c_double * 7)(*[float(x) for x in sc[1:8]])
+                rc = self.lib.tupan_pnacc(_native.MODES[self.mode], ni, ip, nj,
+                                          jp, order, inv, op, stream)
+            else:
+                inv = (ctypes.c_float * 7)(*[float(x) for x in sc[1:8]])
+                rc = self.lib.tupan_pnacc_f32(0, ni, ip, nj, jp, order, inv,
+                                              op, stream)
+            _native.check(rc, self.name)
+            return
         if self.prec == "float64":
             rc = self.lib.tupan_pairwise(kid, _native.MODES[self.mode], ni, ip,
                                          nj, jp, op, eta, stream)
