@@ -17,6 +17,9 @@ def oracle_forces():
     install_oracle("float64")
     yield
     install_hip("float64")
+    # PN runs switch the class-wide flag on, as the reference does
+    from tupan_amd.particles import ParticleSystem
+    ParticleSystem.include_pn_corrections = False
 
 
 def make_ps(d):
@@ -301,3 +304,95 @@ def test_nreg_plummer_second_order():
         errs.append(abs((q.kinetic_energy + q.potential_energy - e0) / e0))
     assert errs[0] < 1e-3
     assert 2.5 < errs[0] / errs[1] < 6.5, errs
+
+
+def pn_group(seed=3):
+    """Six bodies: a compact binary inside a small bound group, speeds of a
+    few tenths, so that (v/c)^2 ~ 1e-3 at c = 20."""
+    rng = np.random.default_rng(seed)
+    n = 6
+    d = {"mass": np.full(n, 1.0 / n), "eps2": np.zeros(n)}
+    pos = rng.normal(0, 0.5, (n, 3))
+    vel = rng.normal(0, 0.3, (n, 3))
+    pos[1] = pos[0] + np.array([0.05, 0.0, 0.0])
+    vel[1] = vel[0] + np.array([0.0, np.sqrt(2 / n / 0.05), 0.0])
+    pos -= (d["mass"][:, None] * pos).sum(0)
+    vel -= (d["mass"][:, None] * vel).sum(0)
+    for k, (p, v) in enumerate(zip(("rx", "ry", "rz"), ("vx", "vy", "vz"))):
+        d[p] = pos[:, k].copy()
+        d[v] = vel[:, k].copy()
+    return d
+
+
+def pn_run(method, tau, nsteps, order=7, c=20.0):
+    from tupan_amd.integrator import Integrator
+    p = make_ps(pn_group())
+    type(p).t_curr = 0.0
+    it = Integrator(tau, 0.0, p, method=method, pn_order=order, clight=c)
+    ps = it.particle_system
+    e0 = ps.kinetic_energy + ps.potential_energy
+    p0, l0, c0 = ps.linear_momentum, ps.angular_momentum, ps.com_r
+    for k in range(nsteps):
+        it.evolve_step(tau * (k + 1))
+    q = it.particle_system
+    return q, e0, p0, l0, c0, type(p).t_curr
+
+
+@pytest.mark.parametrize("method", ["sia21s.dkd", "sia22a.kdk",
+                                    "sia44s.dkd"])
+def test_sia_pn_bookkeeping_conserves_momenta(method):
+    """With the PN impulse and torque kept in pn_mv and pn_am
+    (body.py:496-528), the corrected linear and angular momenta are
+    conserved to rounding, and the corrected centre of mass moves with the
+    corrected momentum, whatever the PN forces do; PN terms did act."""
+    q, e0, p0, l0, c0, t = pn_run(method, 1.0 / 64, 8)
+    assert q.include_pn_corrections
+    assert float(q.pn_ke.abs().max()) > 0 and float(q.pn_mvx.abs().max()) > 0
+    assert np.max(np.abs(q.linear_momentum - p0)) < 1e-15
+    assert np.max(np.abs(q.angular_momentum - l0)) < 1e-14
+    np.testing.assert_allclose(q.com_r, c0 + p0 * t / q.total_mass,
+                               rtol=0, atol=1e-15)
+    e1 = q.kinetic_energy + q.potential_energy
+    assert abs((e1 - e0) / e0) < 5e-3   # 8 steps per binary orbit
+
+
+def test_sia_pn_energy_second_order():
+    """sia21s.dkd with PN kicks: the PN-corrected energy error drops about
+    fourfold when the shared step is halved."""
+    errs = []
+    for tau, n in ((1.0 / 128, 16), (1.0 / 256, 32)):
+        q, e0, *_ = pn_run("sia21s.dkd", tau, n)
+        errs.append(abs((q.kinetic_energy + q.potential_energy - e0) / e0))
+    assert 3.0 < errs[0] / errs[1] < 5.5, errs
+
+
+def test_sia_pn_order_zero_is_newtonian():
+    """pn_order 0 leaves the Newtonian integration untouched; a PN run
+    departs from it at O(1/c^2)."""
+    from tupan_amd.integrator import Integrator
+    out = []
+    for kw in ({}, {"pn_order": 7, "clight": 80.0},
+               {"pn_order": 7, "clight": 160.0}):
+        p = make_ps(pn_group())
+        type(p).t_curr = 0.0
+        it = Integrator(1.0 / 64, 0.0, p, method="sia21s.dkd", **kw)
+        for k in range(4):
+            it.evolve_step((k + 1) / 64)
+        q = it.particle_system
+        assert q.include_pn_corrections == bool(kw)
+        assert ("pn_ke" in q.__dict__) == bool(kw)
+        out.append(q)
+    newt, pn80, pn160 = out
+    d80 = float((newt.vx - pn80.vx).abs().max())
+    d160 = float((newt.vx - pn160.vx).abs().max())
+    assert 3.0 < d80 / d160 < 5.0, (d80, d160)
+    with pytest.raises(TypeError):
+        Integrator(0.1, 0.0, make_ps(pn_group()), method="sia21s.dkd",
+                   pn_order=2)
+
+
+def test_sia_pn_pair_leaf_raises_as_reference():
+    """A hierarchical step that isolates the binary hands FewBody a pair,
+    which the reference refuses with PN corrections (fewbody.py:34-40)."""
+    with pytest.raises(NotImplementedError):
+        pn_run("sia21h.dkd", 1.0 / 16, 2)

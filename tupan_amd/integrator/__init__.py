@@ -19,12 +19,20 @@ class Base(object):
         type(ps).t_curr = time
         self.ps = ps
         self.is_initialized = False
+        # integrator/__init__.py:23-36: a PN order switches on the PN
+        # corrections (used by the SIA methods) and sets the module clight
         pn_order = kwargs.pop("pn_order", 0)
-        kwargs.pop("clight", None)
-        if pn_order > 0:
-            raise NotImplementedError(
-                "post-Newtonian terms are not provided by tupan_amd")
+        clight = kwargs.pop("clight", None)
         type(ps).include_pn_corrections = False
+        if pn_order > 0:
+            if clight is None:
+                raise TypeError(
+                    "'clight' is not defined. Please set the speed of light "
+                    "argument 'clight' when using 'pn_order' > 0.")
+            from ..lib import extensions
+            extensions.clight.pn_order = pn_order
+            extensions.clight.clight = clight
+            type(ps).include_pn_corrections = True
         self.reporter = kwargs.pop("reporter", None)
         self.viewer = kwargs.pop("viewer", None)
         self.dumpper = kwargs.pop("dumpper", None)

@@ -26,8 +26,16 @@ kernel call sees the same particles in the same order as the reference's
 copies, no subset is ever copied out, and the join is a no-op.
 ``SIA(..., inplace=False)`` runs the reference's copy/append form.
 
-Small subsets.  With the HIP backend in FP64 or split-FP32 mode, the 'h'
-variants hand every subset of at most SUBTREE_MAX particles to the device
+Post-Newtonian corrections.  With ``pn_order`` and ``clight`` given, kicks
+add the PN pair accelerations through an auxiliary velocity w (kick_pn and
+the PN branch of sf_kick, sia.py:104-159, 241-294) and keep the PN work,
+impulse and torque in pn_ke, pn_mv, pn_am (drifts move pn_mr), as the
+reference does.  As there, a subset of exactly two particles raises
+NotImplementedError (FewBody's Kepler solver has no PN terms), so with PN
+the 's' and 'a' variants, which never split off pairs, are the usable ones.
+
+Small subsets.  With the HIP backend in FP64 or split-FP32 mode and no PN
+corrections, the 'h' variants hand every subset of at most SUBTREE_MAX particles to the device
 (libtupan_hip: tupan_sia_subtree), which runs that subset's whole recursion
 in one launch with the same operations in the same order, so the deep
 levels -- most of the visits, each a few tiny kernels -- cost one launch per
@@ -127,9 +135,12 @@ def join(slow, fast):
 
 
 def drift(ips, tau):
+    """sia.py:162-171 with drift_n and drift_pn (sia.py:64-101)."""
     ips.rx += ips.vx * tau
     ips.ry += ips.vy * tau
     ips.rz += ips.vz * tau
+    if ips.include_pn_corrections:
+        ips.pn_drift_com_r(tau)
     return ips
 
 
@@ -140,8 +151,38 @@ def kick_n(ips, tau):
     return ips
 
 
+def _pn_half(ps, tau):
+    """v += (a tau + w) / 2."""
+    ps.vx += (ps.ax * tau + ps.wx) / 2
+    ps.vy += (ps.ay * tau + ps.wy) / 2
+    ps.vz += (ps.az * tau + ps.wz) / 2
+
+
+def _pn_book(ps, tau):
+    """PN bookkeeping of one kick, then w <- 2 a_PN tau - w."""
+    ps.pn_kick_ke(tau)
+    ps.pn_kick_lmom(tau)
+    ps.pn_kick_amom(tau)
+    ps.wx[...] = 2 * ps.pnax * tau - ps.wx
+    ps.wy[...] = 2 * ps.pnay * tau - ps.wy
+    ps.wz[...] = 2 * ps.pnaz * tau - ps.wz
+
+
+def kick_pn(ips, tau):
+    """sia.py:104-159: the velocity-dependent PN acceleration enters through
+    the auxiliary velocity w, evaluated once between two half kicks."""
+    _pn_half(ips, tau)
+    ips.set_pnacc(ips)
+    _pn_book(ips, tau)
+    _pn_half(ips, tau)
+    return ips
+
+
 def kick(ips, tau):
+    """sia.py:177-186."""
     ips.set_acc(ips)
+    if ips.include_pn_corrections:
+        return kick_pn(ips, tau)
     return kick_n(ips, tau)
 
 
@@ -152,11 +193,22 @@ def sf_drift(slow, fast, tau, evolve, recurse, bridge):
 
 
 def sf_kick(slow, fast, tau):
+    """sia.py:201-294."""
     if slow.n and fast.n:
         slow.set_acc(fast)
         fast.set_acc(slow)
-        slow = kick_n(slow, tau)
-        fast = kick_n(fast, tau)
+        if slow.include_pn_corrections:
+            _pn_half(slow, tau)
+            _pn_half(fast, tau)
+            slow.set_pnacc(fast)
+            fast.set_pnacc(slow)
+            _pn_book(slow, tau)
+            _pn_book(fast, tau)
+            _pn_half(slow, tau)
+            _pn_half(fast, tau)
+        else:
+            slow = kick_n(slow, tau)
+            fast = kick_n(fast, tau)
     return slow, fast
 
 
@@ -199,8 +251,13 @@ class SIA(Base):
                         for t in ("dkd", "kdk")]
 
     # outputs of the kernels the method calls, registered on the whole
-    # system up front so that slice views share them
+    # system up front so that slice views share them; with PN corrections
+    # also the auxiliary velocity w (sia.py:1002-1005) and the PN
+    # bookkeeping arrays
     AUX = ("tstepij", "ax", "ay", "az")
+    PN_AUX = ("wx", "wy", "wz", "pnax", "pnay", "pnaz", "pn_ke", "pn_mrx",
+              "pn_mry", "pn_mrz", "pn_mvx", "pn_mvy", "pn_mvz", "pn_amx",
+              "pn_amy", "pn_amz")
 
     def __init__(self, eta, time, ps, method, inplace=True,
                  device_subtree=True, **kwargs):
@@ -216,10 +273,12 @@ class SIA(Base):
             else self.scheme.kdk
 
     def do_step(self, ps, tau):
-        if self.inplace:
-            for name in self.AUX:
-                if name not in ps.__dict__:
-                    ps.register_auxiliary_attribute(name, "real")
+        names = self.AUX if self.inplace else ()
+        if ps.include_pn_corrections:
+            names = names + self.PN_AUX
+        for name in names:
+            if name not in ps.__dict__:
+                ps.register_auxiliary_attribute(name, "real")
         self._subtree = _DeviceSubtree.make(self, ps)
         return self.recurse(ps, tau, self.evolve, self.scheme.bridge_sf)
 
@@ -258,7 +317,7 @@ class _DeviceSubtree(object):
         from ..lib import extensions
         from ..lib.hip_backend import HIPKernel
         if not (sia.device_subtree and sia.inplace and sia.update_tstep
-                and not sia.shared_tstep):
+                and not sia.shared_tstep and not ps.include_pn_corrections):
             return None
         kernels = []
         for ext in (extensions.acc, extensions.tstep):

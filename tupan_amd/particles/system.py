@@ -268,6 +268,58 @@ class ParticleSystem(object):
         from ..lib import extensions
         extensions.snap_crackle.calc(self, ps)
 
+    def set_pnacc(self, ps):
+        from ..lib import extensions
+        extensions.pnacc.calc(self, ps)
+
+    # -- post-Newtonian bookkeeping (body.py:464-528) -------------------------
+    # The PN kicks change v with forces the Newtonian diagnostics do not see;
+    # pn_ke, pn_mv and pn_am accumulate minus the work, impulse and torque of
+    # those forces and pn_mr the drift of pn_mv, and the diagnostics below add
+    # them back when include_pn_corrections is set.
+    PN_AUX = ("pnax", "pnay", "pnaz", "pn_ke", "pn_mrx", "pn_mry", "pn_mrz",
+              "pn_mvx", "pn_mvy", "pn_mvz", "pn_amx", "pn_amy", "pn_amz")
+
+    def _aux(self, *names):
+        for name in names:
+            if name not in self.__dict__:
+                self.register_auxiliary_attribute(name, "real")
+        return [self.__dict__[name] for name in names]
+
+    def _pn_force(self):
+        return (self.mass * self.pnax, self.mass * self.pnay,
+                self.mass * self.pnaz)
+
+    def pn_kick_ke(self, tau):
+        """body.py:471-480."""
+        ke, = self._aux("pn_ke")
+        fx, fy, fz = self._pn_force()
+        ke -= (self.vx * fx + self.vy * fy + self.vz * fz) * tau
+
+    def pn_drift_com_r(self, tau):
+        """body.py:482-494."""
+        mrx, mry, mrz, mvx, mvy, mvz = self._aux(
+            "pn_mrx", "pn_mry", "pn_mrz", "pn_mvx", "pn_mvy", "pn_mvz")
+        mrx += mvx * tau
+        mry += mvy * tau
+        mrz += mvz * tau
+
+    def pn_kick_lmom(self, tau):
+        """body.py:496-511."""
+        mvx, mvy, mvz = self._aux("pn_mvx", "pn_mvy", "pn_mvz")
+        fx, fy, fz = self._pn_force()
+        mvx -= fx * tau
+        mvy -= fy * tau
+        mvz -= fz * tau
+
+    def pn_kick_amom(self, tau):
+        """body.py:513-528."""
+        amx, amy, amz = self._aux("pn_amx", "pn_amy", "pn_amz")
+        fx, fy, fz = self._pn_force()
+        amx -= (self.ry * fz - self.rz * fy) * tau
+        amy -= (self.rz * fx - self.rx * fz) * tau
+        amz -= (self.rx * fy - self.ry * fx) * tau
+
     def min_tstep(self):
         return float(self.tstep.abs().min())
 
@@ -291,16 +343,31 @@ class ParticleSystem(object):
     def total_mass(self):
         return fsum(self.mass)
 
+    def _pn_on(self):
+        return bool(self.include_pn_corrections)
+
     @property
     def com_r(self):
+        """body.py:86-107 (PN corrections included when enabled)."""
         mt = self.total_mass
-        return np.array([fsum(self.mass * self.rx), fsum(self.mass * self.ry),
-                         fsum(self.mass * self.rz)]) / mt
+        m = [self.mass * self.rx, self.mass * self.ry, self.mass * self.rz]
+        if self._pn_on():
+            m = [a + b for a, b in zip(m, self._aux("pn_mrx", "pn_mry",
+                                                    "pn_mrz"))]
+        return np.array([fsum(x) for x in m]) / mt
+
+    def _lm(self):
+        p = [self.px, self.py, self.pz]
+        if self._pn_on():
+            p = [a + b for a, b in zip(p, self._aux("pn_mvx", "pn_mvy",
+                                                    "pn_mvz"))]
+        return p
 
     @property
     def com_v(self):
+        """body.py:109-130 (PN corrections included when enabled)."""
         mt = self.total_mass
-        return np.array([fsum(self.px), fsum(self.py), fsum(self.pz)]) / mt
+        return np.array([fsum(x) for x in self._lm()]) / mt
 
     @property
     def com_linear_momentum(self):
@@ -327,14 +394,20 @@ class ParticleSystem(object):
 
     @property
     def linear_momentum(self):
-        return np.array([fsum(self.px), fsum(self.py), fsum(self.pz)])
+        """body.py:178-214 (PN corrections included when enabled)."""
+        return np.array([fsum(x) for x in self._lm()])
 
     @property
     def am(self):
+        """body.py:216-241 (PN corrections included when enabled)."""
         px, py, pz = self.px, self.py, self.pz
-        return torch.stack(((self.ry * pz) - (self.rz * py),
-                            (self.rz * px) - (self.rx * pz),
-                            (self.rx * py) - (self.ry * px)), dim=1)
+        am = [(self.ry * pz) - (self.rz * py),
+              (self.rz * px) - (self.rx * pz),
+              (self.rx * py) - (self.ry * px)]
+        if self._pn_on():
+            am = [a + b for a, b in zip(am, self._aux("pn_amx", "pn_amy",
+                                                      "pn_amz"))]
+        return torch.stack(am, dim=1)
 
     @property
     def angular_momentum(self):
@@ -344,7 +417,11 @@ class ParticleSystem(object):
     # -- energies (body.py:246-321) ---------------------------------------------
     @property
     def ke(self):
-        return 0.5 * self.mass * (self.vx**2 + self.vy**2 + self.vz**2)
+        """body.py:259-274 (PN corrections included when enabled)."""
+        ke = 0.5 * self.mass * (self.vx**2 + self.vy**2 + self.vz**2)
+        if self._pn_on():
+            ke = ke + self._aux("pn_ke")[0]
+        return ke
 
     @property
     def kinetic_energy(self):
