@@ -21,6 +21,11 @@ Also reported:
   block_steps   sia21h.dkd hierarchical block steps on a 262144-particle
                 power-law-mass Plummer sphere, pair interactions per second
                 over whole steps
+  other_kernels Acc, Tstep and PNAcc (order 7, c = 128) on the N = 65536
+                state, split32 and fp64
+  hermite       ahermite4 (eta 0.01) on a 1024-particle Plummer sphere
+                (eps = 4/N) to t = 1: host-driven steps, device steps and
+                device steps replayed as a HIP graph, steps/s and |dE/E|
   cpu_baseline  the CPU oracle (serial C99 restatement of the reference C
                 backend, same operation order, gcc -O3 -std=c99) on a bounded
                 sample of i-rows against all N j-particles, scaled per pair
@@ -66,6 +71,8 @@ def parse():
     ap.add_argument("--sia-tau", type=float, default=0.02 / 64,
                     help="base step of the timed sia21h.dkd block steps")
     ap.add_argument("--sia-steps", type=int, default=2)
+    ap.add_argument("--hermite-n", type=int, default=1024,
+                    help="Plummer N of the ahermite4-to-t=1 leg (0: skip)")
     ap.add_argument("--sia-modes", default="split32,fp64")
     ap.add_argument("--ensemble-lanes", type=int, default=0,
                     help="lanes per system of the ensemble plan (0 = tuned)")
@@ -447,6 +454,73 @@ def cpu_ensemble(nsys):
                       "(%.2f s, %d solves)" % (nsys, t, solves)}
 
 
+def hermite_run(n, eta, t_end, host_steps):
+    """Plummer N = n (equal masses, eps = 4/n, seed 1), ahermite4 with
+    shared block steps to t_end, three ways: steps driven by the host
+    (torch elementwise updates, one library call per force; timed over the
+    first host_steps steps), device steps launched one call per step, and
+    the same device steps replayed as a HIP graph of 64 steps.  Returns the
+    rates, the step count and |dE/E| at t_end."""
+    import torch
+    from tupan_amd.ics.plummer import make_plummer
+    from tupan_amd.integrator import Integrator
+    from tupan_amd.particles import ParticleSystem
+    host = make_plummer(n, 4.0 / n, ("equalmass",), seed=1,
+                        device="cuda:0").to_numpy()
+    out = {"n": n, "eps": 4.0 / n, "method": "ahermite4", "eta": eta,
+           "t_end": t_end, "seed": 1}
+    states = {}
+    for label in ("graph", "device", "host"):
+        ps = ParticleSystem.from_numpy(host, device="cuda:0")
+        type(ps).t_curr = 0.0
+        e0 = ps.kinetic_energy + ps.potential_energy
+        kw = {} if label == "host" else {"device_steps": True}
+        it = Integrator(eta, 0.0, ps, method="ahermite4", **kw)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if label == "graph":
+            it.integrator.evolve(t_end, steps_per_graph=64)
+        else:
+            k = 0
+            while it.time < t_end and (label == "device" or k < host_steps):
+                it.evolve_step(t_end)
+                k += 1
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        q = it.particle_system
+        steps = int(q.nstep[0].item())
+        out[label] = {"steps": steps, "wall_s": wall,
+                      "steps_per_s": steps / wall}
+        if label != "host":
+            e1 = q.kinetic_energy + q.potential_energy
+            out[label]["dE_E"] = abs((e1 - e0) / e0)
+            states[label] = q.to_numpy()
+    out["steps"] = out["graph"]["steps"]
+    out["graph_equals_device"] = all(
+        (states["graph"][k] == states["device"][k]).all()
+        for k in ("rx", "ry", "rz", "vx", "vy", "vz"))
+    return out
+
+
+def cpu_hermite(n, eta):
+    """Serial oracle force work of one ahermite4 step (Tstep + 3 AccJerk on
+    all n x n pairs) of the Hermite workload's initial state; steps/s."""
+    from oracle import Oracle
+    from tupan_amd.ics.plummer import make_plummer
+    host = make_plummer(n, 4.0 / n, ("equalmass",), seed=1,
+                        device="cuda:0").to_numpy()
+    orc = Oracle("float64")
+
+    def step():
+        orc.tstep(host, host, eta)
+        for _ in range(3):
+            orc.acc_jerk(host, host)
+    t = _best(step, 3)
+    return {"value": 1.0 / t, "unit": "steps/s", "cores": 1, "kind": "port",
+            "sample": "Tstep + 3 AccJerk of the N = %d state, best of 3 "
+                      "(%.3f s per step)" % (n, t)}
+
+
 def cpu_baseline(ps, rows):
     """Serial oracle on `rows` i-rows against all j; pairs/s -> Gint/s."""
     import numpy as np
@@ -601,6 +675,11 @@ def main():
             args.sia_n, args.sia_tau, args.sia_steps,
             [m for m in args.sia_modes.split(",") if m])
 
+    if args.hermite_n > 0:
+        log("bench: ahermite4 N=%d to t=1, host, device and graph steps"
+            % args.hermite_n)
+        out["hermite"] = hermite_run(args.hermite_n, 0.01, 1.0, 2000)
+
     if not args.no_cpu:
         rows = args.cpu_rows or 2048
         log("bench: CPU baseline, %d rows" % rows)
@@ -613,6 +692,9 @@ def main():
         if args.ensemble_systems > 0:
             log("bench: CPU baseline, ensemble")
             out["sakura_ensemble"]["cpu_baseline"] = cpu_ensemble(256)
+        if args.hermite_n > 0:
+            log("bench: CPU baseline, Hermite")
+            out["hermite"]["cpu_baseline"] = cpu_hermite(args.hermite_n, 0.01)
         if args.sia_n > 0:
             log("bench: CPU baseline, block steps")
             mode = [m for m in args.sia_modes.split(",") if m][-1]

@@ -459,6 +459,42 @@ int tupan_ensemble_phi(const tupan_ensemble *plan, const double *m,
                        const double *e2, double *phi, void *stream);
 
 /* ---------------------------------------------------------------------------
+ * Device-driven Hermite steps (hermite.py:20-410 with the shared block step
+ * of integrator/__init__.py:48-78).  One call enqueues one whole P(EC)^2
+ * step of order 2, 4, 6 or 8 ('adaptive' = the 'a' methods: Tstep and the
+ * power-of-two block step) on `stream`; the step size is computed on the
+ * device from the control block, so a sequence of calls can be captured as
+ * one HIP graph.  No counterpart in the reference (its driver loop is
+ * Python); the arithmetic is the reference driver's.
+ *
+ *   arrays[TUPAN_H_*]  device arrays of n values: FP64, except nstep (int64);
+ *                      s and c only for orders 6 and 8
+ *   scratch            device FP64 workspace of 32 n values
+ *   control            device FP64 [4]: t_curr, t_end, eta, tau; the call
+ *                      reads t_curr, t_end and eta, writes tau and advances
+ *                      t_curr; once t_curr reaches t_end a call changes
+ *                      nothing (tau = 0)
+ *   eta                the same eta as control[2], for the Tstep kernel
+ *   mode               force mode (TUPAN_MODE_*) of Acc/AccJerk/Tstep
+ * ------------------------------------------------------------------------- */
+#define TUPAN_H_MASS 0
+#define TUPAN_H_EPS2 1
+#define TUPAN_H_RX 2   /* RX RY RZ */
+#define TUPAN_H_VX 5   /* VX VY VZ */
+#define TUPAN_H_AX 8   /* AX AY AZ */
+#define TUPAN_H_JX 11  /* JX JY JZ */
+#define TUPAN_H_SX 14  /* SX SY SZ */
+#define TUPAN_H_CX 17  /* CX CY CZ */
+#define TUPAN_H_TSTEP 20
+#define TUPAN_H_TSTEPIJ 21
+#define TUPAN_H_TIME 22
+#define TUPAN_H_NSTEP 23
+#define TUPAN_H_NROLES 24
+int tupan_hermite_step(int order, int adaptive, int mode, size_t n,
+                       void *const *arrays, double *scratch, double *control,
+                       double eta, void *stream);
+
+/* ---------------------------------------------------------------------------
  * Host layer of the float build.  libtupan_hip_f32.so implements the
  * reference's float ABI (phi_kernel, ... with unsigned int / int / float)
  * on top of these; ifields/jfields/out as for tupan_pairwise.  Sakura and

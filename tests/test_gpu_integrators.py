@@ -189,3 +189,66 @@ def test_sia_pn_gpu_matches_oracle_backend():
             np.testing.assert_allclose(g[k], c[k], rtol=0, atol=1e-11)
         for k in ("vx", "vy", "vz", "pn_ke", "pn_mvx", "pn_amz"):
             np.testing.assert_allclose(g[k], c[k], rtol=0, atol=1e-10)
+
+
+@pytest.mark.parametrize("method,eta,steps", [
+    ("ahermite4", 0.01, 6), ("hermite4", 1.0 / 512, 4),
+    ("ahermite2", 0.005, 4), ("ahermite6", 0.01, 3), ("ahermite8", 0.01, 3),
+])
+def test_device_hermite_steps_match_host_driver(method, eta, steps):
+    """tupan_hermite_step (step size, predictor, corrector and bookkeeping
+    on the device) against the host driver on the same device forces: the
+    same step sizes and time, and states within a few ulps (the host
+    driver's elementwise division by a constant may be torch's reciprocal
+    multiply on the GPU)."""
+    host = cpu_plummer(256, 4.0 / 256, 11)
+    g, tg = run(method, eta, host, steps, "hip")
+    d, td = run(method, eta, host, steps, "hip", device_steps=True)
+    assert tg == td
+    np.testing.assert_array_equal(g["tstep"], d["tstep"])
+    np.testing.assert_array_equal(g["nstep"], d["nstep"])
+    np.testing.assert_array_equal(g["time"], d["time"])
+    for k in ("rx", "ry", "rz", "vx", "vy", "vz"):
+        np.testing.assert_allclose(d[k], g[k], rtol=1e-13, atol=1e-15)
+
+
+def test_device_hermite_graph_replays_equal_eager_steps():
+    """evolve() replaying a graph of 16 device steps equals the same steps
+    launched one by one, bit for bit, and stops at t_end exactly (the steps
+    of the last replay past t_end change nothing)."""
+    from tupan_amd.integrator import Integrator
+    from tupan_amd.particles import ParticleSystem
+    host = cpu_plummer(256, 4.0 / 256, 5)
+    t_end = 0.0625
+    out = []
+    install_hip("float64")
+    for graph in (False, True):
+        ps = ParticleSystem.from_numpy(host, device="cuda:0")
+        type(ps).t_curr = 0.0
+        it = Integrator(0.01, 0.0, ps, method="ahermite4", device_steps=True)
+        if graph:
+            replays = it.integrator.evolve(t_end, steps_per_graph=16)
+            assert replays >= 1
+        else:
+            while it.time < t_end:
+                it.evolve_step(t_end)
+        out.append((it.particle_system.to_numpy(), it.time))
+    (a, ta), (b, tb) = out
+    assert ta == tb == t_end
+    for k in ("rx", "ry", "rz", "vx", "vy", "vz", "ax", "jz", "tstep",
+              "tstepij", "time", "nstep"):
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
+def test_torch_scalar_division_on_device():
+    """Records how torch divides a device tensor by a Python number (true
+    division or multiplication by the reciprocal); the device Hermite step
+    divides as the reference's numpy does."""
+    x = torch.rand(1 << 16, dtype=torch.float64, device="cuda:0") * 10
+    q = (x / 3).cpu().numpy()
+    xn = x.cpu().numpy()
+    true_div = np.array_equal(q, xn / 3)
+    recip = np.array_equal(q, xn * (1.0 / 3))
+    print("torch device x / 3: true division %s, reciprocal multiply %s"
+          % (true_div, recip))
+    assert true_div or recip

@@ -63,6 +63,9 @@ def _declare(lib):
     lib.tupan_pairwise_f32.restype = ctypes.c_int
     lib.tupan_pairwise_f32.argtypes = [ctypes.c_int, ctypes.c_int, _S, _PP, _S,
                                        _PP, _PP, ctypes.c_float, _P]
+    lib.tupan_hermite_step.restype = ctypes.c_int
+    lib.tupan_hermite_step.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                       _S, _PP, _P, _P, ctypes.c_double, _P]
     lib.tupan_pnacc.restype = ctypes.c_int
     lib.tupan_pnacc.argtypes = [ctypes.c_int, _S, _PP, _S, _PP, ctypes.c_ulong,
                                 ctypes.POINTER(ctypes.c_double), _PP, _P]
@@ -185,4 +188,5 @@ DEVICE_ABI = ("tupan_last_error", "tupan_version", "tupan_device_info",
               "tupan_ensemble_phi", "tupan_sakura_ex", "tupan_mlp_sizes",
               "tupan_mlp_grad", "tupan_mlp_forward", "tupan_mlp_adam",
               "tupan_mlp_pack", "tupan_mlp_infer", "tupan_sia_subtree",
-              "tupan_pnacc", "tupan_pnacc_f32", "tupan_host_pnacc_f32")
+              "tupan_pnacc", "tupan_pnacc_f32", "tupan_host_pnacc_f32",
+              "tupan_hermite_step")

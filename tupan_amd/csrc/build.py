@@ -23,7 +23,8 @@ PKG = os.path.dirname(HERE)
 LIBDIR = os.path.join(PKG, "_lib")
 BUILDDIR = os.path.join(PKG, "_build")
 
-SOURCES = ["capi.hip", "pairwise.hip", "sakura.hip", "ensemble.hip", "mlp.hip", "sia.hip"]
+SOURCES = ["capi.hip", "pairwise.hip", "sakura.hip", "ensemble.hip", "mlp.hip", "sia.hip",
+           "hermite.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 

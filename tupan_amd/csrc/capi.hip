@@ -574,6 +574,12 @@ int tupan_pairwise_f32(int kernel, int mode, size_t ni, const float* const* ifie
     return pairwise_call<float>(kernel, mode, ni, ifields, nj, jfields, out, eta, stream);
 }
 
+int tupan_hermite_step(int order, int adaptive, int mode, size_t n, void* const* arrays,
+                       double* scratch, double* control, double eta, void* stream)
+{
+    return hermite_step_call(order, adaptive, mode, n, arrays, scratch, control, eta, stream);
+}
+
 int tupan_pnacc(int mode, size_t ni, const double* const* ifields, size_t nj,
                 const double* const* jfields, unsigned long order, const double* inv7,
                 double* const* out, void* stream)

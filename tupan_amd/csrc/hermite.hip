@@ -1,0 +1,298 @@
+// Device-side Hermite steps (tupan/integrator/hermite.py:20-410 and the
+// shared block step of tupan/integrator/__init__.py:48-78).
+//
+// One call advances the whole system by one P(EC)^2 step with every decision
+// taken on the device: the step size is read from and written to a small
+// control block in device memory, so a sequence of calls can be captured as
+// one HIP graph and replayed with no host round trip.  Per step:
+//
+//   1. 'a' methods: Tstep (eta) into scratch arrays;
+//   2. hermite_tau_kernel: the base step min(|t_end| - |t|, |eta|) and, for
+//      'a' methods, the power-of-two block step below min |tstep| aligned
+//      with t (get_min_block_tstep); tau = 0 once t has reached t_end, which
+//      turns the rest of the step into a no-op;
+//   3. AccJerk (+ SnapCrackle for orders 6, 8) of the current state into the
+//      scratch a0, j0 (s0, c0); the predictor saves r0, v0 and predicts r, v;
+//   4. twice: AccJerk (+ SnapCrackle) of the live state into the scratch a1,
+//      j1 (s1, c1), then the corrector, which also copies them into the live
+//      a, j (s, c);
+//   5. hermite_end_kernel: t += tau, tstep = tau, time += tau, nstep += 1,
+//      tstepij from the scratch Tstep output.
+// Force evaluations write only scratch arrays and every elementwise kernel
+// does nothing when tau = 0, so a step after t_end changes no live array.
+//
+// The predictor and corrector evaluate the reference's polynomials operation
+// by operation as the host driver does on its arrays (one rounding per
+// operation, IEEE division, no contraction: this file is compiled with
+// -ffp-contract=off), so the device-driven trajectory equals the
+// host-driven one bit for bit.  The step-size logic is the host's in double
+// precision; its log2 is the device library's, which can differ from numpy's
+// in the last bit for a minimum time step within an ulp or two of a power of
+// two.
+#include <float.h>
+
+#include "tupan_internal.h"
+
+namespace tupan {
+
+namespace {
+
+constexpr int kCtlT = 0, kCtlTEnd = 1, kCtlEta = 2, kCtlTau = 3;
+constexpr int kTauThreads = 1024;
+
+struct HermiteArrays {
+    double *m, *e2, *r[3], *v[3], *a[3], *j[3], *s[3], *c[3];
+    double *tstep, *tstepij, *time;
+    long long* nstep;
+    // scratch
+    double *r0[3], *v0[3], *a0[3], *j0[3], *s0[3], *c0[3], *ts, *tsij;
+    double *a1[3], *j1[3], *s1[3], *c1[3];
+};
+
+__global__ __launch_bounds__(kTauThreads) void hermite_tau_kernel(int n, const double* ts,
+                                                                   int adaptive, double* ctl)
+{
+    __shared__ double red[kTauThreads];
+    const int tid = static_cast<int>(threadIdx.x);
+    double lo = DBL_MAX;
+    if (adaptive)
+        for (int i = tid; i < n; i += kTauThreads) lo = fmin(lo, fabs(ts[i]));
+    red[tid] = lo;
+    __syncthreads();
+    for (int w = kTauThreads / 2; w > 0; w >>= 1) {
+        if (tid < w) red[tid] = fmin(red[tid], red[tid + w]);
+        __syncthreads();
+    }
+    if (tid != 0) return;
+    const double t = ctl[kCtlT], tend = ctl[kCtlTEnd], eta = ctl[kCtlEta];
+    if ((eta > 0 && t >= tend) || (eta < 0 && t <= tend) || eta == 0) {
+        ctl[kCtlTau] = 0.0;
+        return;
+    }
+    // integrator/__init__.py:48-57
+    double tau = fmin(fabs(tend) - fabs(t), fabs(eta));
+    tau = fmax(tau, fabs(tend) * (2 * DBL_EPSILON));
+    tau = copysign(tau, eta);
+    if (adaptive) {
+        // integrator/__init__.py:59-78
+        const double min_ts = red[0];
+        const long power = static_cast<long>(log2(min_ts) - 1);
+        double bts = ldexp(1.0, static_cast<int>(power));
+        const double t_next = t + bts;
+        while (fmod(t_next, bts) != 0) bts /= 2;
+        bts = copysign(bts, tau);
+        if (fabs(bts) > fabs(tau)) bts = tau;
+        tau = bts;
+    }
+    ctl[kCtlTau] = tau;
+}
+
+// hermite.py:20-410, predictor of H2/H4/H6/H8.
+template <int Order>
+__global__ __launch_bounds__(256) void hermite_predict_kernel(int n, HermiteArrays h,
+                                                              const double* ctl)
+{
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const double tau = ctl[kCtlTau];
+    if (i >= n || tau == 0) return;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double r0 = h.r[d][i], v0 = h.v[d][i];
+        const double a0 = h.a0[d][i];
+        h.r0[d][i] = r0;
+        h.v0[d][i] = v0;
+        double dr, dv;
+        if constexpr (Order == 2) {
+            dr = (a0 * tau / 2 + v0) * tau;
+            dv = a0 * tau;
+        } else if constexpr (Order == 4) {
+            const double j0 = h.j0[d][i];
+            dr = ((j0 * tau / 3 + a0) * tau / 2 + v0) * tau;
+            dv = (j0 * tau / 2 + a0) * tau;
+        } else if constexpr (Order == 6) {
+            const double j0 = h.j0[d][i], s0 = h.s0[d][i];
+            dr = (((s0 * tau / 4 + j0) * tau / 3 + a0) * tau / 2 + v0) * tau;
+            dv = ((s0 * tau / 3 + j0) * tau / 2 + a0) * tau;
+        } else {
+            const double j0 = h.j0[d][i], s0 = h.s0[d][i], c0 = h.c0[d][i];
+            dr = ((((c0 * tau / 5 + s0) * tau / 4 + j0) * tau / 3 + a0) * tau / 2 + v0) * tau;
+            dv = (((c0 * tau / 4 + s0) * tau / 3 + j0) * tau / 2 + a0) * tau;
+        }
+        h.r[d][i] = r0 + dr;
+        h.v[d][i] = v0 + dv;
+    }
+}
+
+// hermite.py:20-410, corrector of H2/H4/H6/H8 (v first, then r with the new v).
+template <int Order>
+__global__ __launch_bounds__(256) void hermite_correct_kernel(int n, HermiteArrays h,
+                                                              const double* ctl)
+{
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const double tau = ctl[kCtlTau];
+    if (i >= n || tau == 0) return;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double r0 = h.r0[d][i], v0 = h.v0[d][i];
+        const double a0 = h.a0[d][i], a1 = h.a1[d][i];
+        h.a[d][i] = a1;
+        if constexpr (Order >= 4) h.j[d][i] = h.j1[d][i];
+        if constexpr (Order >= 6) {
+            h.s[d][i] = h.s1[d][i];
+            h.c[d][i] = h.c1[d][i];
+        }
+        double v1, r1;
+        if constexpr (Order == 2) {
+            v1 = (a0 + a1) * tau / 2 + v0;
+            r1 = (v0 + v1) * tau / 2 + r0;
+        } else if constexpr (Order == 4) {
+            const double j0 = h.j0[d][i], j1 = h.j1[d][i];
+            v1 = ((j0 - j1) * tau / 6 + (a0 + a1)) * tau / 2 + v0;
+            r1 = ((a0 - a1) * tau / 6 + (v0 + v1)) * tau / 2 + r0;
+        } else if constexpr (Order == 6) {
+            const double j0 = h.j0[d][i], j1 = h.j1[d][i];
+            const double s0 = h.s0[d][i], s1 = h.s1[d][i];
+            v1 = (((s0 + s1) * tau / 12 + (j0 - j1)) * tau / 5 + (a0 + a1)) * tau / 2 + v0;
+            r1 = (((j0 + j1) * tau / 12 + (a0 - a1)) * tau / 5 + (v0 + v1)) * tau / 2 + r0;
+        } else {
+            const double j0 = h.j0[d][i], j1 = h.j1[d][i];
+            const double s0 = h.s0[d][i], s1 = h.s1[d][i];
+            const double c0 = h.c0[d][i], c1 = h.c1[d][i];
+            v1 = ((((c0 - c1) * tau / 20 + (s0 + s1)) * tau / 3 + 3 * (j0 - j1)) * tau / 14 +
+                  (a0 + a1)) * tau / 2 + v0;
+            r1 = ((((s0 - s1) * tau / 20 + (j0 + j1)) * tau / 3 + 3 * (a0 - a1)) * tau / 14 +
+                  (v0 + v1)) * tau / 2 + r0;
+        }
+        h.v[d][i] = v1;
+        h.r[d][i] = r1;
+    }
+}
+
+// hermite.py:390-410 (do_step's bookkeeping).  Thread 0 of block 0 also
+// advances t; every thread reads tau before that write is visible to it or
+// not at all, since tau is not changed here.
+__global__ __launch_bounds__(256) void hermite_end_kernel(int n, HermiteArrays h, int adaptive,
+                                                          double* ctl)
+{
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const double tau = ctl[kCtlTau];
+    if (tau == 0) return;
+    if (i < n) {
+        h.tstep[i] = tau;
+        h.time[i] = h.time[i] + tau;
+        h.nstep[i] = h.nstep[i] + 1;
+        if (adaptive) h.tstepij[i] = h.tsij[i];
+    }
+    if (i == 0) ctl[kCtlT] = ctl[kCtlT] + tau;
+}
+
+int forces(int order, int mode, size_t n, const HermiteArrays& h, bool start, hipStream_t s)
+{
+    // AccJerk: i and j fields m rx ry rz e2 vx vy vz
+    const double* in[8] = {h.m, h.r[0], h.r[1], h.r[2], h.e2, h.v[0], h.v[1], h.v[2]};
+    double* const* A = start ? h.a0 : h.a1;
+    double* const* J = start ? h.j0 : h.j1;
+    double* aj[6] = {A[0], A[1], A[2], J[0], J[1], J[2]};
+    const int kernel = order == 2 ? K_ACC : K_ACC_JERK;
+    const double* in5[5] = {h.m, h.r[0], h.r[1], h.r[2], h.e2};
+    int rc = kernel == K_ACC
+                 ? pairwise_call<double>(K_ACC, mode, n, in5, n, in5, aj, 0.0, s)
+                 : pairwise_call<double>(K_ACC_JERK, mode, n, in, n, in, aj, 0.0, s);
+    if (rc != TUPAN_OK || order < 6) return rc;
+    double* const* S = start ? h.s0 : h.s1;
+    double* const* C = start ? h.c0 : h.c1;
+    const double* in14[14] = {h.m,    h.r[0], h.r[1], h.r[2], h.e2,   h.v[0], h.v[1],
+                              h.v[2], A[0],   A[1],   A[2],   J[0],   J[1],   J[2]};
+    double* sc[6] = {S[0], S[1], S[2], C[0], C[1], C[2]};
+    const int smode = mode == TUPAN_MODE_SPLIT32 ? TUPAN_MODE_FP64 : mode;
+    return pairwise_call<double>(K_SNAP_CRACKLE, smode, n, in14, n, in14, sc, 0.0, s);
+}
+
+template <int Order>
+int step_order(int adaptive, int mode, int n, const HermiteArrays& h, double eta, double* ctl,
+               hipStream_t s)
+{
+    const unsigned grid = static_cast<unsigned>((n + 255) / 256);
+    if (adaptive) {
+        const double* in[8] = {h.m, h.r[0], h.r[1], h.r[2], h.e2, h.v[0], h.v[1], h.v[2]};
+        double* out[2] = {h.ts, h.tsij};
+        int rc = pairwise_call<double>(K_TSTEP, mode, n, in, n, in, out, eta, s);
+        if (rc != TUPAN_OK) return rc;
+    }
+    hipLaunchKernelGGL(hermite_tau_kernel, dim3(1), dim3(kTauThreads), 0, s, n, h.ts, adaptive,
+                       ctl);
+    int rc = check_launch("hermite_tau");
+    if (rc == TUPAN_OK) rc = forces(Order, mode, n, h, true, s);
+    if (rc != TUPAN_OK) return rc;
+    hipLaunchKernelGGL((hermite_predict_kernel<Order>), dim3(grid), dim3(256), 0, s, n, h, ctl);
+    rc = check_launch("hermite_predict");
+    for (int k = 0; k < 2 && rc == TUPAN_OK; ++k) {
+        rc = forces(Order, mode, n, h, false, s);
+        if (rc != TUPAN_OK) break;
+        hipLaunchKernelGGL((hermite_correct_kernel<Order>), dim3(grid), dim3(256), 0, s, n, h,
+                           ctl);
+        rc = check_launch("hermite_correct");
+    }
+    if (rc != TUPAN_OK) return rc;
+    hipLaunchKernelGGL(hermite_end_kernel, dim3(grid), dim3(256), 0, s, n, h, adaptive, ctl);
+    return check_launch("hermite_end");
+}
+
+}  // namespace
+
+int hermite_step_call(int order, int adaptive, int mode, size_t n, void* const* arrays,
+                      double* scratch, double* control, double eta, void* stream)
+{
+    if (order != 2 && order != 4 && order != 6 && order != 8)
+        return set_error(TUPAN_ERR_ARG, "Hermite order must be 2, 4, 6 or 8");
+    if (mode != TUPAN_MODE_FP64 && mode != TUPAN_MODE_SPLIT32 && mode != TUPAN_MODE_IEEE64)
+        return set_error(TUPAN_ERR_ARG, "unknown force mode");
+    if (n == 0) return TUPAN_OK;
+    if (n > static_cast<size_t>(INT32_MAX)) return set_error(TUPAN_ERR_ARG, "n exceeds 2^31-1");
+    if (!arrays || !scratch || !control) return set_error(TUPAN_ERR_ARG, "null argument");
+    const int nreq = order >= 6 ? TUPAN_H_NROLES : TUPAN_H_SX;
+    for (int q = 0; q < TUPAN_H_NROLES; ++q)
+        if ((q < nreq || q >= TUPAN_H_TSTEP) && !arrays[q])
+            return set_error(TUPAN_ERR_ARG, "null particle array");
+    HermiteArrays h{};
+    auto d = [&](int q) { return static_cast<double*>(arrays[q]); };
+    h.m = d(TUPAN_H_MASS);
+    h.e2 = d(TUPAN_H_EPS2);
+    for (int k = 0; k < 3; ++k) {
+        h.r[k] = d(TUPAN_H_RX + k);
+        h.v[k] = d(TUPAN_H_VX + k);
+        h.a[k] = d(TUPAN_H_AX + k);
+        h.j[k] = d(TUPAN_H_JX + k);
+        h.s[k] = order >= 6 ? d(TUPAN_H_SX + k) : nullptr;
+        h.c[k] = order >= 6 ? d(TUPAN_H_CX + k) : nullptr;
+    }
+    h.tstep = d(TUPAN_H_TSTEP);
+    h.tstepij = d(TUPAN_H_TSTEPIJ);
+    h.time = d(TUPAN_H_TIME);
+    h.nstep = static_cast<long long*>(arrays[TUPAN_H_NSTEP]);
+    double* sp = scratch;
+    for (int k = 0; k < 3; ++k) {
+        h.r0[k] = sp + (0 + k) * n;
+        h.v0[k] = sp + (3 + k) * n;
+        h.a0[k] = sp + (6 + k) * n;
+        h.j0[k] = sp + (9 + k) * n;
+        h.s0[k] = sp + (12 + k) * n;
+        h.c0[k] = sp + (15 + k) * n;
+        h.a1[k] = sp + (20 + k) * n;
+        h.j1[k] = sp + (23 + k) * n;
+        h.s1[k] = sp + (26 + k) * n;
+        h.c1[k] = sp + (29 + k) * n;
+    }
+    h.ts = sp + 18 * n;
+    h.tsij = sp + 19 * n;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int ni = static_cast<int>(n);
+    switch (order) {
+    case 2: return step_order<2>(adaptive, mode, ni, h, eta, control, s);
+    case 4: return step_order<4>(adaptive, mode, ni, h, eta, control, s);
+    case 6: return step_order<6>(adaptive, mode, ni, h, eta, control, s);
+    default: return step_order<8>(adaptive, mode, ni, h, eta, control, s);
+    }
+}
+
+}  // namespace tupan

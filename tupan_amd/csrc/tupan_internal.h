@@ -99,6 +99,9 @@ int tune_set(int kernel, int slot, int bucket, int shape, int wg_per_cu);
 int tune_get(int kernel, int slot, int bucket, int* shape, int* wg_per_cu);
 int tune_shape(int kernel, int shape, int* block, int* wpt, int* tile, int* unroll);
 
+int hermite_step_call(int order, int adaptive, int mode, size_t n, void* const* arrays,
+                      double* scratch, double* control, double eta, void* stream);
+
 int sakura_call(size_t ni, const double* const* ip, size_t nj, const double* const* jp,
                 double dt, long flag, double* const* out, unsigned long long* counters,
                 int* status, void* stream);
