@@ -396,3 +396,30 @@ def test_sia_pn_pair_leaf_raises_as_reference():
     which the reference refuses with PN corrections (fewbody.py:34-40)."""
     with pytest.raises(NotImplementedError):
         pn_run("sia21h.dkd", 1.0 / 16, 2)
+
+
+def test_diagnostic_report_file(tmp_path):
+    """Diagnostic(fname, time, ps, report_freq) writes the reference's
+    header line, then one line per report_freq reports with 13 columns;
+    te = ke + pe and eerr = (te - te0)/(-pe) as printed."""
+    from tupan_amd.diagnostics import Diagnostic
+    from tupan_amd.integrator import Integrator
+    from tupan_amd.ics.plummer import make_plummer
+    p = make_plummer(32, 0.05, ("equalmass",), seed=2, device="cpu")
+    type(p).t_curr = 0.0
+    fname = str(tmp_path / "diag.txt")
+    rep = Diagnostic(fname, 0.0, p, report_freq=2)
+    it = Integrator(1.0 / 64, 0.0, p, method="ahermite4", reporter=rep)
+    for k in range(6):
+        it.evolve_step((k + 1) / 64)
+    lines = open(fname).read().splitlines()
+    assert lines[0].split() == list(Diagnostic.HEADER)
+    assert lines[0].startswith("#00:time      #01:dtime  #02:ke     ")
+    body = [[float(x) for x in ln.split()] for ln in lines[1:]]
+    assert len(body) == 4 and all(len(r) == 13 for r in body)   # 7 reports
+    for r in body:
+        t, dt, ke, pe, te, vir, eerr = r[:7]
+        assert abs(te - (ke + pe)) < 1e-3 * abs(te)   # ke, pe: 4 digits
+        assert abs(vir - (2 * ke + pe)) < 1e-2
+    assert body[0][0] == 0.0 and body[0][6] == 0.0
+    assert rep.rows[-1]["time"] == it.time
