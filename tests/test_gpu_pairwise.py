@@ -422,6 +422,19 @@ def test_empty_and_degenerate():
     r = extensions.AccJerk("HIP", "float64").calc(one, one)
     for x in r:
         assert float(x[0]) == 0.0
+    # PNAcc: nj == 0, ni == 0, a lone particle, orders 0 and 1 -> zeros
+    for mode in ("fp64", "ieee64", "split32"):
+        pn = extensions.PNAcc("HIP", "float64", mode=mode)
+        for order in (0, 1, 7):
+            set_clight(order, 16.0)
+            for x in pn.calc(ps, empty):
+                assert torch.all(x == 0)
+            assert all(x.numel() == 0 for x in pn.calc(empty, ps))
+            for x in pn.calc(one, one):
+                assert float(x[0]) == 0.0
+            if order < 2:
+                for x in pn.calc(ps, ps):
+                    assert torch.all(x == 0)
     r = extensions.Phi("HIP", "float64").calc(one, one)
     assert float(r[0][0]) == 0.0
 

@@ -31,7 +31,7 @@ def tuned_ensemble_lanes():
 
 SLOTS = {"fp64": 0, "split32": 1, "ieee64": 2, "float32": 3}
 KERNEL_IDS = {"phi": 0, "acc": 1, "acc_jerk": 2, "snap_crackle": 3,
-              "tstep": 4}
+              "tstep": 4, "nreg_x": 5, "nreg_v": 6, "pnacc": 7}
 
 
 def apply(lib):

@@ -17,7 +17,7 @@ import numpy as np
 import torch
 
 from .. import _native
-from . import CACHE, save
+from . import CACHE, load, save
 
 KERNELS = {  # name -> (kernel id, modes)
     "phi": (_native.K_PHI, ("fp64",)),
@@ -25,6 +25,7 @@ KERNELS = {  # name -> (kernel id, modes)
     "acc_jerk": (_native.K_ACC_JERK, ("fp64", "split32", "ieee64")),
     "snap_crackle": (_native.K_SNAP_CRACKLE, ("fp64",)),
     "tstep": (_native.K_TSTEP, ("fp64", "split32")),
+    "pnacc": (_native.K_PNACC, ("fp64", "split32")),
 }
 SLOT = {"fp64": 0, "split32": 1, "ieee64": 2}
 BUCKET_N = (1024, 16384, 65536)
@@ -55,18 +56,23 @@ def time_call(fn, reps):
     return best
 
 
-def tune_pairwise(quick):
+def tune_pairwise(quick, names):
     from ..lib import extensions
     lib = _native.lib()
     out = []
     sizes = BUCKET_N[:2] if quick else BUCKET_N
+    # PNAcc at the reference's own test values (order 7, c = 128)
+    extensions.clight.pn_order, extensions.clight.clight = 7, 128.0
     for b, n in enumerate(sizes):
         ps = cluster(n)
         for name, (kid, modes) in KERNELS.items():
+            if name not in names:
+                continue
             cls = {"phi": extensions.Phi, "acc": extensions.Acc,
                    "acc_jerk": extensions.AccJerk,
                    "snap_crackle": extensions.SnapCrackle,
-                   "tstep": extensions.Tstep}[name]
+                   "tstep": extensions.Tstep,
+                   "pnacc": extensions.PNAcc}[name]
             for mode in modes:
                 ext = cls("HIP", "float64", mode=mode)
                 args = (0.01,) if name == "tstep" else ()
@@ -112,14 +118,25 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--quick", action="store_true")
     ap.add_argument("--ensemble-systems", type=int, default=4096)
+    ap.add_argument("--kernels", default=",".join(KERNELS),
+                    help="comma-separated subset; the cache keeps the other "
+                         "kernels' entries")
     args = ap.parse_args(argv)
     assert torch.cuda.is_available(), "the autotuner runs on the GPU"
+    names = [k for k in args.kernels.split(",") if k]
+    for k in names:
+        if k not in KERNELS:
+            raise SystemExit("unknown kernel %r" % k)
     cu, _ = _native.device_info()
     t0 = time.time()
+    old = load()
+    keep = [e for e in old.get("pairwise", []) if e.get("kernel") not in names]
     data = {"device": {"cu": cu, "name": torch.cuda.get_device_name(0)},
-            "pairwise": tune_pairwise(args.quick)}
+            "pairwise": keep + tune_pairwise(args.quick, names)}
     if args.ensemble_systems > 0:
         data["ensemble"] = tune_ensemble(args.ensemble_systems)
+    elif "ensemble" in old:
+        data["ensemble"] = old["ensemble"]
     data["seconds"] = time.time() - t0
     save(data)
     print("wrote", CACHE)
