@@ -72,7 +72,9 @@ def parse():
                     help="base step of the timed sia21h.dkd block steps")
     ap.add_argument("--sia-steps", type=int, default=2)
     ap.add_argument("--hermite-n", type=int, default=1024,
-                    help="Plummer N of the ahermite4-to-t=1 leg (0: skip)")
+                    help="Plummer N of the ahermite4 leg (0: skip)")
+    ap.add_argument("--hermite-t-end", type=float, default=1.0,
+                    help="end time of the ahermite4 leg")
     ap.add_argument("--sia-modes", default="split32,fp64")
     ap.add_argument("--ensemble-lanes", type=int, default=0,
                     help="lanes per system of the ensemble plan (0 = tuned)")
@@ -676,9 +678,10 @@ def main():
             [m for m in args.sia_modes.split(",") if m])
 
     if args.hermite_n > 0:
-        log("bench: ahermite4 N=%d to t=1, host, device and graph steps"
-            % args.hermite_n)
-        out["hermite"] = hermite_run(args.hermite_n, 0.01, 1.0, 2000)
+        log("bench: ahermite4 N=%d to t=%g, host, device and graph steps"
+            % (args.hermite_n, args.hermite_t_end))
+        out["hermite"] = hermite_run(args.hermite_n, 0.01, args.hermite_t_end,
+                                     2000)
 
     if not args.no_cpu:
         rows = args.cpu_rows or 2048

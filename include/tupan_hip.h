@@ -470,10 +470,14 @@ int tupan_ensemble_phi(const tupan_ensemble *plan, const double *m,
  *   arrays[TUPAN_H_*]  device arrays of n values: FP64, except nstep (int64);
  *                      s and c only for orders 6 and 8
  *   scratch            device FP64 workspace of 32 n values
- *   control            device FP64 [4]: t_curr, t_end, eta, tau; the call
- *                      reads t_curr, t_end and eta, writes tau and advances
- *                      t_curr; once t_curr reaches t_end a call changes
- *                      nothing (tau = 0)
+ *   control            device FP64 [5]: t_curr, t_end, eta, tau, status; the
+ *                      call reads t_curr, t_end and eta, writes tau and
+ *                      advances t_curr; once t_curr reaches t_end a call
+ *                      changes nothing (tau = 0); status (caller-zeroed)
+ *                      becomes 1 for a non-finite t_curr, t_end or eta, 2
+ *                      when Tstep gives no positive finite step, 3 when no
+ *                      aligned block step exists, and then every later call
+ *                      changes nothing; the host must check it
  *   eta                the same eta as control[2], for the Tstep kernel
  *   mode               force mode (TUPAN_MODE_*) of Acc/AccJerk/Tstep
  * ------------------------------------------------------------------------- */

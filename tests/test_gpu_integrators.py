@@ -252,3 +252,23 @@ def test_torch_scalar_division_on_device():
     print("torch device x / 3: true division %s, reciprocal multiply %s"
           % (true_div, recip))
     assert true_div or recip
+
+
+def test_device_hermite_bad_control_raises():
+    """A non-finite end time, set from the host, makes the step-size kernel
+    flag its status and the host raise; the kernels exit normally and the
+    particles are untouched."""
+    from tupan_amd import _native
+    from tupan_amd.integrator import Integrator
+    from tupan_amd.particles import ParticleSystem
+    host = cpu_plummer(64, 4.0 / 64, 3)
+    install_hip("float64")
+    ps = ParticleSystem.from_numpy(host, device="cuda:0")
+    type(ps).t_curr = 0.0
+    it = Integrator(0.01, 0.0, ps, method="ahermite4", device_steps=True)
+    with pytest.raises(_native.TupanHipError):
+        it.evolve_step(float("nan"))
+    q = it.particle_system.to_numpy()
+    for k in ("rx", "vx", "time", "nstep"):
+        np.testing.assert_array_equal(q[k], host[k] if k in host else q[k])
+    assert int(q["nstep"].max()) == 0

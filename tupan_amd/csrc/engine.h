@@ -122,27 +122,28 @@ __global__ __launch_bounds__(256) void reduce_kernel(typename Op::Args a)
 struct Launch {
     int nblock_i;  // i blocks
     int nsplit;    // j splits
-    int jchunk;    // j per split (multiple of the tile)
+    int jchunk;    // j per split (multiple of the minimum chunk)
 };
 
 // Pick the j-split so that the grid has at least `target_wg` workgroups
-// (several per CU) without cutting j ranges below one tile.
+// (several per CU) without cutting j ranges below `min_chunk` j-particles
+// (a multiple of 16; the tile loop handles a partly filled last tile).
 inline Launch plan_launch(int ni, int nj, int block, int wpt, int tile, int target_wg,
-                          int max_split)
+                          int max_split, int min_chunk)
 {
     Launch L;
     const int per_block = block * wpt;
     L.nblock_i = (ni + per_block - 1) / per_block;
     int want = (target_wg + L.nblock_i - 1) / L.nblock_i;
-    int max_by_j = (nj + tile - 1) / tile;
+    int max_by_j = (nj + min_chunk - 1) / min_chunk;
     int ns = want < max_by_j ? want : max_by_j;
     if (ns > max_split) ns = max_split;
     if (ns < 1) ns = 1;
-    // j range per split, rounded to whole tiles, then the split count that
-    // range actually needs
+    // j range per split, rounded to whole multiples of min_chunk, then the
+    // split count that range actually needs
     int chunk = (nj + ns - 1) / ns;
-    chunk = ((chunk + tile - 1) / tile) * tile;
-    if (chunk < tile) chunk = tile;
+    chunk = ((chunk + min_chunk - 1) / min_chunk) * min_chunk;
+    if (chunk < min_chunk) chunk = min_chunk;
     L.jchunk = chunk;
     L.nsplit = nj > 0 ? (nj + chunk - 1) / chunk : 1;
     return L;

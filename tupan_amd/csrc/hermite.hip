@@ -10,7 +10,9 @@
 //   2. hermite_tau_kernel: the base step min(|t_end| - |t|, |eta|) and, for
 //      'a' methods, the power-of-two block step below min |tstep| aligned
 //      with t (get_min_block_tstep); tau = 0 once t has reached t_end, which
-//      turns the rest of the step into a no-op;
+//      turns the rest of the step into a no-op, and also once the control
+//      block's status is nonzero (1: non-finite t, t_end or eta; 2: no
+//      positive finite Tstep; 3: no aligned block step), which it keeps;
 //   3. AccJerk (+ SnapCrackle for orders 6, 8) of the current state into the
 //      scratch a0, j0 (s0, c0); the predictor saves r0, v0 and predicts r, v;
 //   4. twice: AccJerk (+ SnapCrackle) of the live state into the scratch a1,
@@ -37,7 +39,10 @@ namespace tupan {
 
 namespace {
 
-constexpr int kCtlT = 0, kCtlTEnd = 1, kCtlEta = 2, kCtlTau = 3;
+constexpr int kCtlT = 0, kCtlTEnd = 1, kCtlEta = 2, kCtlTau = 3, kCtlStatus = 4;
+// Halvings of the block step are bounded: from a step >= 2^-1075 down to
+// the spacing of t_next takes at most ~2100.
+constexpr int kMaxHalvings = 2200;
 constexpr int kTauThreads = 1024;
 
 struct HermiteArrays {
@@ -65,6 +70,11 @@ __global__ __launch_bounds__(kTauThreads) void hermite_tau_kernel(int n, const d
     }
     if (tid != 0) return;
     const double t = ctl[kCtlT], tend = ctl[kCtlTEnd], eta = ctl[kCtlEta];
+    if (ctl[kCtlStatus] != 0 || !isfinite(t) || !isfinite(tend) || !isfinite(eta)) {
+        ctl[kCtlTau] = 0.0;
+        if (ctl[kCtlStatus] == 0) ctl[kCtlStatus] = 1.0;
+        return;
+    }
     if ((eta > 0 && t >= tend) || (eta < 0 && t <= tend) || eta == 0) {
         ctl[kCtlTau] = 0.0;
         return;
@@ -76,10 +86,30 @@ __global__ __launch_bounds__(kTauThreads) void hermite_tau_kernel(int n, const d
     if (adaptive) {
         // integrator/__init__.py:59-78
         const double min_ts = red[0];
-        const long power = static_cast<long>(log2(min_ts) - 1);
-        double bts = ldexp(1.0, static_cast<int>(power));
+        if (!(min_ts > 0) || !isfinite(min_ts)) {
+            // Tstep gave no positive finite step (NaN or zero inputs)
+            ctl[kCtlTau] = 0.0;
+            ctl[kCtlStatus] = 2.0;
+            return;
+        }
+        const double p = log2(min_ts) - 1;
+        if (p < -1070.0 || p > 1020.0) {
+            ctl[kCtlTau] = 0.0;
+            ctl[kCtlStatus] = 2.0;
+            return;
+        }
+        double bts = ldexp(1.0, static_cast<int>(p));
         const double t_next = t + bts;
-        while (fmod(t_next, bts) != 0) bts /= 2;
+        int halvings = 0;
+        while (fmod(t_next, bts) != 0 && halvings < kMaxHalvings) {
+            bts /= 2;
+            ++halvings;
+        }
+        if (halvings == kMaxHalvings || bts == 0) {
+            ctl[kCtlTau] = 0.0;
+            ctl[kCtlStatus] = 3.0;
+            return;
+        }
         bts = copysign(bts, tau);
         if (fabs(bts) > fabs(tau)) bts = tau;
         tau = bts;

@@ -134,6 +134,7 @@ struct PairArgs {
     T eta;
     T clight[5];  // PNAcc: 1/c^2, 1/c^4, 1/c^5, 1/c^6, 1/c^7
     int order;    // PNAcc: post-Newtonian order
+    int exact;    // host-side launch planning: keep whole-tile j-chunks
 };
 
 // ---------------------------------------------------------------------------

@@ -23,7 +23,13 @@ static int run_pairwise(typename Op::Args a, hipStream_t stream, int wg_per_cu)
     if (a.ni <= 0) return 0;
     const Device& dev = current_device();
     const int target = (wg_per_cu > 0 ? wg_per_cu : 4) * dev.cu_count;
-    Launch L = plan_launch(a.ni, a.nj, BLOCK, WPT, TILE, target, 64);
+    // j-chunks of whole tiles, except for sets well above a tile in the
+    // modes that sum in any order: there 32-particle chunks let a small
+    // system (N ~ 1e3) fill the chip with j-splits.  ieee64 keeps whole
+    // tiles, so any set of at most one tile is one split (the reference's
+    // summation order).
+    const int min_chunk = (!a.exact && a.nj > 4 * TILE) ? 32 : TILE;
+    Launch L = plan_launch(a.ni, a.nj, BLOCK, WPT, TILE, target, 64, min_chunk);
     a.nsplit = L.nsplit;
     a.jchunk = L.jchunk;
     a.partial = nullptr;
@@ -47,34 +53,38 @@ static int run_pairwise(typename Op::Args a, hipStream_t stream, int wg_per_cu)
 template <class Op>
 using Launcher = int (*)(typename Op::Args, hipStream_t, int);
 
-// Shapes for the light interactions (Phi, Acc, AccJerk, Tstep, split).
+// Shapes for the light interactions (Phi, Acc, AccJerk, Tstep, split); the
+// last, one wave per workgroup, is for small systems.
 #define TUPAN_LIGHT_SHAPES(Op)                                                   \
     {                                                                            \
         &run_pairwise<Op, 256, 2, 256, 4>, &run_pairwise<Op, 256, 1, 256, 8>,    \
             &run_pairwise<Op, 256, 4, 256, 2>, &run_pairwise<Op, 128, 2, 128, 4>, \
-            &run_pairwise<Op, 512, 1, 512, 4>                                    \
+            &run_pairwise<Op, 512, 1, 512, 4>, &run_pairwise<Op, 64, 1, 64, 4>   \
     }
 // Shapes for SnapCrackle (twice the registers per i).
 #define TUPAN_HEAVY_SHAPES(Op)                                                   \
     {                                                                            \
         &run_pairwise<Op, 256, 1, 128, 2>, &run_pairwise<Op, 128, 1, 128, 2>,    \
             &run_pairwise<Op, 256, 2, 128, 1>, &run_pairwise<Op, 64, 1, 64, 2>,  \
-            &run_pairwise<Op, 256, 1, 256, 4>                                    \
+            &run_pairwise<Op, 256, 1, 256, 4>, &run_pairwise<Op, 64, 1, 32, 1>   \
     }
 // Shapes for PNAcc (a few hundred operations and ~100 VGPRs per pair).
 #define TUPAN_PN_SHAPES(Op)                                                      \
     {                                                                            \
         &run_pairwise<Op, 256, 1, 128, 1>, &run_pairwise<Op, 128, 1, 128, 1>,    \
             &run_pairwise<Op, 256, 1, 256, 2>, &run_pairwise<Op, 64, 1, 64, 1>,  \
-            &run_pairwise<Op, 512, 1, 256, 1>                                    \
+            &run_pairwise<Op, 512, 1, 256, 1>, &run_pairwise<Op, 64, 1, 32, 1>   \
     }
-constexpr int kShapes = 5;
+constexpr int kShapes = 6;
 static const int kLightShape[kShapes][4] = {
-    {256, 2, 256, 4}, {256, 1, 256, 8}, {256, 4, 256, 2}, {128, 2, 128, 4}, {512, 1, 512, 4}};
+    {256, 2, 256, 4}, {256, 1, 256, 8}, {256, 4, 256, 2}, {128, 2, 128, 4}, {512, 1, 512, 4},
+    {64, 1, 64, 4}};
 static const int kHeavyShape[kShapes][4] = {
-    {256, 1, 128, 2}, {128, 1, 128, 2}, {256, 2, 128, 1}, {64, 1, 64, 2}, {256, 1, 256, 4}};
+    {256, 1, 128, 2}, {128, 1, 128, 2}, {256, 2, 128, 1}, {64, 1, 64, 2}, {256, 1, 256, 4},
+    {64, 1, 32, 1}};
 static const int kPnShape[kShapes][4] = {
-    {256, 1, 128, 1}, {128, 1, 128, 1}, {256, 1, 256, 2}, {64, 1, 64, 1}, {512, 1, 256, 1}};
+    {256, 1, 128, 1}, {128, 1, 128, 1}, {256, 1, 256, 2}, {64, 1, 64, 1}, {512, 1, 256, 1},
+    {64, 1, 32, 1}};
 
 // Size buckets by pair count ni * nj.
 constexpr int kBuckets = 3;
@@ -238,6 +248,7 @@ int pairwise_call(int kernel, int mode, size_t ni, const T* const* ip, size_t nj
     a.ni = static_cast<int>(ni);
     a.nj = static_cast<int>(nj);
     a.eta = eta;
+    a.exact = mode == TUPAN_MODE_IEEE64 ? 1 : 0;
     if (kernel == K_PNACC) {
         if (!pn) return set_error(TUPAN_ERR_ARG, "PNAcc needs the PN order and 1/c^k");
         // pn_terms.h:503-548: order 2 and 3 give 1PN, 4 adds 2PN, 5 2.5PN,

@@ -609,13 +609,14 @@ int sakura_call_ex(size_t ni, const double* const* ip, size_t nj, const double* 
                                           static_cast<double>(kTwoPassPairs));
     if (opt.net || opt.log.target) two_pass = true;
     // the far-pair pass has its own geometry (kFarWpt i per thread)
-    Launch L = plan_launch(a.ni, a.nj, BLOCK, two_pass ? kFarWpt : 1, TILE, 2 * dev.target_wg, 64);
+    Launch L = plan_launch(a.ni, a.nj, BLOCK, two_pass ? kFarWpt : 1, TILE, 2 * dev.target_wg, 64,
+                           TILE);
     size_t nent = static_cast<size_t>(L.nsplit) * ni;
     if (two_pass && nent * kNearSlots > static_cast<size_t>(INT32_MAX)) {
         if (opt.path == 2 || opt.net || opt.log.target)
             return set_error(TUPAN_ERR_ARG, "too many particles for the two-pass Sakura form");
         two_pass = false;
-        L = plan_launch(a.ni, a.nj, BLOCK, 1, TILE, 2 * dev.target_wg, 64);
+        L = plan_launch(a.ni, a.nj, BLOCK, 1, TILE, 2 * dev.target_wg, 64, TILE);
     }
     a.nsplit = L.nsplit;
     a.jchunk = L.jchunk;

@@ -220,14 +220,18 @@ class DeviceSteps(object):
         self.n = ps.n
         self.scratch = torch.empty(32 * max(self.n, 1), dtype=torch.float64,
                                    device=ps.mass.device)
-        self.ctl = torch.zeros(4, dtype=torch.float64, device=ps.mass.device)
+        self.ctl = torch.zeros(5, dtype=torch.float64, device=ps.mass.device)
         ptrs = [ps.__dict__[r].data_ptr() if r in ps.__dict__ else None
                 for r in self.ROLES]
         self.arrays = _native.ptr_array(ptrs)
 
+    STATUS = {1: "non-finite time, t_end or eta",
+              2: "Tstep gave no positive finite time step",
+              3: "no power-of-two block step aligned with the time"}
+
     def set_control(self, t_end):
         self.ctl.copy_(torch.tensor([float(type(self.ps).t_curr), float(t_end),
-                                     float(self.integ.eta), 0.0],
+                                     float(self.integ.eta), 0.0, 0.0],
                                     dtype=torch.float64))
 
     def launch(self):
@@ -239,9 +243,14 @@ class DeviceSteps(object):
         self.native.check(rc, "tupan_hermite_step")
 
     def read_time(self):
-        t = float(self.ctl[0].item())
-        type(self.ps).t_curr = t
-        return t
+        """The device time (one synchronisation); raises if a step failed."""
+        c = self.ctl.cpu().tolist()
+        status = int(c[4])
+        if status:
+            raise self.native.TupanHipError(
+                "tupan_hermite_step: %s" % self.STATUS.get(status, status))
+        type(self.ps).t_curr = c[0]
+        return c[0]
 
     def capture(self, nsteps):
         """HIP graph of nsteps device steps (the arrays, scratch and control
@@ -301,12 +310,17 @@ class Hermite(Base):
         dev.launch()                      # eager first step: sizes workspaces
         replays = 0
         graph = None
-        while (dev.read_time() < t_end) if self.eta > 0 else \
-                (dev.read_time() > t_end):
+        t = dev.read_time()
+        while (t < t_end) if self.eta > 0 else (t > t_end):
             if graph is None:
                 graph = dev.capture(steps_per_graph)
             graph.replay()
             replays += 1
+            t_new = dev.read_time()
+            if t_new == t:
+                raise self.native.TupanHipError(
+                    "device Hermite steps made no progress at t = %r" % t)
+            t = t_new
         if self.reporter:
             self.reporter.diagnostic_report(self.ps)
         return replays

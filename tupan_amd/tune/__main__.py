@@ -77,7 +77,7 @@ def tune_pairwise(quick, names):
                 ext = cls("HIP", "float64", mode=mode)
                 args = (0.01,) if name == "tstep" else ()
                 results = []
-                for shape in range(5):
+                for shape in range(6):
                     for wg in (2, 4, 8):
                         _native.check(lib.tupan_tune_set(kid, SLOT[mode], b,
                                                          shape, wg),
