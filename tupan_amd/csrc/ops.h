@@ -888,7 +888,11 @@ struct OpNregV {
 // 65536-particle Plummer state a Newton step changes neither the median
 // (1.7e-8) nor the 99.9th percentile (2.8e-7) of the relative force error,
 // and costs three more instructions per pair.
-using OpSplitAcc = OpSplitAccT<false>;
-using OpSplitAccJerk = OpSplitAccJerkT<false>;
+#ifndef TUPAN_SPLIT_REFINE
+#define TUPAN_SPLIT_REFINE 0
+#endif
+constexpr bool kSplitRefine = TUPAN_SPLIT_REFINE != 0;
+using OpSplitAcc = OpSplitAccT<kSplitRefine>;
+using OpSplitAccJerk = OpSplitAccJerkT<kSplitRefine>;
 
 }  // namespace tupan
