@@ -25,10 +25,11 @@ static int run_pairwise(typename Op::Args a, hipStream_t stream, int wg_per_cu)
     const int target = (wg_per_cu > 0 ? wg_per_cu : 4) * dev.cu_count;
     // j-chunks of whole tiles, except for sets well above a tile in the
     // modes that sum in any order: there 32-particle chunks let a small
-    // system (N ~ 1e3) fill the chip with j-splits.  ieee64 keeps whole
-    // tiles, so any set of at most one tile is one split (the reference's
-    // summation order).
-    const int min_chunk = (!a.exact && a.nj > 4 * TILE) ? 32 : TILE;
+    // system (N ~ 1e3) fill the chip with j-splits.  ieee64 never cuts a
+    // j-range below 256 particles, so any set of at most 256 is one split
+    // (the reference's summation order) whatever shape is tuned.
+    const int min_chunk = a.exact ? (TILE > 256 ? TILE : 256)
+                                  : (a.nj > 4 * TILE ? 32 : TILE);
     Launch L = plan_launch(a.ni, a.nj, BLOCK, WPT, TILE, target, 64, min_chunk);
     a.nsplit = L.nsplit;
     a.jchunk = L.jchunk;
