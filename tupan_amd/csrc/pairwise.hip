@@ -58,7 +58,7 @@ using Launcher = int (*)(typename Op::Args, hipStream_t, int);
 // last, one wave per workgroup, is for small systems.
 #define TUPAN_LIGHT_SHAPES(Op)                                                   \
     {                                                                            \
-        &run_pairwise<Op, 256, 2, 256, 4>, &run_pairwise<Op, 256, 1, 256, 8>,    \
+        &run_pairwise<Op, 256, 2, 256, 4>, &run_pairwise<Op, 512, 2, 512, 2>,    \
             &run_pairwise<Op, 256, 4, 256, 2>, &run_pairwise<Op, 128, 2, 128, 4>, \
             &run_pairwise<Op, 512, 1, 512, 4>, &run_pairwise<Op, 64, 1, 64, 4>   \
     }
@@ -78,7 +78,7 @@ using Launcher = int (*)(typename Op::Args, hipStream_t, int);
     }
 constexpr int kShapes = 6;
 static const int kLightShape[kShapes][4] = {
-    {256, 2, 256, 4}, {256, 1, 256, 8}, {256, 4, 256, 2}, {128, 2, 128, 4}, {512, 1, 512, 4},
+    {256, 2, 256, 4}, {512, 2, 512, 2}, {256, 4, 256, 2}, {128, 2, 128, 4}, {512, 1, 512, 4},
     {64, 1, 64, 4}};
 static const int kHeavyShape[kShapes][4] = {
     {256, 1, 128, 2}, {128, 1, 128, 2}, {256, 2, 128, 1}, {64, 1, 64, 2}, {256, 1, 256, 4},

@@ -1,9 +1,9 @@
 """Autotuner: ``python -m tupan_amd.tune [--quick]`` on the target GPU.
 
 For every pairwise kernel and arithmetic mode it times each compiled tile
-shape at 2, 4 and 8 workgroups per CU on a uniform random cluster of one
+shape at 2, 4, 8 and 16 workgroups per CU on a uniform random cluster of one
 representative size per size bucket (1024, 16384 and 65536 particles, i =
-j), keeps the fastest (HIP-event kernel time, best of 3), and for the
+j), keeps the fastest (GPU time of the whole call, best of 3), and for the
 ensemble plan it times each lanes-per-system packing on the 4096-system
 workload.  The choices go to tupan_amd/tune/cache.json, which the library
 applies when it loads.
@@ -43,15 +43,20 @@ def cluster(n, seed=0):
 
 
 def time_call(fn, reps):
+    """Best of `reps` GPU times of fn(), from events on the launch stream
+    around the whole call (all-pairs kernel, split combine and the gaps
+    between their launches)."""
     fn()
     torch.cuda.synchronize()
     best = None
     for _ in range(reps):
-        _native.timing(True)
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        t0.record()
         fn()
-        ms, cnt = _native.timing_read()
-        _native.timing(False)
-        t = ms / max(cnt, 1)
+        t1.record()
+        t1.synchronize()
+        t = t0.elapsed_time(t1)
         best = t if best is None else min(best, t)
     return best
 
@@ -78,7 +83,7 @@ def tune_pairwise(quick, names):
                 args = (0.01,) if name == "tstep" else ()
                 results = []
                 for shape in range(6):
-                    for wg in (2, 4, 8):
+                    for wg in (2, 4, 8, 16):
                         _native.check(lib.tupan_tune_set(kid, SLOT[mode], b,
                                                          shape, wg),
                                       "tupan_tune_set")
