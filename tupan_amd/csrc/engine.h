@@ -98,22 +98,33 @@ __global__ __launch_bounds__(BLOCK) void pair_kernel(typename Op::Args a)
     }
 }
 
-// Combine the j-split partials of each i in split order and store.
+// Combine the j-split partials of each i in split order and store.  The
+// partials are loaded kBatch splits at a time before they are combined, so a
+// thread has kBatch * kNacc loads in flight instead of kNacc: for small ni
+// (a few waves) the kernel is bound by memory latency, not bandwidth.
 template <class Op>
 __global__ __launch_bounds__(256) void reduce_kernel(typename Op::Args a)
 {
+    constexpr int kBatch = 8;
     const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= a.ni) return;
     double tot[Op::kNacc];
     const size_t stride = static_cast<size_t>(Op::kNacc) * a.ni;
 #pragma unroll
     for (int k = 0; k < Op::kNacc; ++k) tot[k] = a.partial[static_cast<size_t>(k) * a.ni + i];
-    for (int s = 1; s < a.nsplit; ++s) {
-        double part[Op::kNacc];
+    for (int s0 = 1; s0 < a.nsplit; s0 += kBatch) {
+        double part[kBatch][Op::kNacc];
 #pragma unroll
-        for (int k = 0; k < Op::kNacc; ++k)
-            part[k] = a.partial[s * stride + static_cast<size_t>(k) * a.ni + i];
-        Op::combine(tot, part);
+        for (int b = 0; b < kBatch; ++b) {
+            if (s0 + b < a.nsplit) {
+                const double* src = a.partial + static_cast<size_t>(s0 + b) * stride + i;
+#pragma unroll
+                for (int k = 0; k < Op::kNacc; ++k) part[b][k] = src[static_cast<size_t>(k) * a.ni];
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < kBatch; ++b)
+            if (s0 + b < a.nsplit) Op::combine(tot, part[b]);
     }
     Op::store(a, i, tot);
 }

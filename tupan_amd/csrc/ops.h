@@ -485,13 +485,47 @@ struct OpTstep {
 // Every coordinate is taken relative to it in FP64 and split into hi + lo
 // floats; a pair separation is (hi_i - hi_j) + (lo_i - lo_j), accurate to
 // about one FP32 rounding of the separation itself, and exactly 0 for a
-// particle paired with itself, so the r2 > 0 mask is preserved.  Velocity
-// differences are plain FP32 (relative to the frame).  Accumulation: FP32
-// over kFlush j-particles, then into FP64.
+// particle paired with itself.  Velocity differences are plain FP32 (relative
+// to the frame).  Accumulation: FP32 over kFlush j-particles, then into FP64.
 // ---------------------------------------------------------------------------
 struct SplitFrame {
     double cx, cy, cz, cvx, cvy, cvz;
 };
+
+// Softened 1/R for the split interactions.  R^2 is formed as
+// fma(dz, dz, fma(dy, dy, fma(dx, dx, e2))), so e2 enters before any
+// rounding: rounding r^2 first and adding e2 afterwards rounds twice, and for
+// pairs whose e2 is below half an FP32 ulp of r^2 the second rounding drops e2
+// every time, which makes every far force too strong by ~1.5 e2 / R^2 on
+// average (+1.2e-8 of the virial on a Plummer sphere with eps = 1e-4) and
+// shows up as an energy error proportional to the change of the potential
+// energy.  The mask is R^2 > e2: a pair at zero separation (a particle with
+// itself, or two coincident particles) gives R^2 == e2 exactly and is
+// dropped, as the reference's r2 > 0 drops it; so is a pair whose r^2 is
+// below half an ulp of e2 (a separation under 2^-12 of the softening length),
+// which the reference keeps.
+//
+// The raw v_rsq_f32 estimate is within ~1.6 FP32 ulp but not unbiased: over
+// every float in [1, 4) (both exponent parities, weight 1/x) its mean relative
+// error on gfx950 is kRsqBias.  Accelerations scale as y^3, so the FP32
+// partial sums are multiplied by 1 / (1 + 3 kRsqBias) when they are flushed
+// into the FP64 sums; with the Newton-refined estimate (unbiased) the factor
+// is 1.
+constexpr double kRsqBias = -7.3055e-9;
+
+template <bool kRefine>
+__device__ __forceinline__ float split_rsqrt(float dx, float dy, float dz, float e2)
+{
+    const float R2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, e2)));
+    const float y = kRefine ? rsqrt_refined(R2) : __builtin_amdgcn_rsqf(R2);
+    return R2 > e2 ? y : 0.0f;
+}
+
+template <bool kRefine>
+constexpr double split_scale()
+{
+    return kRefine ? 1.0 : 1.0 / (1.0 + 3.0 * kRsqBias);
+}
 
 template <bool kRefine>
 struct OpSplitAccT {
@@ -536,10 +570,7 @@ struct OpSplitAccT {
         float dx = (r.xh - s.v[0]) + (r.xl - s.v[4]);
         float dy = (r.yh - s.v[1]) + (r.yl - s.v[5]);
         float dz = (r.zh - s.v[2]) + (r.zl - s.v[6]);
-        float e2 = r.e2 + s.v[7];
-        float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
-        float y = kRefine ? rsqrt_refined(r2 + e2) : __builtin_amdgcn_rsqf(r2 + e2);
-        y = r2 > 0.0f ? y : 0.0f;
+        const float y = split_rsqrt<kRefine>(dx, dy, dz, r.e2 + s.v[7]);
         float q3 = (y * y) * (y * s.v[3]);
         c.ax = __builtin_fmaf(-q3, dx, c.ax);
         c.ay = __builtin_fmaf(-q3, dy, c.ay);
@@ -547,9 +578,10 @@ struct OpSplitAccT {
     }
     __device__ static void flush(Acc& c)
     {
-        c.tx += static_cast<double>(c.ax);
-        c.ty += static_cast<double>(c.ay);
-        c.tz += static_cast<double>(c.az);
+        constexpr double sc = split_scale<kRefine>();
+        c.tx = __builtin_fma(static_cast<double>(c.ax), sc, c.tx);
+        c.ty = __builtin_fma(static_cast<double>(c.ay), sc, c.ty);
+        c.tz = __builtin_fma(static_cast<double>(c.az), sc, c.tz);
         c.ax = c.ay = c.az = 0.0f;
     }
     __device__ static double get(const Acc& c, int k) { return k == 0 ? c.tx : (k == 1 ? c.ty : c.tz); }
@@ -618,10 +650,8 @@ struct OpSplitAccJerkT {
         float ux = r.vx - s.v[8];
         float uy = r.vy - s.v[9];
         float uz = r.vz - s.v[10];
-        float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
         float rv = __builtin_fmaf(dz, uz, __builtin_fmaf(dy, uy, dx * ux));
-        float y = kRefine ? rsqrt_refined(r2 + e2) : __builtin_amdgcn_rsqf(r2 + e2);
-        y = r2 > 0.0f ? y : 0.0f;
+        const float y = split_rsqrt<kRefine>(dx, dy, dz, e2);
         float q2 = y * y;
         float q3 = q2 * (y * s.v[3]);
         float alpha = 3.0f * rv * q2;
@@ -639,7 +669,7 @@ struct OpSplitAccJerkT {
     {
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
-            c.t[k] += static_cast<double>(c.s[k]);
+            c.t[k] = __builtin_fma(static_cast<double>(c.s[k]), split_scale<kRefine>(), c.t[k]);
             c.s[k] = 0.0f;
         }
     }
@@ -884,10 +914,11 @@ struct OpNregV {
     }
 };
 
-// The split path uses the raw v_rsq_f32 estimate (within 1.6 ulp): on the
-// 65536-particle Plummer state a Newton step changes neither the median
-// (1.7e-8) nor the 99.9th percentile (2.8e-7) of the relative force error,
-// and costs three more instructions per pair.
+// The split path uses the raw v_rsq_f32 estimate (within 1.6 ulp) with its
+// mean bias compensated (split_scale): on the 65536-particle Plummer state a
+// Newton step changes neither the median (1.7e-8) nor the 99.9th percentile
+// (2.8e-7) of the relative force error and costs four more instructions per
+// pair.  TUPAN_SPLIT_REFINE=1 builds the refined form.
 #ifndef TUPAN_SPLIT_REFINE
 #define TUPAN_SPLIT_REFINE 0
 #endif

@@ -8,7 +8,8 @@ Hermite methods run as device steps replayed in HIP graphs of
 --graph-steps steps (--graph-steps 0: the host-driven driver); other
 methods use the ordinary driver.  The run advances in chunks of --chunk
 time units and stops at --t-end or after the first chunk that ends past
---max-wall seconds.  Prints one JSON line: the time reached, steps, wall
+--max-wall seconds; after every chunk it prints the time reached, |dE/E| and the kinetic and
+potential energies to stderr.  At the end it prints one JSON line: the time reached, steps, wall
 time, |dE/E| with E from FP64 Phi and exactly rounded sums, and the force
 mode.
 """
@@ -47,6 +48,8 @@ def main(argv=None):
                       device="cuda:0")
     type(ps).t_curr = 0.0
     e0 = energy(ps)
+    print("t = 0  K = %.15e  U = %.15e" % (ps.kinetic_energy, ps.potential_energy),
+          file=sys.stderr, flush=True)
     ctype.force_mode = args.mode
     extensions.install("HIP", "float64")
     hermite = "hermite" in args.method
@@ -63,7 +66,11 @@ def main(argv=None):
             while it.time < target:
                 it.evolve_step(target)
         t = it.time
-        print("t = %.6f  %.1f s" % (t, time.perf_counter() - t0),
+        # Phi is FP64 in every force mode, so this is the FP64 energy
+        q = it.particle_system
+        ke, pe = q.kinetic_energy, q.potential_energy
+        print("t = %.6f  %.1f s  |dE/E| = %.3e  K = %.15e  U = %.15e"
+              % (t, time.perf_counter() - t0, abs((ke + pe - e0) / e0), ke, pe),
               file=sys.stderr, flush=True)
         if args.max_wall and time.perf_counter() - t0 > args.max_wall:
             break
