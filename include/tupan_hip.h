@@ -80,13 +80,23 @@ int tupan_timing_read(double *total_ms, int *count);
  * split-FP32, 2 reference-order FP64, 3 float build.  bucket by pair count
  * ni*nj: 0 below 2^22, 1 below 2^30, 2 above.  Default: shape 0, 4
  * workgroups per CU.  The table is process-wide. */
-#define TUPAN_TUNE_SHAPES 5
+#define TUPAN_TUNE_SHAPES 6
 #define TUPAN_TUNE_BUCKETS 3
 int tupan_tune_set(int kernel, int slot, int bucket, int shape, int wg_per_cu);
 int tupan_tune_get(int kernel, int slot, int bucket, int *shape,
                    int *wg_per_cu);
 int tupan_tune_shape(int kernel, int shape, int *block, int *wpt, int *tile,
                      int *unroll);
+
+/* Calibration of the split-FP32 1/R (no counterpart in the reference).  The
+ * split kernels take 1/R from the hardware v_rsq_f32 estimate and divide
+ * their FP32 sums by 1 + 3 b, b the estimate's mean relative error, which is
+ * compiled in.  This measures b on the current device over every float in
+ * [1, 4) (weight 1/x): out[0], out[1] the raw estimate's mean error on
+ * [1, 2) and [2, 4), out[2], out[3] the same after one Newton step, out[4]
+ * the compiled-in b (used for both binades).  Allocates 1.5 MiB of device
+ * memory for the call. */
+int tupan_rsq_bias(double out[5]);
 
 /* ---------------------------------------------------------------------------
  * Host-pointer layer (reference ABI, FP64).  libtupan.h line numbers cite

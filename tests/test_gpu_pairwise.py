@@ -462,3 +462,44 @@ def test_large_random_rows_vs_oracle(mode):
     fx = float(np.sum(d["mass"] * got[0]))
     amag = float(np.sum(d["mass"] * np.abs(got[0])))
     assert abs(fx) <= (1e-12 if mode == "fp64" else 1e-6) * amag
+
+
+def test_split_combine_order():
+    """The j-split partial sums are combined in a fixed order.  With the
+    reference-order FP64 pair kernel each split's partial sum equals the
+    oracle's sum over that j-range bit for bit, so the GPU result can be
+    rebuilt on the host: for N = 1024 cut into 4 splits of 256 (shape
+    64x1x64x4; ieee64 never cuts below 256) the combine of small sets (8
+    lanes per i, lane totals added pairwise) gives (p0 + p1) + (p2 + p3),
+    which differs from the left-to-right sum in the last bit for some i."""
+    import ctypes as C
+    from oracle import Oracle
+    from tupan_amd import _native
+    from tupan_amd.lib import extensions
+    from tupan_amd.tune import KERNEL_IDS, SLOTS
+    lib = _native.lib()
+    kid, slot = KERNEL_IDS["acc_jerk"], SLOTS["ieee64"]
+    old_shape, old_wg = C.c_int(), C.c_int()
+    assert lib.tupan_tune_get(kid, slot, 0, C.byref(old_shape), C.byref(old_wg)) == 0
+    rng = np.random.default_rng(5)
+    n = 1024
+    d = {k: rng.random(n) for k in ("rx", "ry", "rz", "vx", "vy", "vz")}
+    d["mass"] = rng.uniform(0.5, 1.5, n) / n
+    d["eps2"] = np.full(n, 1e-6)
+    ps = device_ps(d)
+    try:
+        assert lib.tupan_tune_set(kid, slot, 0, 5, 16) == 0
+        res = extensions.AccJerk("HIP", "float64", mode="ieee64").calc(ps, ps)
+        got = [r.detach().cpu().numpy() for r in res]
+    finally:
+        lib.tupan_tune_set(kid, slot, 0, old_shape.value, old_wg.value)
+    orc = Oracle("float64")
+    parts = [orc.acc_jerk(d, {k: v[s:s + 256] for k, v in d.items()})
+             for s in range(0, n, 256)]
+    for k in range(6):
+        p = [q[k] for q in parts]
+        tree = (p[0] + p[1]) + (p[2] + p[3])
+        seq = ((p[0] + p[1]) + p[2]) + p[3]
+        assert np.array_equal(got[k], tree), k
+        print("component %d: tree and left-to-right differ for %d of %d"
+              % (k, int(np.sum(tree != seq)), n))

@@ -102,6 +102,20 @@ def test_split32_force_law_unbiased(plummer64k):
         assert abs(beta) <= 2e-9
 
 
+def test_rsq_bias_calibration():
+    """The bias the split kernels compensate is the device's: v_rsq_f32's
+    mean relative error over every float in [1, 4), averaged over the two
+    exponent parities, equals the compiled-in constant to 1e-11 (the
+    hardware estimate is deterministic), and one Newton step leaves
+    |mean| <= 1e-10."""
+    from tupan_amd import _native
+    raw0, raw1, ref0, ref1, b = _native.rsq_bias()
+    print("v_rsq_f32 mean error: %.4e %.4e, refined %.2e %.2e, compiled %.4e"
+          % (raw0, raw1, ref0, ref1, b))
+    assert abs(0.5 * (raw0 + raw1) - b) <= 1e-11
+    assert max(abs(ref0), abs(ref1)) <= 1e-10
+
+
 def test_split32_tstep_error_and_levels(plummer64k):
     """Split-FP32 Tstep against FP64 on the Plummer state: relative error
     median <= 1e-7, 99.9th percentile <= 1e-6, and the power-of-two level a

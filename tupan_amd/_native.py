@@ -55,6 +55,8 @@ def _declare(lib):
     IP = ctypes.POINTER(ctypes.c_int)
     lib.tupan_tune_get.restype = ctypes.c_int
     lib.tupan_tune_get.argtypes = [ctypes.c_int] * 3 + [IP, IP]
+    lib.tupan_rsq_bias.restype = ctypes.c_int
+    lib.tupan_rsq_bias.argtypes = [ctypes.POINTER(ctypes.c_double)]
     lib.tupan_tune_shape.restype = ctypes.c_int
     lib.tupan_tune_shape.argtypes = [ctypes.c_int] * 2 + [IP] * 4
     lib.tupan_pairwise.restype = ctypes.c_int
@@ -171,6 +173,14 @@ def device_info():
     return cu.value, wg.value
 
 
+def rsq_bias():
+    """tupan_rsq_bias: (raw mean error on [1, 2), on [2, 4), refined on
+    [1, 2), on [2, 4), compiled-in bias) of v_rsq_f32 on this device."""
+    out = (ctypes.c_double * 5)()
+    check(lib().tupan_rsq_bias(out), "tupan_rsq_bias")
+    return tuple(out)
+
+
 # Names exported by the shared libraries (used by tests that check the ABI).
 REFERENCE_ABI = ("phi_kernel", "acc_kernel", "acc_jerk_kernel",
                  "snap_crackle_kernel", "tstep_kernel", "sakura_kernel",
@@ -178,6 +188,7 @@ REFERENCE_ABI = ("phi_kernel", "acc_kernel", "acc_jerk_kernel",
                  "pnacc_kernel")
 DEVICE_ABI = ("tupan_last_error", "tupan_version", "tupan_device_info",
               "tupan_tune_set", "tupan_tune_get", "tupan_tune_shape",
+              "tupan_rsq_bias",
               "tupan_timing", "tupan_timing_read",
               "tupan_phi", "tupan_acc", "tupan_acc_jerk", "tupan_snap_crackle",
               "tupan_tstep", "tupan_pairwise", "tupan_pairwise_f32",

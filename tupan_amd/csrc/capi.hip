@@ -286,6 +286,12 @@ int tupan_tune_shape(int kernel, int shape, int* block, int* wpt, int* tile, int
     return tune_shape(kernel, shape, block, wpt, tile, unroll);
 }
 
+int tupan_rsq_bias(double* out)
+{
+    if (!out) return set_error(TUPAN_ERR_ARG, "null output");
+    return rsq_bias_probe(out);
+}
+
 int tupan_device_info(int* cu_count, int* target_workgroups)
 {
     int n = 0;

@@ -129,6 +129,72 @@ __global__ __launch_bounds__(256) void reduce_kernel(typename Op::Args a)
     Op::store(a, i, tot);
 }
 
+// The same combine for small ni, where one thread per i leaves most of the
+// chip idle and waits on memory latency once per batch: T consecutive lanes
+// share an i, lane t combining splits [t c, (t + 1) c) in order (c = ceil(
+// nsplit / T)), then the T lane totals are combined pairwise in a fixed tree
+// (lanes 2k and 2k + 1, then 4k and 4k + 2, ...), so the result is the same
+// on every run.
+template <class Op, int T>
+__global__ __launch_bounds__(256) void reduce_wide_kernel(typename Op::Args a)
+{
+    constexpr int kBatch = 8;
+    static_assert(T > 1 && (T & (T - 1)) == 0 && T <= 64, "T must be a power of two <= 64");
+    const int gid = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const int i = gid / T;
+    const int t = gid % T;
+    const bool valid = i < a.ni;
+    const int c = (a.nsplit + T - 1) / T;
+    const int sb = t * c;
+    const int se = min(a.nsplit, sb + c);
+    const size_t stride = static_cast<size_t>(Op::kNacc) * a.ni;
+    double tot[Op::kNacc];
+#pragma unroll
+    for (int k = 0; k < Op::kNacc; ++k) tot[k] = 0.0;
+    int have = 0;
+    for (int s0 = sb; valid && s0 < se; s0 += kBatch) {
+        double part[kBatch][Op::kNacc];
+#pragma unroll
+        for (int b = 0; b < kBatch; ++b) {
+            if (s0 + b < se) {
+                const double* src = a.partial + static_cast<size_t>(s0 + b) * stride + i;
+#pragma unroll
+                for (int k = 0; k < Op::kNacc; ++k) part[b][k] = src[static_cast<size_t>(k) * a.ni];
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < kBatch; ++b) {
+            if (s0 + b < se) {
+                if (have) {
+                    Op::combine(tot, part[b]);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < Op::kNacc; ++k) tot[k] = part[b][k];
+                    have = 1;
+                }
+            }
+        }
+    }
+    // every lane of the wave takes part in the exchanges
+#pragma unroll
+    for (int off = 1; off < T; off <<= 1) {
+        double other[Op::kNacc];
+#pragma unroll
+        for (int k = 0; k < Op::kNacc; ++k) other[k] = __shfl_xor(tot[k], off);
+        const int ohave = __shfl_xor(have, off);
+        if ((t & (2 * off - 1)) == 0 && ohave) {
+            if (have) {
+                Op::combine(tot, other);
+            } else {
+#pragma unroll
+                for (int k = 0; k < Op::kNacc; ++k) tot[k] = other[k];
+                have = 1;
+            }
+        }
+    }
+    if (valid && t == 0) Op::store(a, i, tot);
+}
+
 // Launch geometry of one all-pairs call.
 struct Launch {
     int nblock_i;  // i blocks

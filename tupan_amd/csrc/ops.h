@@ -511,7 +511,7 @@ struct SplitFrame {
 // partial sums are multiplied by 1 / (1 + 3 kRsqBias) when they are flushed
 // into the FP64 sums; with the Newton-refined estimate (unbiased) the factor
 // is 1.
-constexpr double kRsqBias = -7.3055e-9;
+constexpr double kRsqBias = -7.35832e-9;
 
 template <bool kRefine>
 __device__ __forceinline__ float split_rsqrt(float dx, float dy, float dz, float e2)

@@ -3,6 +3,8 @@
 //
 // This file is compiled with -ffp-contract=off: every fused multiply-add in
 // ops.h is explicit.
+#include <vector>
+
 #include "ops.h"
 #include "pn.h"
 #include "tupan_internal.h"
@@ -17,6 +19,9 @@ namespace tupan {
 // step).  A run-time table, filled from the autotuner's cache, picks the
 // shape and the workgroups-per-CU target per kernel, mode and size bucket.
 // ---------------------------------------------------------------------------
+// Below this many i-particles the j-split combine uses 8 lanes per i.
+constexpr int kWideReduceMaxNi = 32768;
+
 template <class Op, int BLOCK, int WPT, int TILE, int UNROLL>
 static int run_pairwise(typename Op::Args a, hipStream_t stream, int wg_per_cu)
 {
@@ -45,8 +50,13 @@ static int run_pairwise(typename Op::Args a, hipStream_t stream, int wg_per_cu)
                        stream, a);
     timing_end(stream);
     if (L.nsplit > 1) {
-        const unsigned rg = static_cast<unsigned>((a.ni + 255) / 256);
-        hipLaunchKernelGGL((reduce_kernel<Op>), dim3(rg), dim3(256), 0, stream, a);
+        if (a.ni <= kWideReduceMaxNi) {
+            const unsigned rg = static_cast<unsigned>((static_cast<size_t>(a.ni) * 8 + 255) / 256);
+            hipLaunchKernelGGL((reduce_wide_kernel<Op, 8>), dim3(rg), dim3(256), 0, stream, a);
+        } else {
+            const unsigned rg = static_cast<unsigned>((a.ni + 255) / 256);
+            hipLaunchKernelGGL((reduce_kernel<Op>), dim3(rg), dim3(256), 0, stream, a);
+        }
     }
     return check_launch("pairwise");
 }
@@ -267,6 +277,63 @@ int pairwise_call(int kernel, int mode, size_t ni, const T* const* ip, size_t nj
         kernel != K_TSTEP && kernel != K_PNACC)
         mode = TUPAN_MODE_FP64;  // split FP32 exists for the forces, Tstep and PNAcc
     return dispatch<T>(kernel, mode, a, static_cast<hipStream_t>(stream));
+}
+
+// Mean relative error of v_rsq_f32, raw and after the Newton step the split
+// kernels can build with, over every float in [1, 4) (2^24 inputs, both
+// exponent parities, weight 1/x).  Each 256-thread block lies in one binade
+// and writes its weighted sums; the host adds the blocks in order.
+__global__ __launch_bounds__(256) void rsq_bias_kernel(double* part)
+{
+    __shared__ double sh[3][256];
+    const int t = static_cast<int>(threadIdx.x);
+    const unsigned k = blockIdx.x * 256u + static_cast<unsigned>(t);
+    const float x = __uint_as_float(0x3f800000u + k);
+    const double exact = 1.0 / __builtin_sqrt(static_cast<double>(x));
+    const double w = 1.0 / static_cast<double>(x);
+    const float raw = __builtin_amdgcn_rsqf(x);
+    const float ref = rsqrt_refined(x);
+    sh[0][t] = w * ((static_cast<double>(raw) - exact) / exact);
+    sh[1][t] = w * ((static_cast<double>(ref) - exact) / exact);
+    sh[2][t] = w;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) {
+            for (int c = 0; c < 3; ++c) sh[c][t] += sh[c][t + h];
+        }
+        __syncthreads();
+    }
+    if (t < 3) part[3 * blockIdx.x + t] = sh[t][0];
+}
+
+int rsq_bias_probe(double* out)
+{
+    constexpr int kBlocks = (1 << 24) / 256;
+    double* d = nullptr;
+    if (hipMalloc(&d, sizeof(double) * 3 * kBlocks) != hipSuccess) {
+        (void)hipGetLastError();
+        return set_error(TUPAN_ERR_ALLOC, "rsq probe allocation failed");
+    }
+    hipLaunchKernelGGL(rsq_bias_kernel, dim3(kBlocks), dim3(256), 0, nullptr, d);
+    int rc = check_launch("rsq_bias_kernel");
+    std::vector<double> h(3 * kBlocks);
+    if (rc == TUPAN_OK)
+        rc = check_hip(hipMemcpy(h.data(), d, sizeof(double) * h.size(), hipMemcpyDeviceToHost),
+                       "rsq probe copy");
+    (void)hipFree(d);
+    if (rc != TUPAN_OK) return rc;
+    for (int b = 0; b < 2; ++b) {
+        double raw = 0, ref = 0, w = 0;
+        for (int blk = b * kBlocks / 2; blk < (b + 1) * kBlocks / 2; ++blk) {
+            raw += h[3 * blk];
+            ref += h[3 * blk + 1];
+            w += h[3 * blk + 2];
+        }
+        out[b] = raw / w;
+        out[2 + b] = ref / w;
+    }
+    out[4] = kRsqBias;
+    return 0;
 }
 
 template int pairwise_call<double>(int, int, size_t, const double* const*, size_t,

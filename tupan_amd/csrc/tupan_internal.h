@@ -149,4 +149,6 @@ int kepler_pairs_call(size_t npairs, const int* idx, const double* m, double* rx
                       double* rz, const double* e2, double* vx, double* vy, double* vz,
                       double dt, int* status, void* stream);
 
+int rsq_bias_probe(double* out);
+
 }  // namespace tupan
