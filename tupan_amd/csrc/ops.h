@@ -916,9 +916,10 @@ struct OpNregV {
 
 // The split path uses the raw v_rsq_f32 estimate (within 1.6 ulp) with its
 // mean bias compensated (split_scale): on the 65536-particle Plummer state a
-// Newton step changes neither the median (1.7e-8) nor the 99.9th percentile
-// (2.8e-7) of the relative force error and costs four more instructions per
-// pair.  TUPAN_SPLIT_REFINE=1 builds the refined form.
+// Newton step leaves the median and the 99.9th percentile of the relative
+// force error where they are (other FP32 roundings dominate them) and costs
+// four more instructions per pair.  TUPAN_SPLIT_REFINE=1 builds the refined
+// form.
 #ifndef TUPAN_SPLIT_REFINE
 #define TUPAN_SPLIT_REFINE 0
 #endif
