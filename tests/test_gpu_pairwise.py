@@ -503,3 +503,27 @@ def test_split_combine_order():
         assert np.array_equal(got[k], tree), k
         print("component %d: tree and left-to-right differ for %d of %d"
               % (k, int(np.sum(tree != seq)), n))
+
+
+@pytest.mark.parametrize("name", ["phi", "acc", "acc_jerk", "tstep"])
+def test_ieee64_uniform_softening_bitexact(name):
+    """Every particle with the same nonzero eps2: the tile engine forms the
+    pair softening once per tile (e2 + e2) instead of per pair; the result
+    is the reference's bit for bit (one j-split, reference order), with
+    i and j sets of different sizes and a duplicated particle."""
+    from oracle import Oracle
+    rng = np.random.default_rng(17)
+    d = {}
+    for n, tag in ((203, "i"), (241, "j")):
+        x = {k: rng.normal(size=n) for k in ("rx", "ry", "rz", "vx", "vy", "vz")}
+        x["mass"] = rng.uniform(0.5, 2.0, n) / n
+        x["eps2"] = np.full(n, 3.7e-5)
+        d[tag] = x
+    for k in d["i"]:
+        d["j"][k][5] = d["i"][k][9]          # one particle in both sets
+    ii, jj = device_ps(d["i"]), device_ps(d["j"])
+    sc = KERNELS[name][1]
+    got = run_ext(name, "float64", "ieee64", ii, jj, sc)
+    ref = getattr(Oracle("float64"), name)(d["i"], d["j"], *sc)
+    for o, r, e in zip(KERNELS[name][2], got, ref):
+        assert np.array_equal(r, e), (name, o, np.max(np.abs(r - e)))

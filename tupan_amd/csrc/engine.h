@@ -25,9 +25,69 @@
 //   store(a, i, tot)            final outputs from combined accumulators
 #pragma once
 
+#include <type_traits>
+#include <utility>
+
 #include "common.h"
 
 namespace tupan {
+
+// Ops that declare kUniformE2 take the pair softening as an argument on
+// tiles where it is the same for every pair (see pair_kernel).
+template <class Op, class = void>
+struct HasUniformE2 : std::false_type {};
+template <class Op>
+struct HasUniformE2<Op, std::void_t<decltype(Op::kUniformE2)>>
+    : std::integral_constant<bool, Op::kUniformE2> {};
+template <class Op, bool = HasUniformE2<Op>::value>
+struct SoftType {
+    using type = double;
+};
+template <class Op>
+struct SoftType<Op, true> {
+    using type = decltype(Op::slot_e2(std::declval<const typename Op::Slot&>()));
+};
+
+// One tile's interactions: UNROLL slots at a time, each applied to the WPT
+// i-particles, accumulators flushed every CH slots.
+template <class Op, int WPT, int UNROLL, int CH, bool kU, class E2>
+__device__ __forceinline__ void tile_pass(const typename Op::Slot* tile, int cnt,
+                                          const typename Op::IReg (&ir)[WPT],
+                                          typename Op::Acc (&acc)[WPT], E2 e2u)
+{
+    using Slot = typename Op::Slot;
+    for (int k0 = 0; k0 < cnt; k0 += CH) {
+        const int kend = min(cnt, k0 + CH);
+        int k = k0;
+        for (; k + UNROLL <= kend; k += UNROLL) {
+#pragma unroll
+            for (int u = 0; u < UNROLL; ++u) {
+                const Slot s = tile[k + u];
+#pragma unroll
+                for (int w = 0; w < WPT; ++w) {
+                    if constexpr (kU) {
+                        Op::template interact<true>(ir[w], s, acc[w], e2u);
+                    } else {
+                        Op::interact(ir[w], s, acc[w]);
+                    }
+                }
+            }
+        }
+        for (; k < kend; ++k) {
+            const Slot s = tile[k];
+#pragma unroll
+            for (int w = 0; w < WPT; ++w) {
+                if constexpr (kU) {
+                    Op::template interact<true>(ir[w], s, acc[w], e2u);
+                } else {
+                    Op::interact(ir[w], s, acc[w]);
+                }
+            }
+        }
+#pragma unroll
+        for (int w = 0; w < WPT; ++w) Op::flush(acc[w]);
+    }
+}
 
 template <class Op, int BLOCK, int WPT, int TILE, int UNROLL>
 __global__ __launch_bounds__(BLOCK) void pair_kernel(typename Op::Args a)
@@ -35,6 +95,7 @@ __global__ __launch_bounds__(BLOCK) void pair_kernel(typename Op::Args a)
     using Slot = typename Op::Slot;
     using IReg = typename Op::IReg;
     using Acc = typename Op::Acc;
+    constexpr bool kUni = HasUniformE2<Op>::value;
     __shared__ Slot tile[TILE];
 
     const GridPos g = grid_pos(a.nsplit);
@@ -51,6 +112,21 @@ __global__ __launch_bounds__(BLOCK) void pair_kernel(typename Op::Args a)
         Op::zero(acc[w]);
     }
 
+    // Uniform softening: e2 of the block's first i.  A tile takes the
+    // uniform path when every i of the workgroup and every j of the tile
+    // carry exactly that value; the pair value E + E is then formed once,
+    // the same number the per-pair sum gives, so results do not change.
+    bool iuni = true;
+    [[maybe_unused]] typename SoftType<Op>::type e2_first{};
+    if constexpr (kUni) {
+        e2_first = Op::e2_of_i(a, ibase);
+#pragma unroll
+        for (int w = 0; w < WPT; ++w) {
+            const int i = ibase + w * BLOCK + tid;
+            iuni = iuni && (i >= a.ni || Op::e2_of_i(a, i) == e2_first);
+        }
+    }
+
     const int j0 = g.jsplit * a.jchunk;
     const int j1 = min(a.nj, j0 + a.jchunk);
     constexpr int CH = Op::kFlush < TILE ? Op::kFlush : TILE;
@@ -58,26 +134,20 @@ __global__ __launch_bounds__(BLOCK) void pair_kernel(typename Op::Args a)
     for (int jt = j0; jt < j1; jt += TILE) {
         const int cnt = min(TILE, j1 - jt);
         __syncthreads();
-        for (int k = tid; k < cnt; k += BLOCK) Op::load_j(a, fr, jt + k, tile[k]);
-        __syncthreads();
-        for (int k0 = 0; k0 < cnt; k0 += CH) {
-            const int kend = min(cnt, k0 + CH);
-            int k = k0;
-            for (; k + UNROLL <= kend; k += UNROLL) {
-#pragma unroll
-                for (int u = 0; u < UNROLL; ++u) {
-                    const Slot s = tile[k + u];
-#pragma unroll
-                    for (int w = 0; w < WPT; ++w) Op::interact(ir[w], s, acc[w]);
-                }
+        bool tuni = iuni;
+        for (int k = tid; k < cnt; k += BLOCK) {
+            Op::load_j(a, fr, jt + k, tile[k]);
+            if constexpr (kUni) tuni = tuni && Op::slot_e2(tile[k]) == e2_first;
+        }
+        if constexpr (kUni) {
+            if (__syncthreads_and(tuni)) {
+                tile_pass<Op, WPT, UNROLL, CH, true>(tile, cnt, ir, acc, e2_first + e2_first);
+            } else {
+                tile_pass<Op, WPT, UNROLL, CH, false>(tile, cnt, ir, acc, e2_first);
             }
-            for (; k < kend; ++k) {
-                const Slot s = tile[k];
-#pragma unroll
-                for (int w = 0; w < WPT; ++w) Op::interact(ir[w], s, acc[w]);
-            }
-#pragma unroll
-            for (int w = 0; w < WPT; ++w) Op::flush(acc[w]);
+        } else {
+            __syncthreads();
+            tile_pass<Op, WPT, UNROLL, CH, false>(tile, cnt, ir, acc, 0);
         }
     }
 

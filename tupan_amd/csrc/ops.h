@@ -150,6 +150,11 @@ struct OpPhi {
     struct Acc { T phi; };
     static constexpr int kNacc = 1;
     static constexpr int kFlush = 1 << 30;
+    // uniform softening: when every i of the workgroup and every j of a tile
+    // carry the same e2, the engine passes the pair value e2 + e2 once
+    static constexpr bool kUniformE2 = true;
+    __device__ static T e2_of_i(const Args& a, int i) { return a.ip[4][i]; }
+    __device__ static T slot_e2(const Slot& s) { return s.v[4]; }
 
     __device__ static Frame frame(const Args&, int) { return {}; }
     __device__ static void load_i(const Args& a, const Frame&, int i, IReg& r)
@@ -162,12 +167,14 @@ struct OpPhi {
         for (int f = 0; f < 5; ++f) s.v[f] = a.jp[f][j];
     }
     __device__ static void zero(Acc& c) { c.phi = 0; }
-    __device__ static __forceinline__ void interact(const IReg& r, const Slot& s, Acc& c)
+    template <bool kU = false>
+    __device__ static __forceinline__ void interact(const IReg& r, const Slot& s, Acc& c,
+                                                    T e2u = T(0))
     {
         T dx = r.x - s.v[1];
         T dy = r.y - s.v[2];
         T dz = r.z - s.v[3];
-        T e2 = r.e2 + s.v[4];
+        T e2 = kU ? e2u : r.e2 + s.v[4];
         T r2 = dot3<P>(dx, dy, dz, dx, dy, dz);
         T q1, q2;
         P::q12(r2, e2, r2 > T(0), q1, q2);
@@ -195,6 +202,11 @@ struct OpAcc {
     struct Acc { T ax, ay, az; };
     static constexpr int kNacc = 3;
     static constexpr int kFlush = 1 << 30;
+    // uniform softening: when every i of the workgroup and every j of a tile
+    // carry the same e2, the engine passes the pair value e2 + e2 once
+    static constexpr bool kUniformE2 = true;
+    __device__ static T e2_of_i(const Args& a, int i) { return a.ip[4][i]; }
+    __device__ static T slot_e2(const Slot& s) { return s.v[4]; }
 
     __device__ static Frame frame(const Args&, int) { return {}; }
     __device__ static void load_i(const Args& a, const Frame&, int i, IReg& r)
@@ -207,12 +219,14 @@ struct OpAcc {
         for (int f = 0; f < 5; ++f) s.v[f] = a.jp[f][j];
     }
     __device__ static void zero(Acc& c) { c.ax = c.ay = c.az = 0; }
-    __device__ static __forceinline__ void interact(const IReg& r, const Slot& s, Acc& c)
+    template <bool kU = false>
+    __device__ static __forceinline__ void interact(const IReg& r, const Slot& s, Acc& c,
+                                                    T e2u = T(0))
     {
         T dx = r.x - s.v[1];
         T dy = r.y - s.v[2];
         T dz = r.z - s.v[3];
-        T e2 = r.e2 + s.v[4];
+        T e2 = kU ? e2u : r.e2 + s.v[4];
         T r2 = dot3<P>(dx, dy, dz, dx, dy, dz);
         T q1, q2;
         P::q12(r2, e2, r2 > T(0), q1, q2);
@@ -251,6 +265,11 @@ struct OpAccJerk {
     struct Acc { T ax, ay, az, jx, jy, jz; };
     static constexpr int kNacc = 6;
     static constexpr int kFlush = 1 << 30;
+    // uniform softening: when every i of the workgroup and every j of a tile
+    // carry the same e2, the engine passes the pair value e2 + e2 once
+    static constexpr bool kUniformE2 = true;
+    __device__ static T e2_of_i(const Args& a, int i) { return a.ip[4][i]; }
+    __device__ static T slot_e2(const Slot& s) { return s.v[4]; }
 
     __device__ static Frame frame(const Args&, int) { return {}; }
     __device__ static void load_i(const Args& a, const Frame&, int i, IReg& r)
@@ -264,12 +283,14 @@ struct OpAccJerk {
         for (int f = 0; f < 8; ++f) s.v[f] = a.jp[f][j];
     }
     __device__ static void zero(Acc& c) { c.ax = c.ay = c.az = c.jx = c.jy = c.jz = 0; }
-    __device__ static __forceinline__ void interact(const IReg& r, const Slot& s, Acc& c)
+    template <bool kU = false>
+    __device__ static __forceinline__ void interact(const IReg& r, const Slot& s, Acc& c,
+                                                    T e2u = T(0))
     {
         T dx = r.x - s.v[1];
         T dy = r.y - s.v[2];
         T dz = r.z - s.v[3];
-        T e2 = r.e2 + s.v[4];
+        T e2 = kU ? e2u : r.e2 + s.v[4];
         T ux = r.vx - s.v[5];
         T uy = r.vy - s.v[6];
         T uz = r.vz - s.v[7];
@@ -419,6 +440,11 @@ struct OpTstep {
     struct Acc { T sum, mx; };
     static constexpr int kNacc = 2;
     static constexpr int kFlush = 1 << 30;
+    // uniform softening: when every i of the workgroup and every j of a tile
+    // carry the same e2, the engine passes the pair value e2 + e2 once
+    static constexpr bool kUniformE2 = true;
+    __device__ static T e2_of_i(const Args& a, int i) { return a.ip[4][i]; }
+    __device__ static T slot_e2(const Slot& s) { return s.v[4]; }
 
     __device__ static Frame frame(const Args&, int) { return {}; }
     __device__ static void load_i(const Args& a, const Frame&, int i, IReg& r)
@@ -433,12 +459,14 @@ struct OpTstep {
         for (int f = 0; f < 8; ++f) s.v[f] = a.jp[f][j];
     }
     __device__ static void zero(Acc& c) { c.sum = 0; c.mx = 0; }
-    __device__ static __forceinline__ void interact(const IReg& r, const Slot& s, Acc& c)
+    template <bool kU = false>
+    __device__ static __forceinline__ void interact(const IReg& r, const Slot& s, Acc& c,
+                                                    T e2u = T(0))
     {
         T dx = r.x - s.v[1];
         T dy = r.y - s.v[2];
         T dz = r.z - s.v[3];
-        T e2 = r.e2 + s.v[4];
+        T e2 = kU ? e2u : r.e2 + s.v[4];
         T ux = r.vx - s.v[5];
         T uy = r.vy - s.v[6];
         T uz = r.vz - s.v[7];
@@ -537,6 +565,11 @@ struct OpSplitAccT {
     struct Acc { float ax, ay, az; double tx, ty, tz; };
     static constexpr int kNacc = 3;
     static constexpr int kFlush = 64;
+    // uniform softening: when every i of the workgroup and every j of a tile
+    // carry the same e2, the engine passes the pair value e2 + e2 once
+    static constexpr bool kUniformE2 = true;
+    __device__ static float e2_of_i(const Args& a, int i) { return static_cast<float>(a.ip[4][i]); }
+    __device__ static float slot_e2(const Slot& s) { return s.v[7]; }
 
     __device__ static Frame frame(const Args& a, int ibase)
     {
@@ -565,12 +598,14 @@ struct OpSplitAccT {
         c.ax = c.ay = c.az = 0.0f;
         c.tx = c.ty = c.tz = 0.0;
     }
-    __device__ static __forceinline__ void interact(const IReg& r, const Slot& s, Acc& c)
+    template <bool kU = false>
+    __device__ static __forceinline__ void interact(const IReg& r, const Slot& s, Acc& c,
+                                                    float e2u = float(0))
     {
         float dx = (r.xh - s.v[0]) + (r.xl - s.v[4]);
         float dy = (r.yh - s.v[1]) + (r.yl - s.v[5]);
         float dz = (r.zh - s.v[2]) + (r.zl - s.v[6]);
-        const float y = split_rsqrt<kRefine>(dx, dy, dz, r.e2 + s.v[7]);
+        const float y = split_rsqrt<kRefine>(dx, dy, dz, kU ? e2u : r.e2 + s.v[7]);
         float q3 = (y * y) * (y * s.v[3]);
         c.ax = __builtin_fmaf(-q3, dx, c.ax);
         c.ay = __builtin_fmaf(-q3, dy, c.ay);
@@ -606,6 +641,11 @@ struct OpSplitAccJerkT {
     struct Acc { float s[6]; double t[6]; };
     static constexpr int kNacc = 6;
     static constexpr int kFlush = 64;
+    // uniform softening: when every i of the workgroup and every j of a tile
+    // carry the same e2, the engine passes the pair value e2 + e2 once
+    static constexpr bool kUniformE2 = true;
+    __device__ static float e2_of_i(const Args& a, int i) { return static_cast<float>(a.ip[4][i]); }
+    __device__ static float slot_e2(const Slot& s) { return s.v[7]; }
 
     __device__ static Frame frame(const Args& a, int ibase)
     {
@@ -641,12 +681,14 @@ struct OpSplitAccJerkT {
 #pragma unroll
         for (int k = 0; k < 6; ++k) { c.s[k] = 0.0f; c.t[k] = 0.0; }
     }
-    __device__ static __forceinline__ void interact(const IReg& r, const Slot& s, Acc& c)
+    template <bool kU = false>
+    __device__ static __forceinline__ void interact(const IReg& r, const Slot& s, Acc& c,
+                                                    float e2u = float(0))
     {
         float dx = (r.xh - s.v[0]) + (r.xl - s.v[4]);
         float dy = (r.yh - s.v[1]) + (r.yl - s.v[5]);
         float dz = (r.zh - s.v[2]) + (r.zl - s.v[6]);
-        float e2 = r.e2 + s.v[7];
+        float e2 = kU ? e2u : r.e2 + s.v[7];
         float ux = r.vx - s.v[8];
         float uy = r.vy - s.v[9];
         float uz = r.vz - s.v[10];
@@ -701,6 +743,11 @@ struct OpSplitTstep {
     struct Acc { float sum, mx; double tsum; };
     static constexpr int kNacc = 2;
     static constexpr int kFlush = 64;
+    // uniform softening: when every i of the workgroup and every j of a tile
+    // carry the same e2, the engine passes the pair value e2 + e2 once
+    static constexpr bool kUniformE2 = true;
+    __device__ static float e2_of_i(const Args& a, int i) { return static_cast<float>(a.ip[4][i]); }
+    __device__ static float slot_e2(const Slot& s) { return s.v[7]; }
 
     __device__ static Frame frame(const Args& a, int ibase)
     {
@@ -728,12 +775,14 @@ struct OpSplitTstep {
         c.mx = 0.0f;
         c.tsum = 0.0;
     }
-    __device__ static __forceinline__ void interact(const IReg& r, const Slot& s, Acc& c)
+    template <bool kU = false>
+    __device__ static __forceinline__ void interact(const IReg& r, const Slot& s, Acc& c,
+                                                    float e2u = float(0))
     {
         float dx = (r.xh - s.v[0]) + (r.xl - s.v[4]);
         float dy = (r.yh - s.v[1]) + (r.yl - s.v[5]);
         float dz = (r.zh - s.v[2]) + (r.zl - s.v[6]);
-        float e2 = r.e2 + s.v[7];
+        float e2 = kU ? e2u : r.e2 + s.v[7];
         float ux = r.vx - s.v[8];
         float uy = r.vy - s.v[9];
         float uz = r.vz - s.v[10];
