@@ -381,6 +381,20 @@ int tupan_sakura_ex(size_t ni, const double *const *ifields, size_t nj,
                     const tupan_solve_log *log, unsigned long long *counters,
                     int *near_pairs, int *status, void *stream);
 
+/* Sakura pair drift for HIP graph capture (no counterpart in the
+ * reference).  The two-pass form of tupan_sakura_ex without its one host
+ * synchronisation: the near-pair list gets `capacity` entries reserved, the
+ * near passes read the listed count on the device, and a call with more near
+ * pairs than `capacity` sets bit 2 of *status (required, device int) and
+ * leaves its outputs invalid, so the caller checks status after each replay
+ * and retries with a larger capacity.  Make one call with the same sizes and
+ * capacity outside capture first (it sizes the library's workspaces; a
+ * capture must not grow them).  counters as for tupan_sakura. */
+int tupan_sakura_graph(size_t ni, const double *const *ifields, size_t nj,
+                       const double *const *jfields, double dt, long flag,
+                       double *const *out, int capacity,
+                       unsigned long long *counters, int *status, void *stream);
+
 /* Hierarchical SIA step of a small subset on the device (no counterpart in
  * the reference, whose driver visits every level from the host:
  * tupan/integrator/sia.py:1089-1123, 'h' variants).  Runs recurse(S, tau)

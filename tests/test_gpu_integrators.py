@@ -272,3 +272,49 @@ def test_device_hermite_bad_control_raises():
     for k in ("rx", "vx", "time", "nstep"):
         np.testing.assert_array_equal(q[k], host[k] if k in host else q[k])
     assert int(q["nstep"].max()) == 0
+
+
+def _sakura_run(host, t_end, mode, capacity=None):
+    """Fixed-step sakura (eta 1/64) to t_end: 'host' = the host driver,
+    'eager' = device steps one by one, 'graph' = device steps replayed as
+    graphs of 8.  capacity, if given, replaces the probed near-pair
+    capacity (to force list overflows)."""
+    from tupan_amd.integrator import Integrator
+    from tupan_amd.particles import ParticleSystem
+    install_hip("float64")
+    ps = ParticleSystem.from_numpy(host, device="cuda:0")
+    type(ps).t_curr = 0.0
+    kw = {} if mode == "host" else {"device_steps": True}
+    it = Integrator(1.0 / 64, 0.0, ps, method="sakura", **kw)
+    dev = None
+    if mode != "host" and capacity is not None:
+        dev = it.integrator._device()
+        dev.capacity = capacity
+    if mode == "graph":
+        assert it.integrator.evolve(t_end, steps_per_graph=8) >= 1
+    else:
+        while it.time < t_end:
+            it.evolve_step(t_end)
+    out = it.particle_system.to_numpy()
+    return out, it.time, (dev.capacity if dev is not None else None)
+
+
+@pytest.mark.parametrize("capacity", [None, 16])
+def test_device_sakura_graph_matches_host_driver(capacity):
+    """Sakura steps with the pair drifts by tupan_sakura_graph (no host
+    synchronisation), replayed as HIP graphs and one by one, equal the host
+    driver bit for bit on a 2048-particle Plummer sphere with eps = 0 (the
+    host driver's calls take the same two-pass form at ni nj = 2^22); with a
+    near-pair capacity of 16 every first try overflows, is detected from the
+    status word, undone and rerun with a doubled capacity."""
+    host = cpu_plummer(2048, 0.0, 7)
+    t_end = 21.0 / 64   # two replays of 8 steps and 5 single steps
+    ref, t_ref, _ = _sakura_run(host, t_end, "host")
+    for mode in ("eager", "graph"):
+        got, t_got, cap = _sakura_run(host, t_end, mode, capacity)
+        assert t_got == t_ref
+        if capacity is not None:
+            assert cap > capacity
+        for k in ("rx", "ry", "rz", "vx", "vy", "vz", "time", "nstep",
+                  "tstep"):
+            np.testing.assert_array_equal(got[k], ref[k], err_msg=(mode, k))

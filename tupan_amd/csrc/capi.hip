@@ -705,6 +705,19 @@ int tupan_sakura_ex(size_t ni, const double* const* ifields, size_t nj,
     return rc;
 }
 
+int tupan_sakura_graph(size_t ni, const double* const* ifields, size_t nj,
+                       const double* const* jfields, double dt, long flag, double* const* out,
+                       int capacity, unsigned long long* counters, int* status, void* stream)
+{
+    if (!ifields || !jfields || !out) return set_error(TUPAN_ERR_ARG, "null field list");
+    if (capacity < 0) return set_error(TUPAN_ERR_ARG, "negative near-pair capacity");
+    SakuraOptions opt;
+    opt.path = 2;
+    opt.graph_capacity = capacity;
+    return sakura_call_ex(ni, ifields, nj, jfields, dt, flag, out, opt, counters, status,
+                          stream);
+}
+
 int tupan_sia_subtree(int n, void* const* arrays, int narrays, const int* roles, int acc_mode,
                       int tstep_mode, double eta, double tau, int nbridge, const int* bridge_op,
                       const double* bridge_c, int nevolve, const int* evolve_op,

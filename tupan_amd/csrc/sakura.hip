@@ -284,6 +284,7 @@ __global__ __launch_bounds__(BLOCK) void sakura_far_kernel(SakuraArgs a)
 // those entries are appended to `over_list` (in no particular order; each
 // is processed independently).
 constexpr int kScanThreads = 256;
+constexpr int kOverflowWaves = 256;  // grid of the graph-safe overflow pass
 constexpr int kScanPer = 4 * kScanThreads;
 
 __device__ __forceinline__ int block_exclusive_scan(int v, int* lds, int& block_total)
@@ -375,16 +376,26 @@ __global__ __launch_bounds__(kScanThreads) void near_offsets_kernel(int n, const
 }
 
 // Near-pair list: item = (i, j), grouped by (split, i), j ascending.
+// Items at or beyond `cap` are not written (a call whose list is longer
+// than its capacity is flagged by near_capacity_kernel).
 __global__ __launch_bounds__(256) void near_compact_kernel(int n, int ni, const int* count,
                                                            const int* off, const int* near_j,
-                                                           int2* items)
+                                                           int2* items, int cap)
 {
     const int e = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     if (e >= n) return;
     const int c = min(count[e], kNearSlots);
     const int o = off[e];
     const int i = e % ni;
-    for (int k = 0; k < c; ++k) items[o + k] = make_int2(i, near_j[static_cast<size_t>(e) * kNearSlots + k]);
+    for (int k = 0; k < c && o + k < cap; ++k)
+        items[o + k] = make_int2(i, near_j[static_cast<size_t>(e) * kNearSlots + k]);
+}
+
+// Capacity check of the graph-safe form: status bit 2 when the near-pair
+// list is longer than the space reserved for it.
+__global__ void near_capacity_kernel(const int* total, int cap, int* status)
+{
+    if (threadIdx.x == 0 && total[0] > cap && status) atomicOr(status, 2);
 }
 
 // Features of each near pair's first stage, SoA [4][nitems] (kepler.h:
@@ -419,12 +430,16 @@ __global__ __launch_bounds__(128) void near_features_kernel(SakuraArgs a, int ni
 // (kepler.h: Guess).  With a log, the first stage's features, the learned
 // target and the pair's Laguerre steps are recorded per item.
 __global__ __launch_bounds__(128) void sakura_near_kernel(SakuraArgs a, int nitems,
+                                                          const int* nitems_dev,
                                                           const int2* items, double* contrib,
                                                           const float* delta, SolveLogDev log,
                                                           unsigned long long* fallbacks)
 {
+    // nitems: the list's capacity (contrib's stride); nitems_dev, when
+    // given, the number of items listed on the device
     const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    const bool valid = t < nitems;
+    const int count = nitems_dev ? min(*nitems_dev, nitems) : nitems;
+    const bool valid = t < count;
     kep::Counters c;
     if (valid) {
         const int2 it = items[t];
@@ -498,7 +513,8 @@ __global__ __launch_bounds__(256) void near_add_kernel(int n, int ni, int nitems
     double* part = partial + static_cast<size_t>(split) * 6 * ni;
     for (int k = 0; k < 6; ++k) {
         double sum = part[static_cast<size_t>(k) * ni + i];
-        for (int q = 0; q < c; ++q) sum += contrib[static_cast<size_t>(k) * nitems + o + q];
+        for (int q = 0; q < c && o + q < nitems; ++q)
+            sum += contrib[static_cast<size_t>(k) * nitems + o + q];
         part[static_cast<size_t>(k) * ni + i] = sum;
     }
 }
@@ -509,18 +525,13 @@ __global__ __launch_bounds__(256) void near_add_kernel(int n, int ni, int nitems
 // first kNearSlots (in parallel, one per lane), and adds the chunk's
 // contributions to the partial sum through a fixed shuffle tree
 // (deterministic).
-__global__ __launch_bounds__(64) void near_overflow_kernel(SakuraArgs a, const int* over_list,
-                                                           int nover)
+__device__ __forceinline__ void overflow_entry(const SakuraArgs& a, int e, int lane,
+                                               kep::Counters& c)
 {
-    const int w = static_cast<int>(blockIdx.x);
-    const int lane = static_cast<int>(threadIdx.x);
-    if (w >= nover) return;
-    const int e = over_list[w];
     const int split = e / a.ni;
     const int i = e % a.ni;
     const int j0 = split * a.jchunk;
     const int j1 = min(a.nj, j0 + a.jchunk);
-    kep::Counters c;
     double d[6] = {0, 0, 0, 0, 0, 0};
     int seen = 0;  // near pairs before this chunk
     for (int jb = j0; jb < j1; jb += kWave) {
@@ -566,6 +577,19 @@ __global__ __launch_bounds__(64) void near_overflow_kernel(SakuraArgs a, const i
         double* part = a.partial + static_cast<size_t>(split) * 6 * a.ni;
         for (int k = 0; k < 6; ++k) part[static_cast<size_t>(k) * a.ni + i] += d[k];
     }
+}
+
+// The entries are walked grid-stride; nover_dev, when given, holds their
+// number on the device.
+__global__ __launch_bounds__(64) void near_overflow_kernel(SakuraArgs a, const int* over_list,
+                                                           int nover, const int* nover_dev)
+{
+    const int lane = static_cast<int>(threadIdx.x);
+    const int nent = nover_dev ? *nover_dev : nover;
+    kep::Counters c;
+    for (int w = static_cast<int>(blockIdx.x); w < nent; w += static_cast<int>(gridDim.x)) {
+        overflow_entry(a, over_list[w], lane, c);
+    }
     flush_counters(a.counters, true, c, lane);
     if (a.status && c.fail) atomicOr(a.status, 1);
 }
@@ -608,12 +632,19 @@ int sakura_call_ex(size_t ni, const double* const* ip, size_t nj, const double* 
                     (opt.path == 0 && static_cast<double>(ni) * static_cast<double>(nj) >=
                                           static_cast<double>(kTwoPassPairs));
     if (opt.net || opt.log.target) two_pass = true;
+    const bool graph = opt.graph_capacity >= 0;
+    if (graph) {
+        if (opt.net || opt.log.target)
+            return set_error(TUPAN_ERR_ARG, "the graph-safe Sakura form takes no guess network or log");
+        if (!status) return set_error(TUPAN_ERR_ARG, "the graph-safe Sakura form needs a status word");
+        two_pass = true;
+    }
     // the far-pair pass has its own geometry (kFarWpt i per thread)
     Launch L = plan_launch(a.ni, a.nj, BLOCK, two_pass ? kFarWpt : 1, TILE, 2 * dev.target_wg, 64,
                            TILE);
     size_t nent = static_cast<size_t>(L.nsplit) * ni;
     if (two_pass && nent * kNearSlots > static_cast<size_t>(INT32_MAX)) {
-        if (opt.path == 2 || opt.net || opt.log.target)
+        if (opt.path == 2 || opt.net || opt.log.target || graph)
             return set_error(TUPAN_ERR_ARG, "too many particles for the two-pass Sakura form");
         two_pass = false;
         L = plan_launch(a.ni, a.nj, BLOCK, 1, TILE, 2 * dev.target_wg, 64, TILE);
@@ -677,6 +708,33 @@ int sakura_call_ex(size_t ni, const double* const* ip, size_t nj, const double* 
                        bover, total_dev);
     hipLaunchKernelGGL(near_offsets_kernel, dim3(nb), dim3(kScanThreads), 0, s, n,
                        a.near_count, bsum, near_off, total_dev, over_list);
+    const unsigned eg = static_cast<unsigned>((n + 255) / 256);
+    if (graph) {
+        // No host synchronisation: the list gets the caller's capacity, the
+        // near passes read the listed counts on the device, and a longer
+        // list sets status bit 2 (the call's outputs are then invalid).
+        const int cap = opt.graph_capacity > 0 ? opt.graph_capacity : 1;
+        const size_t nt = static_cast<size_t>(cap);
+        const size_t b_items = up(nt * sizeof(int2));
+        const size_t b_con = up(nt * 6 * sizeof(double));
+        char* w2 = static_cast<char*>(workspace(b_items + b_con, s, 2));
+        if (!w2) return set_error(TUPAN_ERR_ALLOC, "workspace allocation failed");
+        int2* items = reinterpret_cast<int2*>(w2);
+        double* contrib = reinterpret_cast<double*>(w2 + b_items);
+        hipLaunchKernelGGL(near_capacity_kernel, dim3(1), dim3(64), 0, s, total_dev, cap, status);
+        hipLaunchKernelGGL(near_compact_kernel, dim3(eg), dim3(256), 0, s, n, a.ni, a.near_count,
+                           near_off, a.near_j, items, cap);
+        hipLaunchKernelGGL(sakura_near_kernel, dim3((cap + 127) / 128), dim3(128), 0, s, a, cap,
+                           total_dev, items, contrib, nullptr, opt.log, opt.fallbacks);
+        hipLaunchKernelGGL(near_add_kernel, dim3(eg), dim3(256), 0, s, n, a.ni, cap,
+                           a.near_count, near_off, contrib, a.partial);
+        hipLaunchKernelGGL(near_overflow_kernel, dim3(kOverflowWaves), dim3(kWave), 0, s, a,
+                           over_list, 0, total_dev + 1);
+        hipLaunchKernelGGL((reduce_kernel<OpSakura>), dim3(rg), dim3(256), 0, s, a);
+        timing_end(s);
+        opt.near_pairs = -1;
+        return check_launch("sakura");
+    }
     int counts[2] = {0, 0};
     int rc = check_hip(hipMemcpyAsync(counts, total_dev, 2 * sizeof(int), hipMemcpyDeviceToHost,
                                       s),
@@ -698,9 +756,8 @@ int sakura_call_ex(size_t ni, const double* const* ip, size_t nj, const double* 
         double* contrib = reinterpret_cast<double*>(w2 + b_items);
         float* feat = opt.net ? reinterpret_cast<float*>(w2 + b_items + b_con) : nullptr;
         float* delta = opt.net ? reinterpret_cast<float*>(w2 + b_items + b_con + b_feat) : nullptr;
-        const unsigned eg = static_cast<unsigned>((n + 255) / 256);
         hipLaunchKernelGGL(near_compact_kernel, dim3(eg), dim3(256), 0, s, n, a.ni,
-                           a.near_count, near_off, a.near_j, items);
+                           a.near_count, near_off, a.near_j, items, total);
         const unsigned ig = static_cast<unsigned>((total + 127) / 128);
         if (opt.net) {
             hipLaunchKernelGGL(near_features_kernel, dim3(ig), dim3(128), 0, s, a, total, items,
@@ -708,14 +765,14 @@ int sakura_call_ex(size_t ni, const double* const* ip, size_t nj, const double* 
             rc = mlp_infer(total, feat, opt.net, delta, s);
             if (rc != TUPAN_OK) return rc;
         }
-        hipLaunchKernelGGL(sakura_near_kernel, dim3(ig), dim3(128), 0, s, a, total, items,
-                           contrib, delta, opt.log, opt.fallbacks);
+        hipLaunchKernelGGL(sakura_near_kernel, dim3(ig), dim3(128), 0, s, a, total, nullptr,
+                           items, contrib, delta, opt.log, opt.fallbacks);
         hipLaunchKernelGGL(near_add_kernel, dim3(eg), dim3(256), 0, s, n, a.ni, total,
                            a.near_count, near_off, contrib, a.partial);
     }
     if (counts[1] > 0) {
         hipLaunchKernelGGL(near_overflow_kernel, dim3(counts[1]), dim3(kWave), 0, s, a,
-                           over_list, counts[1]);
+                           over_list, counts[1], nullptr);
     }
     hipLaunchKernelGGL((reduce_kernel<OpSakura>), dim3(rg), dim3(256), 0, s, a);
     timing_end(s);

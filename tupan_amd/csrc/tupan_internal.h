@@ -119,7 +119,9 @@ struct SakuraOptions {
     const void* net = nullptr;    // packed Kepler-guess network (mlp.hip)
     SolveLogDev log;
     unsigned long long* fallbacks = nullptr;  // device counter of guess fallbacks
-    int near_pairs = -1;          // out: near pairs listed (-1: one-pass form)
+    int near_pairs = -1;          // out: near pairs listed (-1: one-pass or graph-safe form)
+    int graph_capacity = -1;      // >= 0: two-pass form with no host synchronisation and a
+                                  // near-pair list of this capacity (status bit 2 if exceeded)
 };
 
 int sakura_call_ex(size_t ni, const double* const* ip, size_t nj, const double* const* jp,
