@@ -17,6 +17,9 @@ Also reported:
                 the same initial state
   sakura        Kepler solves/s of one Sakura pair-drift call (flag -1,
                 dt 1/64) on a Plummer sphere of N = 16384 with eps = 0
+  sakura_steps  the fixed-step sakura driver on the same sphere, 65 steps
+                driven from the host and as device steps replayed in HIP
+                graphs of 16, steps/s and bitwise agreement
   sakura_ensemble  4096 few-body systems, Sakura dt 1/64 to t = 1
   block_steps   sia21h.dkd hierarchical block steps on a 262144-particle
                 power-law-mass Plummer sphere, pair interactions per second
@@ -180,6 +183,48 @@ def sakura_rate(n):
             "leapfrog_pairs": leap, "kernel_s": kms / 1e3, "call_s": wall,
             "solves_per_s": solves / (kms / 1e3),
             "pairs_per_s": n * n / (kms / 1e3)}
+
+
+def sakura_steps(n, replays=4, per_graph=16):
+    """The fixed-step sakura driver (eta 1/64) on a Plummer sphere of n
+    (equal masses, eps = 0, seed 1), 1 + replays * per_graph steps two ways:
+    driven from the host (two library calls per step, each reading its
+    near-pair count back) and as device steps (one eager step, then HIP
+    graphs of per_graph steps; the timed call includes the capacity probe
+    and the capture).  Returns both rates and whether the final states agree
+    bit for bit."""
+    import torch
+    from tupan_amd.ics.plummer import make_plummer
+    from tupan_amd.integrator import Integrator
+    from tupan_amd.particles import ParticleSystem
+    host = make_plummer(n, 0.0, ("equalmass",), seed=1,
+                        device="cuda:0").to_numpy()
+    nsteps = 1 + replays * per_graph
+    t_end = nsteps / 64.0
+    out = {"n": n, "dt": 1.0 / 64, "steps": nsteps,
+           "steps_per_graph": per_graph}
+    states = {}
+    for label in ("host", "graph"):
+        ps = ParticleSystem.from_numpy(host, device="cuda:0")
+        type(ps).t_curr = 0.0
+        kw = {"device_steps": True} if label == "graph" else {}
+        it = Integrator(1.0 / 64, 0.0, ps, method="sakura", **kw)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if label == "graph":
+            out["replays"] = it.integrator.evolve(t_end,
+                                                  steps_per_graph=per_graph)
+        else:
+            while it.time < t_end:
+                it.evolve_step(t_end)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        out[label] = {"wall_s": wall, "steps_per_s": nsteps / wall}
+        states[label] = it.particle_system.to_numpy()
+    out["graph_equals_host"] = all(
+        (states["graph"][k] == states["host"][k]).all()
+        for k in ("rx", "ry", "rz", "vx", "vy", "vz"))
+    return out
 
 
 def sakura_forms(n, net_path, reps=3):
@@ -665,6 +710,9 @@ def main():
     if args.sakura_n > 0:
         log("bench: Sakura forms and learned guess N=%d" % args.sakura_n)
         out["sakura_forms"] = sakura_forms(args.sakura_n, args.learned_net)
+        log("bench: Sakura driver steps N=%d, host-driven and graphs"
+            % args.sakura_n)
+        out["sakura_steps"] = sakura_steps(args.sakura_n)
 
     if args.ensemble_systems > 0:
         log("bench: Sakura ensemble of %d systems" % args.ensemble_systems)
