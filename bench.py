@@ -569,13 +569,18 @@ def cpu_hermite(n, eta):
 
 
 def cpu_baseline(ps, rows):
-    """Serial oracle on `rows` i-rows against all j; pairs/s -> Gint/s."""
+    """`rows` i-rows against all j on one core, pairs/s -> Gint/s: the
+    reference's own C backend (acc_jerk_kernel.c compiled from its sources
+    with its cffi flags into oracle/_ref) when that build is present, else
+    the oracle (its restatement, same operation order and flags); and the
+    oracle's OpenMP build on all host cores."""
     import numpy as np
-    from oracle import Oracle
+    from oracle import Oracle, Reference, reference_available
     host = ps.to_numpy()
     n = len(host["mass"])
     sub = {k: v[:rows] for k, v in host.items()}
-    orc = Oracle("float64")
+    kind = "reference" if reference_available("float64") else "port"
+    orc = Reference("float64") if kind == "reference" else Oracle("float64")
     best = None
     for _ in range(3):
         t0 = time.perf_counter()
@@ -596,7 +601,7 @@ def cpu_baseline(ps, rows):
     rate_o = rows_omp * n / best_o
     del np
     return ({"value": rate / 1e9, "unit": "Gint/s", "cores": 1,
-             "kind": "port",
+             "kind": kind,
              "sample": "%d i-rows x %d j of the same Plummer state, FP64, "
                        "best of 3 (%.2f s)" % (rows, n, best)},
             {"value": rate_o / 1e9, "unit": "Gint/s", "cores": threads,
