@@ -523,6 +523,42 @@ int tupan_hermite_step(int order, int adaptive, int mode, size_t n,
                        double eta, void *stream);
 
 /* ---------------------------------------------------------------------------
+ * Device-driven Sakura steps (sakura.py:22-50; 'asakura' step size
+ * sakura.py:100-117 with the block step of integrator/__init__.py:48-78).
+ * One call enqueues one whole step: drift tau/2, pair drift (flag -1, tau/2)
+ * and update, pair drift (flag +1) and update, drift tau/2, bookkeeping;
+ * the pair drifts are the graph-safe two-pass form (tupan_sakura_graph) with
+ * dt read from the control block, so a sequence of calls can be captured as
+ * one HIP graph.  No counterpart in the reference (its driver loop is
+ * Python); the arithmetic is the reference driver's.
+ *
+ *   arrays[TUPAN_S_*]  device arrays of n values: FP64, except nstep (int64);
+ *                      DRX..DVZ receive each pair drift's summed changes
+ *   scratch            device FP64 workspace of 2 n values ('asakura' only)
+ *   control            device FP64 [6]: t_curr, t_end, eta, tau, status, dt;
+ *                      as for tupan_hermite_step (status 2: no finite Tstep
+ *                      maximum); dt = tau / 2 is what the pair drifts read
+ *   capacity, counters, status  as for tupan_sakura_graph (status: device
+ *                      int; bit 1 Kepler failure, bit 2 near-pair list
+ *                      longer than capacity: the step's state is invalid)
+ *   mode               force mode (TUPAN_MODE_*) of Tstep
+ * ------------------------------------------------------------------------- */
+#define TUPAN_S_MASS 0
+#define TUPAN_S_EPS2 1
+#define TUPAN_S_RX 2    /* RX RY RZ */
+#define TUPAN_S_VX 5    /* VX VY VZ */
+#define TUPAN_S_DRX 8   /* DRX DRY DRZ DVX DVY DVZ */
+#define TUPAN_S_TSTEP 14
+#define TUPAN_S_TSTEPIJ 15
+#define TUPAN_S_TIME 16
+#define TUPAN_S_NSTEP 17
+#define TUPAN_S_NROLES 18
+int tupan_sakura_step(int adaptive, int mode, size_t n, void *const *arrays,
+                      double *scratch, double *control, double eta,
+                      int capacity, unsigned long long *counters,
+                      int *status, void *stream);
+
+/* ---------------------------------------------------------------------------
  * Host layer of the float build.  libtupan_hip_f32.so implements the
  * reference's float ABI (phi_kernel, ... with unsigned int / int / float)
  * on top of these; ifields/jfields/out as for tupan_pairwise.  Sakura and

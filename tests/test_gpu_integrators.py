@@ -274,18 +274,19 @@ def test_device_hermite_bad_control_raises():
     assert int(q["nstep"].max()) == 0
 
 
-def _sakura_run(host, t_end, mode, capacity=None):
-    """Fixed-step sakura (eta 1/64) to t_end: 'host' = the host driver,
-    'eager' = device steps one by one, 'graph' = device steps replayed as
-    graphs of 8.  capacity, if given, replaces the probed near-pair
-    capacity (to force list overflows)."""
+def _sakura_run(host, t_end, mode, method="sakura", eta=1.0 / 64,
+                capacity=None):
+    """sakura or asakura to t_end: 'host' = the host driver, 'eager' =
+    device steps one by one, 'graph' = device steps replayed as graphs of
+    8.  capacity, if given, replaces the probed near-pair capacity (to force
+    list overflows)."""
     from tupan_amd.integrator import Integrator
     from tupan_amd.particles import ParticleSystem
     install_hip("float64")
     ps = ParticleSystem.from_numpy(host, device="cuda:0")
     type(ps).t_curr = 0.0
     kw = {} if mode == "host" else {"device_steps": True}
-    it = Integrator(1.0 / 64, 0.0, ps, method="sakura", **kw)
+    it = Integrator(eta, 0.0, ps, method=method, **kw)
     dev = None
     if mode != "host" and capacity is not None:
         dev = it.integrator._device()
@@ -299,22 +300,29 @@ def _sakura_run(host, t_end, mode, capacity=None):
     return out, it.time, (dev.capacity if dev is not None else None)
 
 
-@pytest.mark.parametrize("capacity", [None, 16])
-def test_device_sakura_graph_matches_host_driver(capacity):
-    """Sakura steps with the pair drifts by tupan_sakura_graph (no host
-    synchronisation), replayed as HIP graphs and one by one, equal the host
-    driver bit for bit on a 2048-particle Plummer sphere with eps = 0 (the
-    host driver's calls take the same two-pass form at ni nj = 2^22); with a
-    near-pair capacity of 16 every first try overflows, is detected from the
-    status word, undone and rerun with a doubled capacity."""
+@pytest.mark.parametrize("method,eta,t_end,capacity", [
+    ("sakura", 1.0 / 64, 21.0 / 64, None),
+    ("sakura", 1.0 / 64, 21.0 / 64, 16),
+    ("asakura", 0.05, 0.25, None),
+])
+def test_device_sakura_steps_match_host_driver(method, eta, t_end, capacity):
+    """tupan_sakura_step (drifts, pair drifts in the graph-safe two-pass
+    form, updates, the asakura step size and the bookkeeping on the device),
+    one by one and replayed as HIP graphs, equals the host driver bit for bit
+    on a 2048-particle Plummer sphere with eps = 0 (the host driver's calls
+    take the same two-pass form at ni nj = 2^22); with a near-pair capacity
+    of 16 every first try overflows, is detected from the status word,
+    undone and rerun with a doubled capacity."""
     host = cpu_plummer(2048, 0.0, 7)
-    t_end = 21.0 / 64   # two replays of 8 steps and 5 single steps
-    ref, t_ref, _ = _sakura_run(host, t_end, "host")
+    ref, t_ref, _ = _sakura_run(host, t_end, "host", method, eta)
     for mode in ("eager", "graph"):
-        got, t_got, cap = _sakura_run(host, t_end, mode, capacity)
+        got, t_got, cap = _sakura_run(host, t_end, mode, method, eta,
+                                      capacity)
         assert t_got == t_ref
         if capacity is not None:
             assert cap > capacity
-        for k in ("rx", "ry", "rz", "vx", "vy", "vz", "time", "nstep",
-                  "tstep"):
+        keys = ("rx", "ry", "rz", "vx", "vy", "vz", "time", "nstep", "tstep")
+        if method == "asakura":
+            keys += ("tstepij",)
+        for k in keys:
             np.testing.assert_array_equal(got[k], ref[k], err_msg=(mode, k))

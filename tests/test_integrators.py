@@ -426,13 +426,14 @@ def test_diagnostic_report_file(tmp_path):
 
 
 def test_sakura_device_steps_argument_checks():
-    """Device Sakura steps: the adaptive method is refused, evolve needs
-    device_steps and at least one step per graph, and on a host particle
-    system (or the oracle backend) the device path refuses to start."""
+    """Device Sakura steps: evolve needs device_steps and at least one step
+    per graph, and on a host particle system (or the oracle backend) the
+    device path refuses to start."""
     from tupan_amd.integrator import Integrator
-    with pytest.raises(NotImplementedError):
-        Integrator(1.0 / 64, 0.0, make_ps(kepler_pair()), method="asakura",
-                   device_steps=True)
+    it = Integrator(1.0 / 64, 0.0, make_ps(kepler_pair()), method="asakura",
+                    device_steps=True)
+    with pytest.raises(RuntimeError):
+        it.evolve_step(1.0)
     it = Integrator(1.0 / 64, 0.0, make_ps(kepler_pair()), method="sakura")
     with pytest.raises(RuntimeError):
         it.integrator.evolve(1.0)

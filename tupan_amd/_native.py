@@ -100,6 +100,10 @@ def _declare(lib):
                                     ctypes.c_long, _PP, ctypes.c_int, _P,
                                     ctypes.POINTER(SolveLog), _P,
                                     ctypes.POINTER(ctypes.c_int), _P, _P]
+    lib.tupan_sakura_step.restype = ctypes.c_int
+    lib.tupan_sakura_step.argtypes = [ctypes.c_int, ctypes.c_int, _S, _PP, _P,
+                                      _P, ctypes.c_double, ctypes.c_int, _P,
+                                      _P, _P]
     lib.tupan_sakura_graph.restype = ctypes.c_int
     lib.tupan_sakura_graph.argtypes = [_S, _PP, _S, _PP, ctypes.c_double,
                                        ctypes.c_long, _PP, ctypes.c_int, _P,
@@ -204,4 +208,5 @@ DEVICE_ABI = ("tupan_last_error", "tupan_version", "tupan_device_info",
               "tupan_mlp_grad", "tupan_mlp_forward", "tupan_mlp_adam",
               "tupan_mlp_pack", "tupan_mlp_infer", "tupan_sia_subtree",
               "tupan_pnacc", "tupan_pnacc_f32", "tupan_host_pnacc_f32",
-              "tupan_hermite_step", "tupan_sakura_graph")
+              "tupan_hermite_step", "tupan_sakura_graph",
+              "tupan_sakura_step")

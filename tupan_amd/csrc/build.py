@@ -24,7 +24,7 @@ LIBDIR = os.path.join(PKG, "_lib")
 BUILDDIR = os.path.join(PKG, "_build")
 
 SOURCES = ["capi.hip", "pairwise.hip", "sakura.hip", "ensemble.hip", "mlp.hip", "sia.hip",
-           "hermite.hip"]
+           "hermite.hip", "sakura_step.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 

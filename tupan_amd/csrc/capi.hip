@@ -705,6 +705,17 @@ int tupan_sakura_ex(size_t ni, const double* const* ifields, size_t nj,
     return rc;
 }
 
+int tupan_sakura_step(int adaptive, int mode, size_t n, void* const* arrays, double* scratch,
+                      double* control, double eta, int capacity,
+                      unsigned long long* counters, int* status, void* stream)
+{
+    if (mode != TUPAN_MODE_FP64 && mode != TUPAN_MODE_SPLIT32 && mode != TUPAN_MODE_IEEE64)
+        return set_error(TUPAN_ERR_ARG, "unknown mode");
+    if (capacity < 0) return set_error(TUPAN_ERR_ARG, "negative near-pair capacity");
+    return sakura_step_call(adaptive, mode, n, arrays, scratch, control, eta, capacity,
+                            counters, status, stream);
+}
+
 int tupan_sakura_graph(size_t ni, const double* const* ifields, size_t nj,
                        const double* const* jfields, double dt, long flag, double* const* out,
                        int capacity, unsigned long long* counters, int* status, void* stream)

@@ -32,7 +32,14 @@ struct SakuraArgs {
     // of deferred near pairs and their j indices (kNearSlots slots each)
     int* near_count;
     int* near_j;
+    // when set, dt is read from device memory (device-driven steps)
+    const double* dt_dev;
 };
+
+__device__ __forceinline__ double sakura_dt(const SakuraArgs& a)
+{
+    return a.dt_dev ? *a.dt_dev : a.dt;
+}
 
 // Near pairs listed per (j-split, i); further ones are solved by
 // near_overflow_kernel.
@@ -72,7 +79,7 @@ struct OpSakura {
     {
         r.m = a.ip[0][i]; r.x = a.ip[1][i]; r.y = a.ip[2][i]; r.z = a.ip[3][i];
         r.e2 = a.ip[4][i]; r.vx = a.ip[5][i]; r.vy = a.ip[6][i]; r.vz = a.ip[7][i];
-        r.dt = a.dt;
+        r.dt = sakura_dt(a);
         r.flag = a.flag;
     }
     __device__ static void load_j(const Args& a, const Frame&, int j, Slot& s)
@@ -412,7 +419,7 @@ __global__ __launch_bounds__(128) void near_features_kernel(SakuraArgs a, int ni
     const double v0[3] = {a.ip[5][i] - a.jp[5][j], a.ip[6][i] - a.jp[6][j],
                           a.ip[7][i] - a.jp[7][j]};
     const double m = a.ip[0][i] + a.jp[0][j];
-    const double dts = kep::first_stage(a.dt, a.flag, q, v0);
+    const double dts = kep::first_stage(sakura_dt(a), a.flag, q, v0);
     double k[5], s0;
     kep::SolveFeatures f;
     if (!kep::solve_setup(dts, m, e2, q, v0, k, s0, &f)) {
@@ -461,7 +468,7 @@ __global__ __launch_bounds__(128) void sakura_near_kernel(SakuraArgs a, int nite
         }
         const bool logging = log.target != nullptr && t < log.capacity;
         double s_conv = __builtin_nan("");
-        kep::drift_twobody(a.dt, a.flag, m, e2, p, v, c, delta ? &gs : nullptr,
+        kep::drift_twobody(sakura_dt(a), a.flag, m, e2, p, v, c, delta ? &gs : nullptr,
                            logging ? &s_conv : nullptr);
         const double mu = mj / m;
         const size_t n = static_cast<size_t>(nitems);
@@ -473,7 +480,7 @@ __global__ __launch_bounds__(128) void sakura_near_kernel(SakuraArgs a, int nite
         contrib[5 * n + t] = mu * (v[2] - v0[2]);
         if (logging) {
             double q[3] = {p0[0], p0[1], p0[2]};
-            const double dts = kep::first_stage(a.dt, a.flag, q, v0);
+            const double dts = kep::first_stage(sakura_dt(a), a.flag, q, v0);
             double k[5], s0;
             kep::SolveFeatures f{};
             float target = __builtin_nanf("");
@@ -548,7 +555,7 @@ __device__ __forceinline__ void overflow_entry(const SakuraArgs& a, int e, int l
         double p[3] = {p0[0], p0[1], p0[2]};
         double v[3] = {v0[0], v0[1], v0[2]};
         kep::Counters scratch;
-        const bool near = inr && !kep::drift_twobody_far(a.dt, a.flag, m, e2, p, v, scratch);
+        const bool near = inr && !kep::drift_twobody_far(sakura_dt(a), a.flag, m, e2, p, v, scratch);
         const unsigned long long mask = __ballot(near);
         const int rank = seen + __popcll(mask & ((1ull << lane) - 1ull));
         seen += __popcll(mask);
@@ -556,7 +563,7 @@ __device__ __forceinline__ void overflow_entry(const SakuraArgs& a, int e, int l
         if (near && rank >= kNearSlots) {
             p[0] = p0[0]; p[1] = p0[1]; p[2] = p0[2];
             v[0] = v0[0]; v[1] = v0[1]; v[2] = v0[2];
-            kep::drift_twobody(a.dt, a.flag, m, e2, p, v, c);
+            kep::drift_twobody(sakura_dt(a), a.flag, m, e2, p, v, c);
             const double mu = mj / m;
             cd[0] = mu * (p[0] - p0[0]);
             cd[1] = mu * (p[1] - p0[1]);
@@ -625,6 +632,7 @@ int sakura_call_ex(size_t ni, const double* const* ip, size_t nj, const double* 
     a.flag = flag;
     a.counters = counters;
     a.status = status;
+    a.dt_dev = opt.dt_dev;
     constexpr int BLOCK = 128, TILE = 64;
     const Device& dev = current_device();
     hipStream_t s = static_cast<hipStream_t>(stream);

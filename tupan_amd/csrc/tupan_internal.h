@@ -101,6 +101,9 @@ int tune_shape(int kernel, int shape, int* block, int* wpt, int* tile, int* unro
 
 int hermite_step_call(int order, int adaptive, int mode, size_t n, void* const* arrays,
                       double* scratch, double* control, double eta, void* stream);
+int sakura_step_call(int adaptive, int mode, size_t n, void* const* arrays, double* scratch,
+                     double* ctl, double eta, int capacity, unsigned long long* counters,
+                     int* status, void* stream);
 
 int sakura_call(size_t ni, const double* const* ip, size_t nj, const double* const* jp,
                 double dt, long flag, double* const* out, unsigned long long* counters,
@@ -122,6 +125,7 @@ struct SakuraOptions {
     int near_pairs = -1;          // out: near pairs listed (-1: one-pass or graph-safe form)
     int graph_capacity = -1;      // >= 0: two-pass form with no host synchronisation and a
                                   // near-pair list of this capacity (status bit 2 if exceeded)
+    const double* dt_dev = nullptr;  // device dt (replaces the dt argument)
 };
 
 int sakura_call_ex(size_t ni, const double* const* ip, size_t nj, const double* const* jp,
