@@ -527,3 +527,40 @@ def test_ieee64_uniform_softening_bitexact(name):
     ref = getattr(Oracle("float64"), name)(d["i"], d["j"], *sc)
     for o, r, e in zip(KERNELS[name][2], got, ref):
         assert np.array_equal(r, e), (name, o, np.max(np.abs(r - e)))
+
+
+def test_full_prefix_sweep_ieee64_vs_oracle():
+    """The reference's TestCase1 sweep in full (tupan/tests/test_extensions.py
+    :28-83): 256 particles of two member types (the set is two concatenated
+    halves), masses U[0, 1), eps2 = 0, positions and velocities U[0, 10)
+    (seeded here), i = all 256 against every j-prefix 1..256 and the swapped
+    roles, for the eight pairwise kernels; the reference-order FP64 mode
+    must equal the oracle bit for bit at every size."""
+    from oracle import Oracle
+    rng = np.random.default_rng(29)
+    n = 256
+    halves = []
+    for m in (n - n // 2, n // 2):
+        h = {"mass": rng.random(m), "eps2": np.zeros(m)}
+        for k in ("rx", "ry", "rz", "vx", "vy", "vz", "ax", "ay", "az",
+                  "jx", "jy", "jz"):
+            h[k] = 10.0 * rng.random(m)
+        halves.append(h)
+    d = {k: np.concatenate([h[k] for h in halves]) for k in halves[0]}
+    orc = Oracle("float64")
+    inv = (1.0 / 128.0,)
+    pn = (7,) + tuple(inv[0] ** k for k in range(1, 8))
+    for name in sorted(KERNELS):
+        sc = KERNELS[name][1]
+        osc = pn if name == "pnacc" else sc
+        for jdx in range(1, n + 1):
+            sub = {k: v[:jdx] for k, v in d.items()}
+            for hi, hj in ((d, sub), (sub, d)):
+                # fresh device sets: outputs land in the i-set's arrays
+                # (Acc writes ax, which NREG_V and SnapCrackle read)
+                ii = device_ps(hi)
+                jj = ii if hj is hi else device_ps(hj)
+                got = run_ext(name, "float64", "ieee64", ii, jj, sc)
+                ref = getattr(orc, name)(hi, hj, *osc)
+                for o, r, e in zip(KERNELS[name][2], got, ref):
+                    assert np.array_equal(r, e, equal_nan=True), (name, jdx, o)
