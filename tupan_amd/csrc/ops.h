@@ -787,11 +787,12 @@ struct OpSplitTstep {
         float uy = r.vy - s.v[9];
         float uz = r.vz - s.v[10];
         float mt = r.m + s.v[3];
-        float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+        // R^2 with e2 in the first fma, masked as in split_rsqrt (R^2 > e2)
+        const float R2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, e2)));
         float rv = __builtin_fmaf(dz, uz, __builtin_fmaf(dy, uy, dx * ux));
         float v2 = __builtin_fmaf(uz, uz, __builtin_fmaf(uy, uy, ux * ux));
-        const bool mask = r2 > 0.0f;
-        float q1 = __builtin_amdgcn_rsqf(r2 + e2);
+        const bool mask = R2 > e2;
+        float q1 = __builtin_amdgcn_rsqf(R2);
         q1 = mask ? q1 : 0.0f;
         float q2 = q1 * q1;
         float phi = mt * q1;
