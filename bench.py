@@ -675,6 +675,10 @@ def main():
     flops = FLOPS_PER_PAIR * pairs
     out["roofline"] = {
         "bound": "valu",
+        "bound_note": "vector-ALU compute: the pair interaction is not a "
+                      "contraction, so no MFMA; peak is the published FP32 "
+                      "vector peak, achieved counts the reference's 42 flops "
+                      "per pair",
         "kernel": "pair_kernel<OpSplitAccJerk>",
         "achieved": flops / k_split / 1e12,
         "peak": PEAK_FP32 / 1e12,
