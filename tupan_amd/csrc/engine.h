@@ -168,6 +168,53 @@ __global__ __launch_bounds__(BLOCK) void pair_kernel(typename Op::Args a)
     }
 }
 
+// Small sets in one launch: G consecutive lanes (G divides 64) share an
+// i-particle, BLOCK / G of them per workgroup; of every LDS tile lane group
+// member g takes the slots k = g + G m (interleaved, so the group reads
+// neighbouring slots); the G partial sums of an i are then combined in a
+// fixed shuffle tree (lanes 2k and 2k + 1, then 4k and 4k + 2, ...), so
+// results are the same on every run and the call needs no j-split partials
+// or second kernel.
+template <class Op, int BLOCK, int G, int TILE>
+__global__ __launch_bounds__(BLOCK) void pair_small_kernel(typename Op::Args a)
+{
+    using Slot = typename Op::Slot;
+    static_assert(64 % G == 0 && TILE % G == 0 && BLOCK % 64 == 0,
+                  "G must divide 64 and TILE; BLOCK whole waves");
+    constexpr int kIpb = BLOCK / G;
+    __shared__ Slot tile[TILE];
+    const int tid = static_cast<int>(threadIdx.x);
+    const int il = tid / G;
+    const int g = tid % G;
+    const int ibase = static_cast<int>(blockIdx.x) * kIpb;
+    const typename Op::Frame fr = Op::frame(a, ibase);
+    const int i = ibase + il;
+    typename Op::IReg ir;
+    typename Op::Acc acc;
+    Op::load_i(a, fr, i < a.ni ? i : a.ni - 1, ir);
+    Op::zero(acc);
+    for (int jt = 0; jt < a.nj; jt += TILE) {
+        const int cnt = min(TILE, a.nj - jt);
+        __syncthreads();
+        for (int k = tid; k < cnt; k += BLOCK) Op::load_j(a, fr, jt + k, tile[k]);
+        __syncthreads();
+        // TILE / G slots per lane per tile, within any Op's flush interval
+        for (int k = g; k < cnt; k += G) Op::interact(ir, tile[k], acc);
+        Op::flush(acc);
+    }
+    double tot[Op::kNacc];
+#pragma unroll
+    for (int k = 0; k < Op::kNacc; ++k) tot[k] = Op::get(acc, k);
+#pragma unroll
+    for (int off = 1; off < G; off <<= 1) {
+        double other[Op::kNacc];
+#pragma unroll
+        for (int k = 0; k < Op::kNacc; ++k) other[k] = __shfl_xor(tot[k], off);
+        if ((g & (2 * off - 1)) == 0) Op::combine(tot, other);
+    }
+    if (g == 0 && i < a.ni) Op::store(a, i, tot);
+}
+
 // Combine the j-split partials of each i in split order and store.  The
 // partials are loaded kBatch splits at a time before they are combined, so a
 // thread has kBatch * kNacc loads in flight instead of kNacc: for small ni

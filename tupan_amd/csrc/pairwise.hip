@@ -22,10 +22,37 @@ namespace tupan {
 // Below this many i-particles the j-split combine uses 8 lanes per i.
 constexpr int kWideReduceMaxNi = 32768;
 
+// Calls of at most kSmallPairs pairs with kSmallMinJ < nj <= kSmallMaxJ
+// (in any mode but ieee64, whose sums keep the reference's order) take
+// pair_small_kernel: a few-thousand-particle system in one launch instead of
+// j-split partials and a combine kernel.  Sets of at most kSmallMinJ stay on
+// the tile engine, whose one-tile sums the device SIA subtrees reproduce bit
+// for bit; a few i against many j (block-step levels) keep the j-splits,
+// which spread j over the chip.
+constexpr long long kSmallPairs = 1ll << 21;
+constexpr int kSmallMinJ = 256;
+constexpr int kSmallMaxJ = 2048;
+constexpr int kSmallBlock = 256;
+constexpr int kSmallGroups = 64;
+constexpr int kSmallTile = 512;
+
 template <class Op, int BLOCK, int WPT, int TILE, int UNROLL>
 static int run_pairwise(typename Op::Args a, hipStream_t stream, int wg_per_cu)
 {
     if (a.ni <= 0) return 0;
+    if (!a.exact && a.nj > kSmallMinJ && a.nj <= kSmallMaxJ &&
+        static_cast<long long>(a.ni) * a.nj <= kSmallPairs) {
+        a.nsplit = 1;
+        a.jchunk = a.nj;
+        a.partial = nullptr;
+        constexpr int ipb = kSmallBlock / kSmallGroups;
+        const unsigned grid = static_cast<unsigned>((a.ni + ipb - 1) / ipb);
+        timing_begin(stream);
+        hipLaunchKernelGGL((pair_small_kernel<Op, kSmallBlock, kSmallGroups, kSmallTile>),
+                           dim3(grid), dim3(kSmallBlock), 0, stream, a);
+        timing_end(stream);
+        return check_launch("pairwise");
+    }
     const Device& dev = current_device();
     const int target = (wg_per_cu > 0 ? wg_per_cu : 4) * dev.cu_count;
     // j-chunks of whole tiles, except for sets well above a tile in the
