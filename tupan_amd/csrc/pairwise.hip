@@ -34,7 +34,15 @@ constexpr int kSmallMinJ = 256;
 constexpr int kSmallMaxJ = 2048;
 constexpr int kSmallBlock = 256;
 constexpr int kSmallGroups = 64;
-constexpr int kSmallTile = 512;
+// the largest power-of-two tile of at most 1024 slots within 64 KiB of LDS
+// (one tile, and one wait on global memory, for a 1024-particle set)
+template <class Op>
+constexpr int small_tile()
+{
+    int t = 1024;
+    while (t > kSmallGroups && t * static_cast<int>(sizeof(typename Op::Slot)) > 65536) t /= 2;
+    return t;
+}
 
 template <class Op, int BLOCK, int WPT, int TILE, int UNROLL>
 static int run_pairwise(typename Op::Args a, hipStream_t stream, int wg_per_cu)
@@ -48,7 +56,7 @@ static int run_pairwise(typename Op::Args a, hipStream_t stream, int wg_per_cu)
         constexpr int ipb = kSmallBlock / kSmallGroups;
         const unsigned grid = static_cast<unsigned>((a.ni + ipb - 1) / ipb);
         timing_begin(stream);
-        hipLaunchKernelGGL((pair_small_kernel<Op, kSmallBlock, kSmallGroups, kSmallTile>),
+        hipLaunchKernelGGL((pair_small_kernel<Op, kSmallBlock, kSmallGroups, small_tile<Op>()>),
                            dim3(grid), dim3(kSmallBlock), 0, stream, a);
         timing_end(stream);
         return check_launch("pairwise");
