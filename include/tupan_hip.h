@@ -79,7 +79,9 @@ int tupan_timing_read(double *total_ms, int *count);
  * workgroups-per-CU target of the launch planner.  slot: 0 FP64, 1
  * split-FP32, 2 reference-order FP64, 3 float build.  bucket by pair count
  * ni*nj: 0 below 2^22, 1 below 2^30, 2 above.  Default: shape 0, 4
- * workgroups per CU.  The table is process-wide. */
+ * workgroups per CU.  The table is process-wide.  Calls of at most 2^21
+ * pairs with 256 < nj <= 2048 (any mode but reference-order FP64) run a
+ * one-launch small-set kernel that does not consult the table. */
 #define TUPAN_TUNE_SHAPES 6
 #define TUPAN_TUNE_BUCKETS 3
 int tupan_tune_set(int kernel, int slot, int bucket, int shape, int wg_per_cu);
