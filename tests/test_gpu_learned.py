@@ -192,3 +192,34 @@ def test_learned_guess_gain_and_identical_results():
     print("median |delta - guess|: plain %.3e learned %.3e"
           % (err_plain, err_learned))
     assert err_learned < err_plain
+
+
+@pytest.mark.parametrize("flag", [-1, 2])
+def test_graph_safe_two_pass_abi(plummer4k, flag):
+    """tupan_sakura_graph (the two-pass form without its host read of the
+    near-pair count) through the C ABI: with room for the list it equals
+    the synchronising two-pass form bit for bit, work counters included;
+    with a capacity below the listed count it sets status bit 2."""
+    import ctypes
+    from tupan_amd import _native
+    from tupan_amd.lib.hip_backend import current_stream_handle
+    ref, cref, near = sakura_run(plummer4k, 1.0 / 64, flag, "two-pass")
+    assert near > 1
+    lib = _native.lib()
+    fields = _native.ptr_array([getattr(plummer4k, f).data_ptr() for f in F8])
+    for cap, want in ((near, 0), (near - 1, 2)):
+        outs = [torch.zeros(4096, dtype=torch.float64, device="cuda:0")
+                for _ in range(6)]
+        cnt = torch.zeros(5, dtype=torch.int64, device="cuda:0")
+        status = torch.zeros(1, dtype=torch.int32, device="cuda:0")
+        rc = lib.tupan_sakura_graph(
+            4096, fields, 4096, fields, 1.0 / 64, flag,
+            _native.ptr_array([o.data_ptr() for o in outs]), cap,
+            cnt.data_ptr(), status.data_ptr(), current_stream_handle())
+        _native.check(rc, "tupan_sakura_graph")
+        torch.cuda.synchronize()
+        assert int(status.item()) == want, (cap, int(status.item()))
+        if want == 0:
+            for o, r in zip(outs, ref):
+                np.testing.assert_array_equal(o.cpu().numpy(), r)
+            assert cnt.cpu().tolist()[:4] == cref[:4]
