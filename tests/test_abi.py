@@ -106,3 +106,27 @@ def test_unknown_backend_and_precision():
     with pytest.raises(KeyError):
         extensions.Phi("HIP", "float16")
     assert isinstance(extensions.pnacc, extensions.PNAcc)
+
+
+def test_new_entry_points_reject_bad_arguments():
+    """Argument checks that run before any device work (so they hold with or
+    without a GPU): unknown mode, negative near-pair capacity, null outputs
+    and null control blocks give TUPAN_ERR_ARG with a message."""
+    from tupan_amd import _native
+    lib = _native.lib()
+    ERR_ARG = -1
+    null = None
+    arrays = _native.ptr_array([None] * 18)
+    assert lib.tupan_sakura_step(0, 99, 4, arrays, null, null, 0.01, 16,
+                                 null, null, null) == ERR_ARG
+    assert lib.tupan_sakura_step(0, 0, 4, arrays, null, null, 0.01, -1,
+                                 null, null, null) == ERR_ARG
+    assert "capacity" in _native.last_error()
+    fields = _native.ptr_array([None] * 8)
+    assert lib.tupan_sakura_graph(4, fields, 4, fields, 0.01, -1, None, 16,
+                                  null, null, null) == ERR_ARG
+    assert lib.tupan_sakura_graph(4, fields, 4, fields, 0.01, -1,
+                                  _native.ptr_array([None] * 6), -5, null,
+                                  null, null) == ERR_ARG
+    assert lib.tupan_rsq_bias(None) == ERR_ARG
+    assert _native.last_error()
