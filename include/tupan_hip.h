@@ -25,7 +25,9 @@
  * Output arrays are overwritten, never accumulated into.  Kernels keep no
  * pointer past the call.  The device layer uses a per-device scratch
  * workspace for j-split partial sums, so calls on one device must not
- * overlap in time on different streams.
+ * overlap in time on different streams.  A workspace that a larger call
+ * outgrows is replaced by one at least twice its size and kept allocated
+ * until the process ends, so HIP graphs captured earlier stay valid.
  */
 #ifndef TUPAN_HIP_H
 #define TUPAN_HIP_H

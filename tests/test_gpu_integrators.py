@@ -326,3 +326,51 @@ def test_device_sakura_steps_match_host_driver(method, eta, t_end, capacity):
             keys += ("tstepij",)
         for k in keys:
             np.testing.assert_array_equal(got[k], ref[k], err_msg=(mode, k))
+
+
+def test_sakura_graph_replays_survive_workspace_growth():
+    """A captured graph keeps the scratch workspace addresses it was
+    recorded with.  Between its replays a call on a 64 times larger set
+    (near-pair list beyond the recorded one) makes the library outgrow
+    those workspaces; the outgrown buffers must stay valid, so the replays
+    after it still equal the host driver bit for bit."""
+    from tupan_amd.integrator import Integrator
+    from tupan_amd.particles import ParticleSystem
+    eta, t_end = 1.0 / 64, 13.0 / 64
+    host = cpu_plummer(2048, 0.0, 7)
+    ref, t_ref, _ = _sakura_run(host, t_end, "host", "sakura", eta)
+    install_hip("float64")
+    ps = ParticleSystem.from_numpy(host, device="cuda:0")
+    type(ps).t_curr = 0.0
+    it = Integrator(eta, 0.0, ps, method="sakura", device_steps=True)
+    sak = it.integrator
+    sak.initialize(t_end)
+    dev = sak._device()
+    dev.set_control(t_end)
+    dev.probe_capacity()
+    t, grown = dev.run(dev.launch)
+    assert not grown
+    graph = dev.capture(4)
+    t, grown = dev.run(graph.replay)
+    assert not grown
+    # outgrow the workspaces: the two-pass probe of a 131072-particle set
+    # (about 11 times the near pairs of the 2048-particle one); the device's
+    # free memory shows that a workspace grew
+    from tupan_amd.ics.plummer import make_plummer
+    big = make_plummer(131072, 0.0, ("equalmass",), seed=3, device="cuda:0")
+    it_big = Integrator(eta, 0.0, big, method="sakura", device_steps=True)
+    dev_big = it_big.integrator._device()
+    torch.cuda.synchronize()
+    free0 = torch.cuda.mem_get_info()[0]
+    dev_big.probe_capacity()
+    torch.cuda.synchronize()
+    assert torch.cuda.mem_get_info()[0] < free0
+    del dev_big, it_big, big
+    for _ in range(2):
+        t, grown = dev.run(graph.replay)
+        assert not grown
+    torch.cuda.synchronize()
+    assert t == t_ref
+    got = ps.to_numpy()
+    for k in ("rx", "ry", "rz", "vx", "vy", "vz", "time", "nstep", "tstep"):
+        np.testing.assert_array_equal(got[k], ref[k], err_msg=k)

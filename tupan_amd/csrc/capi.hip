@@ -20,6 +20,7 @@ std::mutex g_dev_mutex;
 std::vector<Device> g_devices;
 std::vector<void*> g_stage;       // host-layer staging buffers, per device
 std::vector<size_t> g_stage_bytes;
+std::vector<void*> g_retired;     // outgrown workspaces, kept until exit
 
 Device& device_slot()
 {
@@ -55,21 +56,26 @@ void* workspace(size_t bytes, hipStream_t, int slot)
     if (slot < 0 || slot > 3) return nullptr;
     Device& d = device_slot();
     if (d.ws_bytes[slot] >= bytes) return d.ws[slot];
+    // at least twice the old size, so a slot's retired buffers never add up
+    // to more than its current one
     size_t want = bytes + bytes / 4;
-    if (d.ws[slot]) {
-        // earlier launches may still read the old buffer
-        (void)hipDeviceSynchronize();
-        (void)hipFree(d.ws[slot]);
-        d.ws[slot] = nullptr;
-        d.ws_bytes[slot] = 0;
-    }
-    if (hipMalloc(&d.ws[slot], want) != hipSuccess) {
+    if (want < 2 * d.ws_bytes[slot]) want = 2 * d.ws_bytes[slot];
+    void* p = nullptr;
+    if (hipMalloc(&p, want) != hipSuccess) {
         (void)hipGetLastError();
-        d.ws[slot] = nullptr;
         return nullptr;
     }
+    if (d.ws[slot]) {
+        // Retired, not freed: a HIP graph captured earlier holds the old
+        // address in its kernel nodes and may be replayed after this call
+        // (and launches still in flight may use it), so it stays valid until
+        // the process ends.
+        std::lock_guard<std::mutex> lock(g_dev_mutex);
+        g_retired.push_back(d.ws[slot]);
+    }
+    d.ws[slot] = p;
     d.ws_bytes[slot] = want;
-    return d.ws[slot];
+    return p;
 }
 
 static void* staging(size_t bytes)

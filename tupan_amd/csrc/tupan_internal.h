@@ -67,8 +67,10 @@ const Device& current_device();
 
 // Scratch workspace `slot` (0..3) of at least `bytes` on the current device.
 // Slots are independent buffers, so one call can hold several at once.
-// Growing one synchronises the device before the old buffer is released.
-// Returns null on allocation failure.
+// Growing one allocates a new buffer of at least twice the size and keeps the
+// old one allocated until the process ends (captured HIP graphs may still
+// address it); no synchronisation.  Returns null on allocation failure (the
+// old buffer then stays current).
 void* workspace(size_t bytes, hipStream_t stream, int slot = 0);
 
 // Error state (thread-local message).  set_error returns `code`.
