@@ -464,6 +464,55 @@ def test_large_random_rows_vs_oracle(mode):
     assert abs(fx) <= (1e-12 if mode == "fp64" else 1e-6) * amag
 
 
+@pytest.mark.parametrize("mode", ["fp64", "split32"])
+def test_block_step_size_salpeter_rows_vs_oracle(mode):
+    """The block-step workload's size: Plummer N = 262144 with a Salpeter
+    mass function (slope -2.35, masses 0.1-10 normalised to total mass 1),
+    eps = 1e-4.  Acc of the whole set, and of an uneven contiguous i-subset
+    (one block-step level, a view into the same arrays) against all j:
+    sampled rows of both against the oracle (FP64 within 1e-13 of |a|;
+    split-FP32 within 8 FP32 ulps of the row's sum of contribution
+    magnitudes) and momentum conservation over all rows."""
+    from oracle import Oracle
+    from tupan_amd.ics.imf import IMF
+    from tupan_amd.ics.plummer import make_plummer
+    from tupan_amd.lib import extensions
+    n = 262144
+    ps = make_plummer(n, 1e-4, IMF.power_law(-2.35, 0.1, 10.0), seed=5,
+                      device="cuda:0")
+    d = ps.to_numpy()
+    acc = extensions.Acc("HIP", "float64", mode=mode)
+    full = [r.detach().cpu().numpy().copy() for r in acc.calc(ps, ps)]
+    lo, hi = 100003, 100003 + 37717
+    part = [r.detach().cpu().numpy().copy()
+            for r in acc.calc(ps[lo:hi], ps)]
+    keys = ("mass", "rx", "ry", "rz", "eps2", "vx", "vy", "vz")
+    dj = {k: d[k] for k in keys}
+    rng = np.random.default_rng(5)
+    for got, rows, off in ((full, rng.choice(n, 24, replace=False), 0),
+                           (part, lo + rng.choice(hi - lo, 12,
+                                                  replace=False), lo)):
+        di = {k: d[k][rows] for k in keys}
+        ref = Oracle("float64").acc(di, dj)
+        g = [got[k][rows - off] for k in range(3)]
+        if mode == "fp64":
+            e = vec_rel_err(g, ref[:3])
+            print("N = %d %s rows at %d: %.3e of |a|" % (n, mode, off, e))
+            assert e <= 1e-13, (off, e)
+        else:
+            sa, _ = abs_sums(di, dj)
+            err = np.sqrt(sum((g[k] - ref[k])**2 for k in range(3)))
+            e = float(np.max(err / sa))
+            print("N = %d %s rows at %d: %.3e of sum |contrib|"
+                  % (n, mode, off, e))
+            assert e <= 8 * 2.0**-24, (off, e)
+    m = d["mass"]
+    for k in range(3):
+        f = float(np.sum(m * full[k]))
+        amag = float(np.sum(m * np.abs(full[k])))
+        assert abs(f) <= (1e-12 if mode == "fp64" else 1e-6) * amag, k
+
+
 def test_split_combine_order():
     """The j-split partial sums are combined in a fixed order.  With the
     reference-order FP64 pair kernel each split's partial sum equals the
