@@ -443,3 +443,19 @@ def test_sakura_device_steps_argument_checks():
         it.integrator.evolve(1.0, steps_per_graph=0)
     with pytest.raises(RuntimeError):
         it.evolve_step(1.0)
+
+
+def test_hermite_device_steps_argument_checks():
+    """Device Hermite steps take the same checks as the Sakura ones: evolve
+    needs device_steps and at least one step per graph, and on a host
+    particle system the device path refuses to start."""
+    from tupan_amd.integrator import Integrator
+    it = Integrator(1.0 / 64, 0.0, make_ps(kepler_pair()), method="ahermite4")
+    with pytest.raises(RuntimeError):
+        it.integrator.evolve(1.0)
+    it = Integrator(1.0 / 64, 0.0, make_ps(kepler_pair()), method="ahermite4",
+                    device_steps=True)
+    with pytest.raises(ValueError):
+        it.integrator.evolve(1.0, steps_per_graph=0)
+    with pytest.raises(RuntimeError):
+        it.evolve_step(1.0)

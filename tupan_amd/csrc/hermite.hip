@@ -54,6 +54,13 @@ struct HermiteArrays {
     double *a1[3], *j1[3], *s1[3], *c1[3];
 };
 
+// min(a, b) that returns NaN when either is NaN, as numpy's min over an
+// array with a NaN does (fmin would drop it and pick a step from the rest).
+__device__ __forceinline__ double min_nan(double a, double b)
+{
+    return (a != a || b != b) ? __builtin_nan("") : fmin(a, b);
+}
+
 __global__ __launch_bounds__(kTauThreads) void hermite_tau_kernel(int n, const double* ts,
                                                                    int adaptive, double* ctl)
 {
@@ -61,11 +68,11 @@ __global__ __launch_bounds__(kTauThreads) void hermite_tau_kernel(int n, const d
     const int tid = static_cast<int>(threadIdx.x);
     double lo = DBL_MAX;
     if (adaptive)
-        for (int i = tid; i < n; i += kTauThreads) lo = fmin(lo, fabs(ts[i]));
+        for (int i = tid; i < n; i += kTauThreads) lo = min_nan(lo, fabs(ts[i]));
     red[tid] = lo;
     __syncthreads();
     for (int w = kTauThreads / 2; w > 0; w >>= 1) {
-        if (tid < w) red[tid] = fmin(red[tid], red[tid + w]);
+        if (tid < w) red[tid] = min_nan(red[tid], red[tid + w]);
         __syncthreads();
     }
     if (tid != 0) return;

@@ -303,6 +303,10 @@ class Hermite(Base):
         """Advance to t_end by device steps, replaying a HIP graph of
         steps_per_graph steps; steps past t_end inside the last replay change
         nothing.  Returns the number of replays."""
+        if not self.device_steps:
+            raise RuntimeError("evolve needs device_steps=True")
+        if steps_per_graph < 1:
+            raise ValueError("steps_per_graph must be at least 1")
         if not self.is_initialized:
             self.initialize(t_end)
         dev = self._device()
