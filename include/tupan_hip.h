@@ -309,6 +309,33 @@ int tupan_pairwise_f32(int kernel, int mode, size_t ni,
                        const float *const *jfields, float *const *out,
                        float eta, void *stream);
 
+/* tupan_pairwise without allocation (no counterpart in the reference; for
+ * callers that cannot allocate device memory: code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results as tupan_pairwise, except that the j-split
+ * partial sums go to a fixed device array of TUPAN_NOALLOC_SCRATCH_BYTES
+ * (64 MiB) that the library holds on every device it runs on until the
+ * process ends, instead of the growable workspace, and that no timing
+ * events are recorded.  The launch plan caps its j-split count at what that
+ * array holds (splits x partial fields x ni doubles), so a call that needs
+ * more partial sums than the maximum runs with fewer j-splits (down to one,
+ * which needs none) rather than overflowing it: results equal
+ * tupan_pairwise's bit for bit when the cap does not bind, and to FP64
+ * summation order otherwise.  A plan that still exceeded the array would
+ * return TUPAN_ERR_ARG.  The first call on a device looks up that device's
+ * array (hipGetSymbolAddress) and fails with TUPAN_ERR_HIP if it cannot.
+ * Every call on one device shares that device's array: calls that overlap
+ * in time on one device (different streams, threads, or replays of graphs
+ * that hold this call) race on it and give undefined results.  The array
+ * needs no clearing between calls: every partial a call combines is
+ * written by that call first.  TUPAN_K_PNACC returns TUPAN_ERR_ARG, as for
+ * tupan_pairwise. */
+#define TUPAN_NOALLOC_SCRATCH_BYTES (64u * 1024u * 1024u)
+int tupan_pairwise_noalloc(int kernel, int mode, size_t ni,
+                           const double *const *ifields, size_t nj,
+                           const double *const *jfields, double *const *out,
+                           double eta, void *stream);
+
 /* PN pair accelerations (pnacc_kernel.c:5-58, libtupan.h:121-149) on device
  * arrays.  ifields/jfields: m rx ry rz e2 vx vy vz; out: pnax pnay pnaz;
  * inv7[k-1] = 1/c^k for k = 1..7 (a host array).  Modes TUPAN_MODE_FP64 and

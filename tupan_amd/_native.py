@@ -62,6 +62,8 @@ def _declare(lib):
     lib.tupan_pairwise.restype = ctypes.c_int
     lib.tupan_pairwise.argtypes = [ctypes.c_int, ctypes.c_int, _S, _PP, _S, _PP,
                                    _PP, ctypes.c_double, _P]
+    lib.tupan_pairwise_noalloc.restype = ctypes.c_int
+    lib.tupan_pairwise_noalloc.argtypes = lib.tupan_pairwise.argtypes
     lib.tupan_pairwise_f32.restype = ctypes.c_int
     lib.tupan_pairwise_f32.argtypes = [ctypes.c_int, ctypes.c_int, _S, _PP, _S,
                                        _PP, _PP, ctypes.c_float, _P]
@@ -200,6 +202,7 @@ DEVICE_ABI = ("tupan_last_error", "tupan_version", "tupan_device_info",
               "tupan_timing", "tupan_timing_read",
               "tupan_phi", "tupan_acc", "tupan_acc_jerk", "tupan_snap_crackle",
               "tupan_tstep", "tupan_pairwise", "tupan_pairwise_f32",
+              "tupan_pairwise_noalloc",
               "tupan_sakura", "tupan_kepler_pairs", "tupan_host_pairwise_f32",
               "tupan_host_sakura_f32", "tupan_host_kepler_f32",
               "tupan_ensemble_create", "tupan_ensemble_destroy",

@@ -580,6 +580,13 @@ int tupan_pairwise(int kernel, int mode, size_t ni, const double* const* ifields
     return pairwise_call<double>(kernel, mode, ni, ifields, nj, jfields, out, eta, stream);
 }
 
+int tupan_pairwise_noalloc(int kernel, int mode, size_t ni, const double* const* ifields,
+                           size_t nj, const double* const* jfields, double* const* out,
+                           double eta, void* stream)
+{
+    return pairwise_noalloc_call(kernel, mode, ni, ifields, nj, jfields, out, eta, stream);
+}
+
 int tupan_pairwise_f32(int kernel, int mode, size_t ni, const float* const* ifields, size_t nj,
                        const float* const* jfields, float* const* out, float eta, void* stream)
 {

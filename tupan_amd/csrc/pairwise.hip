@@ -3,6 +3,7 @@
 //
 // This file is compiled with -ffp-contract=off: every fused multiply-add in
 // ops.h is explicit.
+#include <mutex>
 #include <vector>
 
 #include "ops.h"
@@ -44,8 +45,18 @@ constexpr int small_tile()
     return t;
 }
 
+// Device memory a caller lends for the j-split partial sums instead of the
+// library's growable workspace: `doubles` values at `p`.  The launch plan
+// then caps its j-split count so the partials fit, and the call records no
+// timing events.
+struct FixedScratch {
+    double* p;
+    size_t doubles;
+};
+
 template <class Op, int BLOCK, int WPT, int TILE, int UNROLL>
-static int run_pairwise(typename Op::Args a, hipStream_t stream, int wg_per_cu)
+static int run_pairwise(typename Op::Args a, hipStream_t stream, int wg_per_cu,
+                        const FixedScratch* fixed)
 {
     if (a.ni <= 0) return 0;
     if (!a.exact && a.nj > kSmallMinJ && a.nj <= kSmallMaxJ &&
@@ -55,10 +66,10 @@ static int run_pairwise(typename Op::Args a, hipStream_t stream, int wg_per_cu)
         a.partial = nullptr;
         constexpr int ipb = kSmallBlock / kSmallGroups;
         const unsigned grid = static_cast<unsigned>((a.ni + ipb - 1) / ipb);
-        timing_begin(stream);
+        if (!fixed) timing_begin(stream);
         hipLaunchKernelGGL((pair_small_kernel<Op, kSmallBlock, kSmallGroups, small_tile<Op>()>),
                            dim3(grid), dim3(kSmallBlock), 0, stream, a);
-        timing_end(stream);
+        if (!fixed) timing_end(stream);
         return check_launch("pairwise");
     }
     const Device& dev = current_device();
@@ -70,20 +81,33 @@ static int run_pairwise(typename Op::Args a, hipStream_t stream, int wg_per_cu)
     // (the reference's summation order) whatever shape is tuned.
     const int min_chunk = a.exact ? (TILE > 256 ? TILE : 256)
                                   : (a.nj > 4 * TILE ? 32 : TILE);
-    Launch L = plan_launch(a.ni, a.nj, BLOCK, WPT, TILE, target, 64, min_chunk);
+    // at most 64 j-splits; with lent scratch, at most as many as it holds
+    int max_split = 64;
+    if (fixed) {
+        const size_t per_split = static_cast<size_t>(Op::kNacc) * static_cast<size_t>(a.ni);
+        const size_t fit = fixed->doubles / per_split;
+        if (fit < static_cast<size_t>(max_split)) max_split = fit > 1 ? static_cast<int>(fit) : 1;
+    }
+    Launch L = plan_launch(a.ni, a.nj, BLOCK, WPT, TILE, target, max_split, min_chunk);
     a.nsplit = L.nsplit;
     a.jchunk = L.jchunk;
     a.partial = nullptr;
     if (L.nsplit > 1) {
         size_t need = static_cast<size_t>(L.nsplit) * Op::kNacc * a.ni * sizeof(double);
-        a.partial = static_cast<double*>(workspace(need, stream));
-        if (!a.partial) return set_error(TUPAN_ERR_ALLOC, "workspace allocation failed");
+        if (fixed) {
+            if (need > fixed->doubles * sizeof(double))
+                return set_error(TUPAN_ERR_ARG, "j-split partials exceed the fixed scratch");
+            a.partial = fixed->p;
+        } else {
+            a.partial = static_cast<double*>(workspace(need, stream));
+            if (!a.partial) return set_error(TUPAN_ERR_ALLOC, "workspace allocation failed");
+        }
     }
     const unsigned grid = static_cast<unsigned>(L.nblock_i) * static_cast<unsigned>(L.nsplit);
-    timing_begin(stream);
+    if (!fixed) timing_begin(stream);
     hipLaunchKernelGGL((pair_kernel<Op, BLOCK, WPT, TILE, UNROLL>), dim3(grid), dim3(BLOCK), 0,
                        stream, a);
-    timing_end(stream);
+    if (!fixed) timing_end(stream);
     if (L.nsplit > 1) {
         if (a.ni <= kWideReduceMaxNi) {
             const unsigned rg = static_cast<unsigned>((static_cast<size_t>(a.ni) * 8 + 255) / 256);
@@ -97,7 +121,7 @@ static int run_pairwise(typename Op::Args a, hipStream_t stream, int wg_per_cu)
 }
 
 template <class Op>
-using Launcher = int (*)(typename Op::Args, hipStream_t, int);
+using Launcher = int (*)(typename Op::Args, hipStream_t, int, const FixedScratch*);
 
 // Shapes for the light interactions (Phi, Acc, AccJerk, Tstep, split); the
 // last, one wave per workgroup, is for small systems.
@@ -160,25 +184,27 @@ static int mode_slot(int mode, bool f32)
 
 template <class Op>
 static int launch_tuned(const Launcher<Op> (&table)[kShapes], int kernel, int slot,
-                        typename Op::Args& a, hipStream_t s)
+                        typename Op::Args& a, hipStream_t s, const FixedScratch* fixed)
 {
     const TuneEntry& t = g_tune[kernel][slot][size_bucket(a.ni, a.nj)];
     const int shape = (t.shape >= 0 && t.shape < kShapes) ? t.shape : 0;
-    return table[shape](a, s, t.wg_per_cu);
+    return table[shape](a, s, t.wg_per_cu, fixed);
 }
 
 #define TUPAN_RUN(Op, SHAPES, kernel, slot)                                      \
     do {                                                                         \
         static const Launcher<Op> table[kShapes] = SHAPES(Op);                  \
-        return launch_tuned<Op>(table, kernel, slot, a, s);                      \
+        return launch_tuned<Op>(table, kernel, slot, a, s, fixed);               \
     } while (0)
 
 // Kernel selection by name and mode.
 template <class T>
-static int dispatch(int kernel, int mode, PairArgs<T>& a, hipStream_t s);
+static int dispatch(int kernel, int mode, PairArgs<T>& a, hipStream_t s,
+                    const FixedScratch* fixed);
 
 template <>
-int dispatch<double>(int kernel, int mode, PairArgs<double>& a, hipStream_t s)
+int dispatch<double>(int kernel, int mode, PairArgs<double>& a, hipStream_t s,
+                     const FixedScratch* fixed)
 {
     const int slot = mode_slot(mode, false);
     switch (kernel) {
@@ -219,7 +245,8 @@ int dispatch<double>(int kernel, int mode, PairArgs<double>& a, hipStream_t s)
 }
 
 template <>
-int dispatch<float>(int kernel, int, PairArgs<float>& a, hipStream_t s)
+int dispatch<float>(int kernel, int, PairArgs<float>& a, hipStream_t s,
+                    const FixedScratch* fixed)
 {
     const int slot = mode_slot(0, true);
     switch (kernel) {
@@ -270,9 +297,17 @@ int tune_shape(int kernel, int shape, int* block, int* wpt, int* tile, int* unro
     return TUPAN_OK;
 }
 
+namespace {
+int noalloc_scratch(double** out);
+}
+
+// `fixed`: null for the growable workspace; else lent scratch, whose device
+// address is looked up (noalloc_scratch) once the arguments have passed
+// their checks.
 template <class T>
-int pairwise_call(int kernel, int mode, size_t ni, const T* const* ip, size_t nj,
-                  const T* const* jp, T* const* out, T eta, void* stream, const PnParams* pn)
+static int pairwise_impl(int kernel, int mode, size_t ni, const T* const* ip, size_t nj,
+                         const T* const* jp, T* const* out, T eta, void* stream,
+                         const PnParams* pn, FixedScratch* fixed)
 {
     const int nin = kernel_in_fields(kernel);
     const int nout = kernel_out_fields(kernel);
@@ -311,8 +346,58 @@ int pairwise_call(int kernel, int mode, size_t ni, const T* const* ip, size_t nj
     if (mode == TUPAN_MODE_SPLIT32 && kernel != K_ACC && kernel != K_ACC_JERK &&
         kernel != K_TSTEP && kernel != K_PNACC)
         mode = TUPAN_MODE_FP64;  // split FP32 exists for the forces, Tstep and PNAcc
-    return dispatch<T>(kernel, mode, a, static_cast<hipStream_t>(stream));
+    if (fixed && !fixed->p) {
+        const int rc = noalloc_scratch(&fixed->p);
+        if (rc != TUPAN_OK) return rc;
+    }
+    return dispatch<T>(kernel, mode, a, static_cast<hipStream_t>(stream), fixed);
 }
+
+template <class T>
+int pairwise_call(int kernel, int mode, size_t ni, const T* const* ip, size_t nj,
+                  const T* const* jp, T* const* out, T eta, void* stream, const PnParams* pn)
+{
+    return pairwise_impl<T>(kernel, mode, ni, ip, nj, jp, out, eta, stream, pn, nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// tupan_pairwise_noalloc: the pairwise kernels with their j-split partial
+// sums in a fixed device array instead of the growable workspace, so the
+// call allocates nothing.  Each device that loads this library holds its
+// own copy of `scratch` (64 MiB) until the process ends.  In host code
+// `scratch` names a host-side stand-in, never the device array: the device
+// address comes from hipGetSymbolAddress, taken once per device with that
+// device current, and only that address reaches kernels.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoallocDoubles = TUPAN_NOALLOC_SCRATCH_BYTES / sizeof(double);
+static __device__ __align__(256) double scratch[kNoallocDoubles];
+
+namespace {
+constexpr int kMaxNoallocDevices = 64;
+std::mutex g_noalloc_mutex;
+double* g_noalloc_ptr[kMaxNoallocDevices] = {};
+
+// Device address of `scratch` on the current device (looked up on the first
+// call on that device).
+int noalloc_scratch(double** out)
+{
+    int id = 0;
+    int rc = check_hip(hipGetDevice(&id), "hipGetDevice");
+    if (rc != TUPAN_OK) return rc;
+    if (id < 0 || id >= kMaxNoallocDevices)
+        return set_error(TUPAN_ERR_ARG, "device index beyond the no-alloc table (64 devices)");
+    std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+    if (!g_noalloc_ptr[id]) {
+        void* ptr = nullptr;
+        rc = check_hip(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (rc != TUPAN_OK) return rc;
+        if (!ptr) return set_error(TUPAN_ERR_HIP, "hipGetSymbolAddress(scratch) gave no address");
+        g_noalloc_ptr[id] = static_cast<double*>(ptr);
+    }
+    *out = g_noalloc_ptr[id];
+    return TUPAN_OK;
+}
+}  // namespace
 
 // Mean relative error of v_rsq_f32, raw and after the Newton step the split
 // kernels can build with, over every float in [1, 4) (2^24 inputs, both
@@ -369,6 +454,13 @@ int rsq_bias_probe(double* out)
     }
     out[4] = kRsqBias;
     return 0;
+}
+
+int pairwise_noalloc_call(int kernel, int mode, size_t ni, const double* const* ip, size_t nj,
+                          const double* const* jp, double* const* out, double eta, void* stream)
+{
+    FixedScratch fs{nullptr, kNoallocDoubles};
+    return pairwise_impl<double>(kernel, mode, ni, ip, nj, jp, out, eta, stream, nullptr, &fs);
 }
 
 template int pairwise_call<double>(int, int, size_t, const double* const*, size_t,

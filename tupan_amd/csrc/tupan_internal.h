@@ -96,6 +96,11 @@ int pairwise_call(int kernel, int mode, size_t ni, const T* const* ip, size_t nj
                   const T* const* jp, T* const* out, T eta, void* stream,
                   const PnParams* pn = nullptr);
 
+// pairwise_call (no PN parameters) with the j-split partial sums in the
+// library's fixed device array: allocates nothing (pairwise.hip).
+int pairwise_noalloc_call(int kernel, int mode, size_t ni, const double* const* ip, size_t nj,
+                          const double* const* jp, double* const* out, double eta, void* stream);
+
 // Tuning table of the pairwise kernels (pairwise.hip).
 int tune_set(int kernel, int slot, int bucket, int shape, int wg_per_cu);
 int tune_get(int kernel, int slot, int bucket, int* shape, int* wg_per_cu);
