@@ -651,3 +651,52 @@ def test_small_set_kernel_vs_oracle(name):
                 e = vec_rel_err([np.asarray(got[k], np.float64) for k in grp],
                                 [ref[k] for k in grp])
                 assert e <= bar, (name, prec, mode, len(hi["mass"]), grp, e)
+
+
+@pytest.mark.parametrize("n", [600, 3000])
+def test_split32_fp32_overflow_falls_back_to_fp64(n):
+    """Unsoftened close pairs overflow the split-FP32 pair terms (m / R^3 at
+    R = 1e-15 is ~1e42, beyond FP32): the kernels evaluate every i whose
+    sums come out non-finite again in FP64, so split32 Acc, AccJerk, Tstep
+    and PNAcc give the FP64 path's values there instead of inf or NaN.  One
+    pair approaches and one recedes (the Tstep pair term's sign case).
+    N = 600 runs the one-launch small-set kernel (the lanes of an i each
+    redo their own j), N = 3000 the tile engine (an i redoes its j-split).
+    Close-pair rows: within 1e-12 of the FP64 path (both FP64, summed in a
+    different order); every other row finite and within 1e-5."""
+    from tupan_amd.lib import extensions
+    rng = np.random.default_rng(21)
+    d = {k: rng.random(n) for k in ("rx", "ry", "rz", "vx", "vy", "vz")}
+    d["mass"] = np.full(n, 1.0 / n) * (1 + rng.random(n))
+    d["eps2"] = np.zeros(n)
+    # rows 0..3 lie near row 0, the frame of the first workgroup: 0-1 at
+    # 1e-15 approaching, 2-3 at 1e-15 receding, 1e-6 from 0-1
+    x0 = (d["rx"][0], d["ry"][0], d["rz"][0])
+    d["rx"][1] = x0[0] + 1e-15
+    d["ry"][1], d["rz"][1] = x0[1], x0[2]
+    d["rx"][2], d["ry"][2], d["rz"][2] = x0[0], x0[1] + 1e-6, x0[2]
+    d["rx"][3], d["ry"][3], d["rz"][3] = x0[0] + 1e-15, x0[1] + 1e-6, x0[2]
+    d["vx"][0], d["vx"][1] = 1.0, -1.0
+    d["vx"][2], d["vx"][3] = -1.0, 1.0
+    ps = device_ps(d)
+    close = [0, 1, 2, 3]
+    set_clight(7, 128.0)
+    for name, cls, sc in (("acc", "Acc", ()), ("acc_jerk", "AccJerk", ()),
+                          ("tstep", "Tstep", (1.0 / 64,)),
+                          ("pnacc", "PNAcc", ())):
+        ref = [r.detach().cpu().numpy() for r in getattr(extensions, cls)(
+            "HIP", "float64", mode="fp64").calc(ps, ps, *sc)]
+        got = [r.detach().cpu().numpy() for r in getattr(extensions, cls)(
+            "HIP", "float64", mode="split32").calc(ps, ps, *sc)]
+        for grp in KERNELS[name][3]:
+            g = [got[k] for k in grp]
+            r = [ref[k] for k in grp]
+            assert np.all(np.isfinite(np.stack(g))), name
+            e_close = vec_rel_err([x[close] for x in g], [x[close] for x in r])
+            rest = np.ones(n, bool)
+            rest[close] = False
+            e_rest = vec_rel_err([x[rest] for x in g], [x[rest] for x in r])
+            print("%s n=%d: close rows %.2e, others %.2e" % (name, n, e_close,
+                                                             e_rest))
+            assert e_close <= 1e-12, (name, e_close)
+            assert e_rest <= 1e-5, (name, e_rest)

@@ -48,6 +48,58 @@ struct SoftType<Op, true> {
     using type = decltype(Op::slot_e2(std::declval<const typename Op::Slot&>()));
 };
 
+// FP64 fallback of the split-FP32 interactions.  An unsoftened close pair's
+// terms (m / R^3, m^2 / R^6 ...) overflow FP32 long before FP64, and an
+// overflow in any term reaches the sums as an infinity or a NaN (every pair
+// term is a product with the overflowing factor), so a finite result needs
+// no second look and the test costs a few instructions per i, not per pair.
+// An i whose sums come out non-finite is evaluated again by the FP64
+// interaction of the same kernel on the FP64 inputs:
+//   Op::Fallback64      in the pair kernel's epilogue, over the same j-range
+//                       (Acc, AccJerk, Tstep: their FP64 loops fit in the
+//                       registers the FP32 loop already holds);
+//   Op::PostFallback64  by fallback_kernel after the call, over every j, one
+//                       thread per i (PNAcc, whose FP64 polynomial inlined
+//                       in the pair kernel would cost it occupancy).
+// Non-finite inputs give non-finite FP64 sums, as in the FP64 kernel.
+template <class Op, class = void>
+struct HasFallback : std::false_type {};
+template <class Op>
+struct HasFallback<Op, std::void_t<typename Op::Fallback64>> : std::true_type {};
+template <class Op, class = void>
+struct HasPostFallback : std::false_type {};
+template <class Op>
+struct HasPostFallback<Op, std::void_t<typename Op::PostFallback64>> : std::true_type {};
+
+template <int N>
+__device__ __forceinline__ bool all_finite(const double (&t)[N])
+{
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < N; ++k) ok = ok && __builtin_isfinite(t[k]);
+    return ok;
+}
+
+// tot = the sums of i over j = j0, j0 + jstep, ... below j1 by the FP64
+// interaction F on the FP64 inputs.
+template <class F>
+__device__ void fallback64(const typename F::Args& a, int i, int j0, int j1, int jstep,
+                           double* tot)
+{
+    const typename F::Frame fr = F::frame(a, 0);
+    typename F::IReg r;
+    typename F::Acc acc;
+    F::load_i(a, fr, i, r);
+    F::zero(acc);
+    for (int j = j0; j < j1; j += jstep) {
+        typename F::Slot s;
+        F::load_j(a, fr, j, s);
+        F::interact(r, s, acc);
+    }
+    F::flush(acc);
+    for (int k = 0; k < F::kNacc; ++k) tot[k] = F::get(acc, k);
+}
+
 // One tile's interactions: UNROLL slots at a time, each applied to the WPT
 // i-particles, accumulators flushed every CH slots.
 template <class Op, int WPT, int UNROLL, int CH, bool kU, class E2>
@@ -158,6 +210,9 @@ __global__ __launch_bounds__(BLOCK) void pair_kernel(typename Op::Args a)
         double tot[Op::kNacc];
 #pragma unroll
         for (int k = 0; k < Op::kNacc; ++k) tot[k] = Op::get(acc[w], k);
+        if constexpr (HasFallback<Op>::value) {
+            if (!all_finite(tot)) fallback64<typename Op::Fallback64>(a, i, j0, j1, 1, tot);
+        }
         if (a.nsplit == 1) {
             Op::store(a, i, tot);
         } else {
@@ -205,6 +260,11 @@ __global__ __launch_bounds__(BLOCK) void pair_small_kernel(typename Op::Args a)
     double tot[Op::kNacc];
 #pragma unroll
     for (int k = 0; k < Op::kNacc; ++k) tot[k] = Op::get(acc, k);
+    if constexpr (HasFallback<Op>::value) {
+        // lane g took the j with j % G == g (TILE is a multiple of G)
+        if (i < a.ni && !all_finite(tot))
+            fallback64<typename Op::Fallback64>(a, i, g, a.nj, G, tot);
+    }
 #pragma unroll
     for (int off = 1; off < G; off <<= 1) {
         double other[Op::kNacc];
@@ -310,6 +370,23 @@ __global__ __launch_bounds__(256) void reduce_wide_kernel(typename Op::Args a)
         }
     }
     if (valid && t == 0) Op::store(a, i, tot);
+}
+
+// Op::PostFallback64 after a call: every i whose stored outputs (the sums,
+// for the ops that have one) are non-finite, again in FP64 over every j.
+template <class Op>
+__global__ __launch_bounds__(256) void fallback_kernel(typename Op::Args a)
+{
+    using F = typename Op::PostFallback64;
+    static_assert(F::kNacc == Op::kNacc, "fallback must produce the same sums");
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i >= a.ni) return;
+    double tot[Op::kNacc];
+#pragma unroll
+    for (int k = 0; k < Op::kNacc; ++k) tot[k] = static_cast<double>(a.out[k][i]);
+    if (all_finite(tot)) return;
+    fallback64<F>(a, i, 0, a.nj, 1, tot);
+    Op::store(a, i, tot);
 }
 
 // Launch geometry of one all-pairs call.

@@ -559,6 +559,8 @@ template <bool kRefine>
 struct OpSplitAccT {
     using Args = PairArgs<double>;
     using Frame = SplitFrame;
+    // sums that overflow FP32 are recomputed in FP64 (engine.h: fallback64)
+    using Fallback64 = OpAcc<Fast64>;
     // xh yh zh m | xl yl zl e2
     using Slot = Slots<float, 8>;
     struct IReg { float xh, yh, zh, xl, yl, zl, e2; };
@@ -635,6 +637,7 @@ template <bool kRefine>
 struct OpSplitAccJerkT {
     using Args = PairArgs<double>;
     using Frame = SplitFrame;
+    using Fallback64 = OpAccJerk<Fast64>;
     // xh yh zh m | xl yl zl e2 | vx vy vz pad
     using Slot = Slots<float, 12>;
     struct IReg { float xh, yh, zh, xl, yl, zl, e2, vx, vy, vz; };
@@ -731,12 +734,15 @@ struct OpSplitAccJerkT {
 // Split-FP32 Tstep: the pair terms of OpTstep in FP32 on split relative
 // coordinates (frame as for AccJerk), the sum of w^2 flushed into FP64 every
 // kFlush j-particles, the maximum kept in FP32 (it is one pair's term, so it
-// carries that term's FP32 error and no accumulated one).  The FP32 range
-// bounds the pair terms: ga ~ m^2 / R^6, so R must stay above ~1e-6 in N-body
-// units for total masses near 1 (softening of 1e-4 guarantees it).
+// carries that term's FP32 error and no accumulated one).  The pair terms
+// reach ga ~ m^2 / R^6, beyond FP32 for R below ~1e-6 in N-body units with
+// total masses near 1 (a softening of 1e-4 keeps every R above it); an i
+// whose FP32 sums overflow there is evaluated again by OpTstep<Fast64>
+// (engine.h: fallback64).
 struct OpSplitTstep {
     using Args = PairArgs<double>;
     using Frame = SplitFrame;
+    using Fallback64 = OpTstep<Fast64>;
     // xh yh zh m | xl yl zl e2 | vx vy vz pad
     using Slot = Slots<float, 12>;
     struct IReg { float xh, yh, zh, xl, yl, zl, e2, m, vx, vy, vz, eta; };

@@ -54,6 +54,18 @@ struct FixedScratch {
     size_t doubles;
 };
 
+// The FP64 pass of the ops whose FP32 overflow fallback runs after the call.
+template <class Op>
+static void launch_post_fallback(const typename Op::Args& a, hipStream_t stream)
+{
+    if constexpr (HasPostFallback<Op>::value) {
+        if (a.nj > 0) {
+            const unsigned g = static_cast<unsigned>((a.ni + 255) / 256);
+            hipLaunchKernelGGL((fallback_kernel<Op>), dim3(g), dim3(256), 0, stream, a);
+        }
+    }
+}
+
 template <class Op, int BLOCK, int WPT, int TILE, int UNROLL>
 static int run_pairwise(typename Op::Args a, hipStream_t stream, int wg_per_cu,
                         const FixedScratch* fixed)
@@ -70,6 +82,7 @@ static int run_pairwise(typename Op::Args a, hipStream_t stream, int wg_per_cu,
         hipLaunchKernelGGL((pair_small_kernel<Op, kSmallBlock, kSmallGroups, small_tile<Op>()>),
                            dim3(grid), dim3(kSmallBlock), 0, stream, a);
         if (!fixed) timing_end(stream);
+        launch_post_fallback<Op>(a, stream);
         return check_launch("pairwise");
     }
     const Device& dev = current_device();
@@ -117,6 +130,7 @@ static int run_pairwise(typename Op::Args a, hipStream_t stream, int wg_per_cu,
             hipLaunchKernelGGL((reduce_kernel<Op>), dim3(rg), dim3(256), 0, stream, a);
         }
     }
+    launch_post_fallback<Op>(a, stream);
     return check_launch("pairwise");
 }
 

@@ -437,12 +437,17 @@ struct OpPNAcc {
 // A r + B v is summed in FP32 and flushed into FP64 every kFlush j-particles.
 // The PN acceleration is a correction of relative size (v/c)^2 to the
 // Newtonian one, so its FP32 relative error (~1e-6) is ~1e-10 of the total
-// force for v/c ~ 0.01.  FP32 range: the 3PN terms reach m^4/R^5, so R must
-// stay above ~1e-6 in N-body units.
+// force for v/c ~ 0.01.  FP32 range: the 3PN terms reach m^4/R^5, beyond
+// FP32 for R below ~1e-6 in N-body units; an i whose FP32 sums overflow is
+// evaluated again by OpPNAcc<Fast64> after the call (engine.h:
+// fallback_kernel).
 // ---------------------------------------------------------------------------
 struct OpSplitPNAcc {
     using Args = PairArgs<double>;
     using Frame = SplitFrame;
+    // sums that overflow FP32 are recomputed in FP64 after the call
+    // (engine.h: fallback_kernel)
+    using PostFallback64 = OpPNAcc<Fast64>;
     using P = Fast32;
     // xh yh zh m | xl yl zl e2 | ux uy uz v2 | vx vy vz m2 | v4 pad pad pad
     using Slot = Slots<float, 20>;

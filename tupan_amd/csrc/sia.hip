@@ -113,6 +113,9 @@ __device__ void pair_op(const SubtreeArgs& a, int ilo, int ni, int jlo, int nj, 
         double tot[Op::kNacc];
 #pragma unroll
         for (int k = 0; k < Op::kNacc; ++k) tot[k] = Op::get(acc, k);
+        if constexpr (HasFallback<Op>::value) {
+            if (!all_finite(tot)) fallback64<typename Op::Fallback64>(pa, lane, 0, nj, 1, tot);
+        }
         Op::store(pa, lane, tot);
     }
     __syncthreads();
