@@ -94,7 +94,10 @@ def test_plummer_1024_ahermite4_energy():
         it.evolve_step(1.0 / 16)
         nstep += 1
     e1 = energy(it.particle_system)
-    assert abs((e1 - e0) / e0) < 1e-6
+    err = abs((e1 - e0) / e0)
+    print("ahermite4 N=1024 to t=1/16: %d steps, |dE/E| = %.3e" % (nstep,
+                                                                  err))
+    assert err < 1e-12
     assert nstep > 5
 
 
@@ -142,8 +145,8 @@ def test_sia_device_subtree_equals_host_levels(method, n, mode):
 
 def test_plummer_1024_ahermite4_to_t1():
     """The 1024-particle workload in full: Plummer, eps = 4/N, ahermite4
-    with eta = 0.01 to t = 1; |dE/E| at t = 1 stays below 1e-6 and every
-    particle ends at t = 1."""
+    with eta = 0.01 to t = 1; |dE/E| at t = 1 stays below 1e-11 (measured
+    1.3e-13 on an MI355X) and every particle ends at t = 1."""
     from tupan_amd.diagnostics import energy
     from tupan_amd.ics.plummer import make_plummer
     from tupan_amd.integrator import Integrator
@@ -162,7 +165,7 @@ def test_plummer_1024_ahermite4_to_t1():
     print("ahermite4 N=1024 to t=1: %d steps, |dE/E| = %.3e" % (nstep, err))
     assert it.time == 1.0
     assert torch.all(q.time == 1.0)
-    assert err < 1e-6
+    assert err < 1e-11
 
 
 def test_sia_pn_gpu_matches_oracle_backend():
@@ -272,6 +275,56 @@ def test_device_hermite_bad_control_raises():
     for k in ("rx", "vx", "time", "nstep"):
         np.testing.assert_array_equal(q[k], host[k] if k in host else q[k])
     assert int(q["nstep"].max()) == 0
+
+
+# Three unsoftened bodies (found by a search with the oracle) whose Tstep at
+# eta = 2 is NaN for body 1 only: its receding partner makes its w^2 sum fall
+# below -1, so dt = eta / sqrt(1 + sum) has no value, while bodies 0 and 2
+# get finite steps.
+NAN_TSTEP_BODIES = {
+    "mass": [0.0050455183982152825, 0.0029851304450925534,
+             0.001907530456191219],
+    "rx": [0.0, 0.05803323859868507, 0.029869613281892263],
+    "ry": [0.0, 0.0, 0.006719948779563594],
+    "rz": [0.0, 0.0, 0.0],
+    "eps2": [0.0, 0.0, 0.0],
+    "vx": [3.1669457576042443, 2.6407219741636783, 1.2667052456498304],
+    "vy": [-2.2035098806466507, 0.05202897425988651, 0.6836861907765345],
+    "vz": [0.0, 0.0, 0.0],
+}
+
+
+@pytest.mark.parametrize("mode", ["fp64", "split32"])
+def test_device_hermite_nan_tstep_raises(mode):
+    """A NaN Tstep of one particle stops a device ahermite step (status 2,
+    TupanHipError), as the host driver's min over the steps propagates the
+    NaN and fails; the particles are untouched.  A step-size reduction that
+    skipped the NaN would take a step from the other two particles."""
+    from oracle import Oracle
+    from tupan_amd import _native
+    from tupan_amd.integrator import Integrator
+    from tupan_amd.particles import ParticleSystem
+    host = {k: np.array(v) for k, v in NAN_TSTEP_BODIES.items()}
+    ts = Oracle("float64").tstep(host, host, 2.0)[0]
+    assert np.isnan(ts[1]) and np.all(np.isfinite(ts[[0, 2]]))
+    install_hip("float64", mode=mode)
+    try:
+        ps = ParticleSystem.from_numpy(host, device="cuda:0")
+        type(ps).t_curr = 0.0
+        it = Integrator(2.0, 0.0, ps, method="ahermite4", device_steps=True)
+        with pytest.raises(_native.TupanHipError):
+            it.evolve_step(8.0)
+        q = it.particle_system.to_numpy()
+        for k in ("rx", "vx"):
+            np.testing.assert_array_equal(q[k], host[k])
+        assert int(q["nstep"].max()) == 0
+        ps = ParticleSystem.from_numpy(host, device="cuda:0")
+        type(ps).t_curr = 0.0
+        it = Integrator(2.0, 0.0, ps, method="ahermite4")
+        with pytest.raises(ValueError):
+            it.evolve_step(8.0)
+    finally:
+        install_hip("float64")
 
 
 def _sakura_run(host, t_end, mode, method="sakura", eta=1.0 / 64,

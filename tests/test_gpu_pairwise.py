@@ -190,14 +190,15 @@ def test_split32_golden(golden):
             assert float(np.max(d / sa)) <= 8 * 2.0**-24
 
 
-def tstep_error_scale(hi, hj, eta):
+def tstep_error_scale(hi, hj, eta, extra_speed=0.0):
     """First-order FP32 error model of split-FP32 Tstep, per i-particle, for
     tstep and tstepij (relative, in units of 2^-24).  Each pair term
     w = w0 - ga*rv carries errors of a few ulps of the magnitudes entering
     it: |w0| with the velocity difference widened by U = |v_i - v_0| +
     |v_j - v_0| (velocities are rounded to FP32 relative to the first
     i-particle v_0), and |ga| |r| (|u| + U); tstep = eta / sqrt(1 + S) then
-    halves the relative error of 1 + S."""
+    halves the relative error of 1 + S.  extra_speed widens U further (a
+    bound for a frame velocity that hi[...][0] does not hold)."""
     v0 = np.array([hi[k][0] for k in ("vx", "vy", "vz")])
     vj = np.stack([hj[k] for k in ("vx", "vy", "vz")]).T
     sc_t, sc_m = [], []
@@ -205,7 +206,8 @@ def tstep_error_scale(hi, hj, eta):
         d = np.stack([hi[k][i] - hj[k] for k in ("rx", "ry", "rz")])
         u = np.stack([hi[k][i] - hj[k] for k in ("vx", "vy", "vz")])
         vi = np.array([hi[k][i] for k in ("vx", "vy", "vz")])
-        U = np.linalg.norm(vi - v0) + np.linalg.norm(vj - v0, axis=1)
+        U = (np.linalg.norm(vi - v0) + np.linalg.norm(vj - v0, axis=1)
+             + extra_speed)
         r2 = (d * d).sum(0)
         mask = r2 > 0
         with np.errstate(all="ignore"):
@@ -352,6 +354,34 @@ def test_host_pointer_abi(golden):
                      ctypes.c_ulong(nj), *[p(B[k]) for k in FIELDS[:8]],
                      ctypes.c_ulong(7), *inv, *[p(o) for o in out3])
     assert_close("pnacc", out3, g, "s1_pnacc_AB", 1e-13)
+    # the other four names of libtupan.h: phi, snap_crackle, nreg_X, nreg_V
+    out4 = [np.zeros(ni)]
+    lib.phi_kernel.restype = None
+    lib.phi_kernel(ctypes.c_ulong(ni), *[p(A[k]) for k in FIELDS[:5]],
+                   ctypes.c_ulong(nj), *[p(B[k]) for k in FIELDS[:5]],
+                   *[p(o) for o in out4])
+    assert_close("phi", out4, g, "s1_phi_AB", 1e-13)
+    out5 = [np.zeros(ni) for _ in range(6)]
+    lib.snap_crackle_kernel.restype = None
+    lib.snap_crackle_kernel(ctypes.c_ulong(ni), *[p(A[k]) for k in FIELDS],
+                            ctypes.c_ulong(nj), *[p(B[k]) for k in FIELDS],
+                            *[p(o) for o in out5])
+    assert_close("snap_crackle", out5, g, "s1_snap_crackle_AB", 1e-13)
+    dt = KERNELS["nreg_x"][1][0]
+    out6 = [np.zeros(ni) for _ in range(7)]
+    lib.nreg_Xkernel.restype = None
+    lib.nreg_Xkernel(ctypes.c_ulong(ni), *[p(A[k]) for k in FIELDS[:8]],
+                     ctypes.c_ulong(nj), *[p(B[k]) for k in FIELDS[:8]],
+                     ctypes.c_double(dt), *[p(o) for o in out6])
+    assert_close("nreg_x", out6, g, "s1_nreg_x_AB", 1e-13)
+    vfields = ("mass", "vx", "vy", "vz", "ax", "ay", "az")
+    out7 = [np.zeros(ni) for _ in range(4)]
+    lib.nreg_Vkernel.restype = None
+    lib.nreg_Vkernel(ctypes.c_ulong(ni), *[p(A[k]) for k in vfields],
+                     ctypes.c_ulong(nj), *[p(B[k]) for k in vfields],
+                     ctypes.c_double(KERNELS["nreg_v"][1][0]),
+                     *[p(o) for o in out7])
+    assert_close("nreg_v", out7, g, "s1_nreg_v_AB", 1e-13)
 
 
 def test_host_pointer_abi_f32(golden):
@@ -511,6 +541,63 @@ def test_block_step_size_salpeter_rows_vs_oracle(mode):
         f = float(np.sum(m * full[k]))
         amag = float(np.sum(m * np.abs(full[k])))
         assert abs(f) <= (1e-12 if mode == "fp64" else 1e-6) * amag, k
+
+
+@pytest.mark.parametrize("mode", ["fp64", "split32"])
+def test_block_step_workload_tstep_rows_vs_oracle(mode):
+    """Tstep (eta = 0.02) on the block-step workload itself: Plummer
+    N = 262144, Salpeter masses (slope -2.35 on [0.1, 10], total mass 1),
+    eps = 1e-4, seed 1.  The whole set and an uneven contiguous i-subset
+    (one level, a view into the same arrays) against all j; sampled rows
+    against the oracle: FP64 within 1e-13 relative; split-FP32 within 1e-6
+    relative and within 8 units of the first-order FP32 error model
+    (tstep_error_scale, with the frame velocity bounded by the largest
+    speed, since the workgroup's first i-particle is not known here)."""
+    from oracle import Oracle
+    from tupan_amd.ics.imf import IMF
+    from tupan_amd.ics.plummer import make_plummer
+    from tupan_amd.lib import extensions
+    n, eta = 262144, 0.02
+    ps = make_plummer(n, 1e-4, IMF.power_law(-2.35, 0.1, 10.0), seed=1,
+                      device="cuda:0")
+    d = ps.to_numpy()
+    ext = extensions.Tstep("HIP", "float64", mode=mode)
+    full = [r.detach().cpu().numpy().copy() for r in ext.calc(ps, ps, eta)]
+    lo, hi = 70001, 70001 + 23459
+    part = [r.detach().cpu().numpy().copy()
+            for r in ext.calc(ps[lo:hi], ps, eta)]
+    keys = ("mass", "rx", "ry", "rz", "eps2", "vx", "vy", "vz")
+    dj = {k: d[k] for k in keys}
+    vmax = float(np.max(np.sqrt(d["vx"]**2 + d["vy"]**2 + d["vz"]**2)))
+    rng = np.random.default_rng(7)
+    for got, rows, off in ((full, rng.choice(n, 24, replace=False), 0),
+                           (part, lo + rng.choice(hi - lo, 12,
+                                                  replace=False), lo)):
+        di = {k: d[k][rows] for k in keys}
+        ref = Oracle("float64").tstep(di, dj, eta)
+        for k, o in enumerate(("tstep", "tstepij")):
+            e = np.abs(got[k][rows - off] - ref[k]) / ref[k]
+            print("N = %d %s %s rows at %d: max rel %.3e"
+                  % (n, mode, o, off, e.max()))
+            if mode == "fp64":
+                assert float(e.max()) <= 1e-13, (o, off)
+            else:
+                assert float(e.max()) <= 1e-6, (o, off)
+        if mode == "split32":
+            # frame velocity v0 unknown: |v_i - v0| + |v_j - v0| is at most
+            # |v_i| + |v_j| + 2 vmax; the model takes v0 = 0 and the
+            # widened speeds through its U term
+            hi_ = dict(di)
+            for c in ("vx", "vy", "vz"):
+                hi_[c] = np.concatenate([[0.0], di[c]])
+            for c in ("mass", "rx", "ry", "rz", "eps2"):
+                hi_[c] = np.concatenate([[di[c][0]], di[c]])
+            sc = tstep_error_scale(hi_, dj, eta, extra_speed=2 * vmax)
+            for k in range(2):
+                e = np.abs(got[k][rows - off] - ref[k]) / ref[k]
+                units = float(np.max(e / (sc[k][1:] * 2.0**-24)))
+                print("  error model units (%s): %.2f" % (k, units))
+                assert units <= 8, (k, off, units)
 
 
 def test_split_combine_order():

@@ -5,8 +5,13 @@ seed 1) through size-independent properties:
     the reference C backend): <= 1e-13 per particle, normalised by |a_i|;
   * FP64 rows against the CPU oracle on a sample of particles;
   * split-FP32 against FP64: the per-particle relative force error
-    |da_i|/|a_i| has median <= 1e-7 and 99.9th percentile <= 1e-6; the
-    jerk's median <= 1e-6 (the jerk is a derivative and cancels harder);
+    e_i = |da_i|/|a_i|.  The 1e-7 force-error target is judged on its root
+    mean square over the particles (the usual N-body force-accuracy
+    measure), which must be <= 1e-7; beside it the median <= 5e-8, the
+    99.9th percentile <= 5e-7 and the maximum <= 5e-6 (measured on an
+    MI355X: median 1.4e-8, 99.9th percentile 2.7e-7, maximum 1.7e-6; the
+    largest values sit where the net force nearly cancels); the jerk's
+    median <= 1e-6 (the jerk is a derivative and cancels harder);
   * split-FP32 forces carry no common scale error (virial within 2e-9 of
     FP64's), the condition for |dE/E| <= 1e-10 over one N-body time;
   * split-FP32 Tstep against FP64: relative error median <= 1e-7, 99.9th
@@ -66,12 +71,15 @@ def test_split32_force_error(plummer64k):
     split = accjerk(plummer64k, "split32")
     ea = per_particle(split, fast, (0, 1, 2))
     ej = per_particle(split, fast, (3, 4, 5))
-    print("split32 |da|/|a|: median %.3e p99.9 %.3e max %.3e"
-          % (np.median(ea), np.percentile(ea, 99.9), ea.max()))
+    rms = float(np.sqrt(np.mean(ea * ea)))
+    print("split32 |da|/|a|: rms %.3e median %.3e p99.9 %.3e max %.3e"
+          % (rms, np.median(ea), np.percentile(ea, 99.9), ea.max()))
     print("split32 |dj|/|j|: median %.3e p99.9 %.3e max %.3e"
           % (np.median(ej), np.percentile(ej, 99.9), ej.max()))
-    assert np.median(ea) <= 1e-7
-    assert np.percentile(ea, 99.9) <= 1e-6
+    assert rms <= 1e-7
+    assert np.median(ea) <= 5e-8
+    assert np.percentile(ea, 99.9) <= 5e-7
+    assert ea.max() <= 5e-6
     assert np.median(ej) <= 1e-6
 
 
