@@ -89,7 +89,32 @@ def parse():
                          "(counter calibration under rocprofv3 --pmc)")
     ap.add_argument("--modes", default="split32,fp64,ieee64",
                     help="force modes to time (the first is the headline)")
+    ap.add_argument("--skip", default="",
+                    help="comma-separated parts to leave out: sakura_steps, "
+                         "ensemble_graph, force_error")
     return ap.parse_args()
+
+
+def cpu_model():
+    """The host CPU's model name (first processor in /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_kernels():
+    """(timed library, kind): the reference's own C backend compiled from its
+    sources into oracle/_ref when that build is present ("reference"), else
+    the oracle, its restatement in the same operation order ("port")."""
+    from oracle import Oracle, Reference, reference_available
+    if reference_available("float64"):
+        return Reference("float64"), "reference"
+    return Oracle("float64"), "port"
 
 
 def log(msg):
@@ -131,6 +156,42 @@ def time_kernel(ps, cls_name, mode, steps, *scalars):
     _native.timing(False)
     t = kms / 1e3 / max(cnt, 1)
     return {"kernel_ms": t * 1e3, "gint_s": ps.n * ps.n / t / 1e9}
+
+
+def force_error(ps, rows=32):
+    """Split-FP32 AccJerk against the FP64 path on the same state, per
+    particle |da_i| / |a_i| and |dj_i| / |j_i|: median, 99.9th percentile,
+    maximum and root mean square; and the FP64 path against the oracle
+    (the reference C backend's arithmetic) on `rows` sampled rows."""
+    from oracle import Oracle
+    from tupan_amd.lib import extensions
+    got = {}
+    for mode in ("split32", "fp64"):
+        res = extensions.AccJerk("HIP", "float64", mode=mode).calc(ps, ps)
+        got[mode] = [r.detach().cpu().numpy().copy() for r in res]
+
+    def per_particle(a, b, grp):
+        g = np.stack([a[k] for k in grp])
+        r = np.stack([b[k] for k in grp])
+        return np.sqrt(((g - r)**2).sum(0)) / np.sqrt((r * r).sum(0))
+
+    def stats(e):
+        return {"median": float(np.median(e)),
+                "p99.9": float(np.percentile(e, 99.9)),
+                "max": float(e.max()), "rms": float(np.sqrt(np.mean(e * e)))}
+
+    out = {"vs": "FP64 path on the same state",
+           "acc": stats(per_particle(got["split32"], got["fp64"], (0, 1, 2))),
+           "jerk": stats(per_particle(got["split32"], got["fp64"],
+                                      (3, 4, 5)))}
+    host = ps.to_numpy()
+    idx = np.random.default_rng(1).choice(ps.n, rows, replace=False)
+    ref = Oracle("float64").acc_jerk({k: v[idx] for k, v in host.items()},
+                                     host)
+    e64 = per_particle([g[idx] for g in got["fp64"][:3]], ref[:3],
+                       (0, 1, 2))
+    out["fp64_vs_oracle"] = {"rows": rows, "max": float(e64.max())}
+    return out
 
 
 def energy_run(ps0, mode, steps):
@@ -291,7 +352,7 @@ def sakura_forms(n, net_path, reps=3):
     return out
 
 
-def ensemble_rate(nsys, lanes):
+def ensemble_rate(nsys, lanes, graph=True):
     """The ensemble workload: nsys Plummer systems with sizes U[3,64] and
     log-uniform masses in [0.1, 1], Sakura with dt = 1/64 to t = 1, every
     pair-drift stage one launch over all systems.  Timed twice from the same
@@ -325,18 +386,20 @@ def ensemble_rate(nsys, lanes):
     counts = ens.counters.cpu().tolist()
     ens.counters = None
     # the same 64 steps as replays of one captured 8-step HIP graph
-    gdrv = SakuraEnsemble(ps_init, ens)
-    graph = gdrv.capture(1.0 / 64, 8)
-    ens.reset_status()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(drv.nsteps // 8):
-        graph.replay()
-    torch.cuda.synchronize()
-    wall_graph = time.perf_counter() - t0
-    ens.check_status()
-    same = all(torch.equal(getattr(ps, k), getattr(ps_init, k))
-               for k in ("rx", "ry", "rz", "vx", "vy", "vz"))
+    wall_graph = same = None
+    if graph:
+        gdrv = SakuraEnsemble(ps_init, ens)
+        g = gdrv.capture(1.0 / 64, 8)
+        ens.reset_status()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(drv.nsteps // 8):
+            g.replay()
+        torch.cuda.synchronize()
+        wall_graph = time.perf_counter() - t0
+        ens.check_status()
+        same = all(torch.equal(getattr(ps, k), getattr(ps_init, k))
+                   for k in ("rx", "ry", "rz", "vx", "vy", "vz"))
     solves, iters, retries, leap = [int(x) for x in counts]
     sizes = np.diff(ens.offsets)
     pairs = int(np.sum(sizes.astype(np.int64)**2))
@@ -436,26 +499,32 @@ def _best(fn, reps):
 
 
 def cpu_sakura(n, rows):
-    """Serial oracle Sakura (flag -1, dt = 1/64) on `rows` i-rows of the
-    Plummer N = n state against all n j; Kepler solves/s and pairs/s."""
+    """Serial Sakura (flag -1, dt = 1/64) of the reference C backend (or
+    the oracle) on `rows` i-rows of the Plummer N = n state against all n j;
+    Kepler solves/s and pairs/s.  The solve count is the oracle's on the
+    same sample (it makes the reference's branch decisions)."""
     from oracle import Oracle
     from tupan_amd.ics.plummer import make_plummer
     host = make_plummer(n, 0.0, ("equalmass",), seed=1,
                         device="cuda:0").to_numpy()
     sub = {k: v[:rows] for k, v in host.items()}
+    lib, kind = cpu_kernels()
+    t = _best(lambda: lib.sakura(sub, host, 1.0 / 64, -1), 2)
     orc = Oracle("float64")
     orc.counters_reset()
-    t = _best(lambda: orc.sakura(sub, host, 1.0 / 64, -1), 1)
+    orc.sakura(sub, host, 1.0 / 64, -1)
     solves = orc.counters()[0]
     return {"value": solves / t / 1e6, "unit": "M Kepler solves/s",
-            "pairs_per_s": rows * n / t, "cores": 1, "kind": "port",
-            "sample": "%d i-rows x %d j, Sakura flag -1, dt 1/64 (%.2f s, "
-                      "%d solves)" % (rows, n, t, solves)}
+            "pairs_per_s": rows * n / t, "cores": 1, "kind": kind,
+            "cpu_model": cpu_model(),
+            "sample": "%d i-rows x %d j, Sakura flag -1, dt 1/64, best of 2 "
+                      "(%.2f s, %d solves)" % (rows, n, t, solves)}
 
 
 def cpu_block_steps(n, rows, mix):
-    """Serial oracle Tstep and Acc on `rows` i-rows of the block-step
-    workload's initial state against all n j; the step's interaction mix
+    """Serial Tstep and Acc of the reference C backend (or the oracle) on
+    `rows` i-rows of the block-step workload's initial state against all n
+    j; the step's interaction mix
     `mix` ({"Tstep": pairs, "Acc": pairs}) at those rates, in Gint/s."""
     from oracle import Oracle
     from tupan_amd.ics.imf import IMF
@@ -463,42 +532,48 @@ def cpu_block_steps(n, rows, mix):
     host = make_plummer(n, 1e-4, IMF.power_law(-2.35, 0.1, 10.0), seed=1,
                         device="cuda:0").to_numpy()
     sub = {k: v[:rows] for k, v in host.items()}
-    orc = Oracle("float64")
-    tt = _best(lambda: orc.tstep(sub, host, 0.02), 1)
-    ta = _best(lambda: orc.acc(sub, host), 1)
+    lib, kind = cpu_kernels()
+    tt = _best(lambda: lib.tstep(sub, host, 0.02), 1)
+    ta = _best(lambda: lib.acc(sub, host), 1)
     rt, ra = rows * n / tt, rows * n / ta
     pt, pa = mix.get("Tstep", 0), mix.get("Acc", 0)
     step_s = pt / rt + pa / ra
     return {"value": (pt + pa) / step_s / 1e9, "unit": "Gint/s", "cores": 1,
-            "kind": "port", "tstep_gint_s": rt / 1e9, "acc_gint_s": ra / 1e9,
+            "kind": kind, "cpu_model": cpu_model(),
+            "tstep_gint_s": rt / 1e9, "acc_gint_s": ra / 1e9,
             "sample": "%d i-rows x %d j of Tstep (%.2f s) and Acc (%.2f s), "
                       "weighted by the step's interaction counts"
                       % (rows, n, tt, ta)}
 
 
 def cpu_ensemble(nsys):
-    """Serial oracle Sakura pair drifts (flags -1 and +1, dt 1/128) on the
-    first `nsys` systems of the ensemble workload's initial state, system by
-    system; Kepler solves/s."""
+    """Serial Sakura pair drifts of the reference C backend (or the oracle)
+    (flags -1 and +1, dt 1/128) on the first `nsys` systems of the ensemble
+    workload's initial state, system by system; Kepler solves/s (solve
+    count from the oracle on the same calls)."""
     from oracle import Oracle
     from tupan_amd.ics.ensemble import make_ensemble
     ps, ens = make_ensemble(nsys=4096, nmin=3, nmax=64, seed=1,
                             device="cuda:0")
     host = ps.to_numpy()
     off = ens.offsets
+    subs = [{k: v[off[s]:off[s + 1]] for k, v in host.items()}
+            for s in range(nsys)]
+    lib, kind = cpu_kernels()
+
+    def run(impl):
+        for sub in subs:
+            for flag in (-1, 1):
+                impl.sakura(sub, sub, 1.0 / 128, flag)
+    t = _best(lambda: run(lib), 2)
     orc = Oracle("float64")
     orc.counters_reset()
-    t0 = time.perf_counter()
-    for s in range(nsys):
-        sub = {k: v[off[s]:off[s + 1]] for k, v in host.items()}
-        for flag in (-1, 1):
-            orc.sakura(sub, sub, 1.0 / 128, flag)
-    t = time.perf_counter() - t0
+    run(orc)
     solves = orc.counters()[0]
     return {"value": solves / t / 1e6, "unit": "M Kepler solves/s",
-            "cores": 1, "kind": "port",
-            "sample": "first %d of 4096 systems, 2 pair-drift calls each "
-                      "(%.2f s, %d solves)" % (nsys, t, solves)}
+            "cores": 1, "kind": kind, "cpu_model": cpu_model(),
+            "sample": "first %d of 4096 systems, 2 pair-drift calls each, "
+                      "best of 2 (%.2f s, %d solves)" % (nsys, t, solves)}
 
 
 def hermite_run(n, eta, t_end, host_steps):
@@ -550,20 +625,21 @@ def hermite_run(n, eta, t_end, host_steps):
 
 
 def cpu_hermite(n, eta):
-    """Serial oracle force work of one ahermite4 step (Tstep + 3 AccJerk on
-    all n x n pairs) of the Hermite workload's initial state; steps/s."""
-    from oracle import Oracle
+    """Serial force work of one ahermite4 step (Tstep + 3 AccJerk on all
+    n x n pairs) of the Hermite workload's initial state, by the reference
+    C backend (or the oracle); steps/s."""
     from tupan_amd.ics.plummer import make_plummer
     host = make_plummer(n, 4.0 / n, ("equalmass",), seed=1,
                         device="cuda:0").to_numpy()
-    orc = Oracle("float64")
+    lib, kind = cpu_kernels()
 
     def step():
-        orc.tstep(host, host, eta)
+        lib.tstep(host, host, eta)
         for _ in range(3):
-            orc.acc_jerk(host, host)
+            lib.acc_jerk(host, host)
     t = _best(step, 3)
-    return {"value": 1.0 / t, "unit": "steps/s", "cores": 1, "kind": "port",
+    return {"value": 1.0 / t, "unit": "steps/s", "cores": 1, "kind": kind,
+            "cpu_model": cpu_model(),
             "sample": "Tstep + 3 AccJerk of the N = %d state, best of 3 "
                       "(%.3f s per step)" % (n, t)}
 
@@ -601,11 +677,11 @@ def cpu_baseline(ps, rows):
     rate_o = rows_omp * n / best_o
     del np
     return ({"value": rate / 1e9, "unit": "Gint/s", "cores": 1,
-             "kind": kind,
+             "kind": kind, "cpu_model": cpu_model(),
              "sample": "%d i-rows x %d j of the same Plummer state, FP64, "
                        "best of 3 (%.2f s)" % (rows, n, best)},
             {"value": rate_o / 1e9, "unit": "Gint/s", "cores": threads,
-             "kind": "port",
+             "kind": "port", "cpu_model": cpu_model(),
              "sample": "%d i-rows x %d j, OpenMP over i, best of 2 (%.2f s)"
                        % (rows_omp, n, best_o)})
 
@@ -636,6 +712,7 @@ def main():
     pairs = float(n) * n
 
     modes = args.modes.split(",")
+    skip = set(x for x in args.skip.split(",") if x)
     res = {}
     for mode in modes:
         k = args.steps if mode != "ieee64" else max(2, args.steps // 4)
@@ -693,6 +770,10 @@ def main():
                         "frac": flops / k_fp64 / PEAK_FP64},
     }
 
+    if "force_error" not in skip and "split32" in res and "fp64" in res:
+        log("bench: split-FP32 force error against FP64")
+        out["force_error"] = force_error(ps)
+
     out["other_kernels"] = {
         "%s_%s" % (name.lower(), mode): time_kernel(ps, name, mode, 5, *sc)
         for name, sc in (("Acc", ()), ("Tstep", (0.01,)))
@@ -719,14 +800,16 @@ def main():
     if args.sakura_n > 0:
         log("bench: Sakura forms and learned guess N=%d" % args.sakura_n)
         out["sakura_forms"] = sakura_forms(args.sakura_n, args.learned_net)
-        log("bench: Sakura driver steps N=%d, host-driven and graphs"
-            % args.sakura_n)
-        out["sakura_steps"] = sakura_steps(args.sakura_n)
+        if "sakura_steps" not in skip:
+            log("bench: Sakura driver steps N=%d, host-driven and graphs"
+                % args.sakura_n)
+            out["sakura_steps"] = sakura_steps(args.sakura_n)
 
     if args.ensemble_systems > 0:
         log("bench: Sakura ensemble of %d systems" % args.ensemble_systems)
-        out["sakura_ensemble"] = ensemble_rate(args.ensemble_systems,
-                                               args.ensemble_lanes)
+        out["sakura_ensemble"] = ensemble_rate(
+            args.ensemble_systems, args.ensemble_lanes,
+            graph="ensemble_graph" not in skip)
 
     if args.sia_n > 0:
         log("bench: block steps N=%d" % args.sia_n)
