@@ -57,6 +57,23 @@ TRAFFIC_65536 = {
 }
 
 
+# Sakura far-pair kernel (flag -1, dt 1/64, Plummer N = 16384, eps = 0):
+# the reference's own operation count for a far pair, one operator = one
+# flop (sakura_kernel_common.h:45-123, 194-243: pair set-up and sums 36,
+# pre-drift 6, far/near test 13, leapfrog 32), and the hardware counts per
+# pair from rocprofv3 (SQ_INSTS_VALU x 64 / pairs, SQ_INSTS_VALU_FLOPS_FP64
+# x 64 / pairs; profiles/sakura_plummer16384_far_counters.md).
+SAKURA_FAR_REF_FLOPS = 87
+SAKURA_FAR_HW = {"valu_per_pair": 126.5, "fp64_flops_per_pair": 127.0}
+# Ensemble Sakura kernel (4096 systems, one pair-drift stage): hardware
+# counts per pair drift from the same profile (nearly every pair is a
+# Kepler solve, whose iterations the reference does not count).
+ENSEMBLE_HW = {"valu_per_pair": 2307.0, "fp64_flops_per_pair": 2090.0}
+CLOCK_PEAK = 2.4e9           # MI355X peak engine clock
+SIMDS = 1024                 # 256 CUs x 4 SIMDs
+FP64_CYCLES_PER_WAVE_INSTR = 4
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -235,15 +252,44 @@ def sakura_rate(n):
     ext.calc(ps, ps, 1.0 / 64, -1)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    kms, _ = _native.timing_read()
+    each = _native.timing_read_each()
+    kms = sum(each)
     _native.timing(False)
     solves, iters, retries, leap = [int(x) for x in
                                     ext.kernel.counters.cpu().tolist()]
-    return {"n": n, "pairs": n * n, "kepler_solves": solves,
-            "laguerre_iterations": iters, "subdivisions": retries,
-            "leapfrog_pairs": leap, "kernel_s": kms / 1e3, "call_s": wall,
-            "solves_per_s": solves / (kms / 1e3),
-            "pairs_per_s": n * n / (kms / 1e3)}
+    out = {"n": n, "pairs": n * n, "kepler_solves": solves,
+           "laguerre_iterations": iters, "subdivisions": retries,
+           "leapfrog_pairs": leap, "kernel_s": kms / 1e3, "call_s": wall,
+           "solves_per_s": solves / (kms / 1e3),
+           "iterations_per_s": iters / (kms / 1e3),
+           "pairs_per_s": n * n / (kms / 1e3)}
+    if len(each) == 2:
+        # two-pass form: the far-pair kernel, then the rest of the call
+        far_s = each[0] / 1e3
+        out["far_kernel_s"] = far_s
+        out["roofline"] = sakura_roofline(n * n, far_s)
+    return out
+
+
+def sakura_roofline(pairs, far_s):
+    """Roofline of the far-pair kernel: FP64 FLOP/s by the reference's
+    operation count and by the hardware's, and the VALU issue rate, against
+    MI355X's vector FP64 peak (one wave instruction per 4 cycles per SIMD)."""
+    alg = SAKURA_FAR_REF_FLOPS * pairs / far_s
+    hw = SAKURA_FAR_HW["fp64_flops_per_pair"] * pairs / far_s
+    issue = SAKURA_FAR_HW["valu_per_pair"] * pairs / 64 / far_s
+    peak_issue = SIMDS * CLOCK_PEAK / FP64_CYCLES_PER_WAVE_INSTR
+    return {"bound": "valu", "kernel": "sakura_far_kernel",
+            "achieved": alg / 1e12, "peak": PEAK_FP64 / 1e12,
+            "unit": "TFLOP/s", "frac": alg / PEAK_FP64,
+            "flops_per_pair": SAKURA_FAR_REF_FLOPS,
+            "flops_counted_as": "the reference's operations per far pair "
+                                "(one operator, division or square root = 1)",
+            "hw_fp64_flops_per_pair": SAKURA_FAR_HW["fp64_flops_per_pair"],
+            "hw_achieved": hw / 1e12, "hw_frac": hw / PEAK_FP64,
+            "valu_instructions_per_pair": SAKURA_FAR_HW["valu_per_pair"],
+            "valu_issue_frac": issue / peak_issue,
+            "traffic": None}
 
 
 def sakura_steps(n, replays=4, per_graph=16):
@@ -403,6 +449,19 @@ def ensemble_rate(nsys, lanes, graph=True):
     solves, iters, retries, leap = [int(x) for x in counts]
     sizes = np.diff(ens.offsets)
     pairs = int(np.sum(sizes.astype(np.int64)**2))
+    per_launch = kms / 1e3 / max(launches, 1)
+    hw = ENSEMBLE_HW["fp64_flops_per_pair"] * pairs / per_launch
+    issue = ENSEMBLE_HW["valu_per_pair"] * pairs / 64 / per_launch
+    roofline = {"bound": "valu", "kernel": "ensemble_sakura_kernel",
+                "achieved": hw / 1e12, "peak": PEAK_FP64 / 1e12,
+                "unit": "TFLOP/s", "frac": hw / PEAK_FP64,
+                "flops_counted_as": "hardware FP64 FLOPs per pair drift "
+                                    "(rocprofv3 SQ_INSTS_VALU_FLOPS_FP64)",
+                "flops_per_pair": ENSEMBLE_HW["fp64_flops_per_pair"],
+                "valu_instructions_per_pair": ENSEMBLE_HW["valu_per_pair"],
+                "valu_issue_frac": issue / (SIMDS * CLOCK_PEAK /
+                                            FP64_CYCLES_PER_WAVE_INSTR),
+                "launch_s": per_launch, "traffic": None}
     return {"systems": nsys, "particles": int(ens.ntot),
             "sum_n2": pairs, "steps": drv.nsteps, "dt": 1.0 / 64,
             "lanes_per_system_min": lanes, "launches": launches,
@@ -414,6 +473,8 @@ def ensemble_rate(nsys, lanes, graph=True):
             "solves_per_s": solves / (kms / 1e3),
             "pair_drifts_per_s": 2 * drv.nsteps * pairs / (kms / 1e3),
             "iterations_per_solve": iters / max(solves, 1),
+            "iterations_per_s": iters / (kms / 1e3),
+            "roofline": roofline,
             "dE_E_quantiles": energy_quantiles(drv, e0)}
 
 

@@ -71,6 +71,11 @@ int tupan_device_info(int *cu_count, int *target_workgroups);
  * process-wide and not thread-safe. */
 int tupan_timing(int enable);
 int tupan_timing_read(double *total_ms, int *count);
+/* The recorded launches one by one: ms[k] for k < min(max, *count), in
+ * launch order (no counterpart in the reference).  The two-pass Sakura form
+ * records two intervals per call: the far-pair kernel, then the rest of
+ * the call. */
+int tupan_timing_read_each(double *ms, int max, int *count);
 
 /* Launch tuning of the pairwise kernels (no counterpart in the reference,
  * whose OpenCL build fixes its tile with -D LSIZE/WPT/UNROLL macros,

@@ -49,6 +49,10 @@ def _declare(lib):
     lib.tupan_timing_read.restype = ctypes.c_int
     lib.tupan_timing_read.argtypes = [ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_int)]
+    lib.tupan_timing_read_each.restype = ctypes.c_int
+    lib.tupan_timing_read_each.argtypes = [ctypes.POINTER(ctypes.c_double),
+                                           ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_int)]
     for fn in ("tupan_tune_set",):
         getattr(lib, fn).restype = ctypes.c_int
         getattr(lib, fn).argtypes = [ctypes.c_int] * 5
@@ -175,6 +179,17 @@ def timing_read():
     return ms.value, cnt.value
 
 
+def timing_read_each():
+    """Milliseconds of each recorded launch interval since timing(True)."""
+    cnt = ctypes.c_int()
+    check(lib().tupan_timing_read_each(None, 0, ctypes.byref(cnt)),
+          "tupan_timing_read_each")
+    out = (ctypes.c_double * max(1, cnt.value))()
+    check(lib().tupan_timing_read_each(out, cnt.value, ctypes.byref(cnt)),
+          "tupan_timing_read_each")
+    return list(out)[:cnt.value]
+
+
 def device_info():
     cu = ctypes.c_int()
     wg = ctypes.c_int()
@@ -199,7 +214,7 @@ REFERENCE_ABI = ("phi_kernel", "acc_kernel", "acc_jerk_kernel",
 DEVICE_ABI = ("tupan_last_error", "tupan_version", "tupan_device_info",
               "tupan_tune_set", "tupan_tune_get", "tupan_tune_shape",
               "tupan_rsq_bias",
-              "tupan_timing", "tupan_timing_read",
+              "tupan_timing", "tupan_timing_read", "tupan_timing_read_each",
               "tupan_phi", "tupan_acc", "tupan_acc_jerk", "tupan_snap_crackle",
               "tupan_tstep", "tupan_pairwise", "tupan_pairwise_f32",
               "tupan_pairwise_noalloc",

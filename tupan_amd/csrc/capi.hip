@@ -277,6 +277,23 @@ int tupan_timing_read(double* total_ms, int* count)
     return TUPAN_OK;
 }
 
+int tupan_timing_read_each(double* ms, int max, int* count)
+{
+    const int n = static_cast<int>(g_events_used);
+    if (count) *count = n;
+    if (!ms && max > 0) return set_error(TUPAN_ERR_ARG, "null output");
+    for (int k = 0; k < n && k < max; ++k) {
+        int rc = check_hip(hipEventSynchronize(g_events[k].second), "event sync");
+        if (rc != TUPAN_OK) return rc;
+        float t = 0.0f;
+        rc = check_hip(hipEventElapsedTime(&t, g_events[k].first, g_events[k].second),
+                       "event time");
+        if (rc != TUPAN_OK) return rc;
+        ms[k] = t;
+    }
+    return TUPAN_OK;
+}
+
 int tupan_tune_set(int kernel, int slot, int bucket, int shape, int wg_per_cu)
 {
     return tune_set(kernel, slot, bucket, shape, wg_per_cu);

@@ -472,6 +472,37 @@ __device__ inline bool far_stage(double dt, double m, double e2, double* p, doub
     return true;
 }
 
+// drift_twobody_far for a flag known at compile time (kFlag: -1, 1, -2, 2;
+// any other value leaves the pair unchanged).  The same operations in the
+// same order as the run-time form below, without the flag tests in the
+// pair loop.
+template <long kFlag>
+__device__ __forceinline__ bool drift_twobody_far_t(double dt, double m, double e2, double* p,
+                                                    double* v, Counters& cnt)
+{
+    Counters c;
+    bool ok = true;
+    if constexpr (kFlag == -1) {
+        p[0] -= v[0] * dt; p[1] -= v[1] * dt; p[2] -= v[2] * dt;
+        ok = far_stage<false>(dt, m, e2, p, v, c);
+    } else if constexpr (kFlag == 1) {
+        ok = far_stage<false>(dt, m, e2, p, v, c);
+        if (ok) { p[0] -= v[0] * dt; p[1] -= v[1] * dt; p[2] -= v[2] * dt; }
+    } else if constexpr (kFlag == -2) {
+        p[0] -= v[0] * dt / 2; p[1] -= v[1] * dt / 2; p[2] -= v[2] * dt / 2;
+        ok = far_stage<false>(dt, m, e2, p, v, c);
+        if (ok) { p[0] -= v[0] * dt / 2; p[1] -= v[1] * dt / 2; p[2] -= v[2] * dt / 2; }
+    } else if constexpr (kFlag == 2) {
+        ok = far_stage<false>(dt / 2, m, e2, p, v, c);
+        if (ok) {
+            p[0] -= v[0] * dt; p[1] -= v[1] * dt; p[2] -= v[2] * dt;
+            ok = far_stage<false>(dt / 2, m, e2, p, v, c);
+        }
+    }
+    if (ok) cnt.leapfrog += c.leapfrog;
+    return ok;
+}
+
 template <bool kFast = false>
 __device__ inline bool drift_twobody_far(double dt, long flag, double m, double e2, double* p,
                                          double* v, Counters& cnt)
