@@ -459,6 +459,41 @@ int tupan_sia_subtree(int n, void *const *arrays, int narrays, const int *roles,
                       double *t_curr, unsigned long long *pairs, int *status,
                       void *stream);
 
+/* Device-driven shared-step SIA steps: the 's' (fixed step) and 'a'
+ * (adaptive block step) variants of the SIA methods
+ * (tupan/integrator/sia.py:1089-1123 with the block step of
+ * integrator/__init__.py:48-78), in which every particle is slow at every
+ * step.  No counterpart in the reference (its driver loop is Python); the
+ * arithmetic is the reference driver's.  One call enqueues a whole step on
+ * `stream`: for adaptive steps Tstep (tstep_mode, eta) into scratch, the step
+ * size into the control block as tupan_hermite_step computes it, then for
+ * k < nops: ops[k] = 0 drift r += v h, 1 kick (Acc(all, all) in acc_mode
+ * into scratch, copied to the a arrays, v += a h), with
+ * h = coef_evolve[k] * (coef_bridge[k] * tau) (the scheme's bridge and
+ * evolve coefficients, nops <= 1024), then tstepij (adaptive), tstep = tau,
+ * time += tau, nstep += 1, t_curr += tau.
+ *   arrays[TUPAN_SIA_*]  device arrays of n >= 3 values: FP64, except nstep
+ *                        (int64)
+ *   scratch              device FP64 workspace of 5 n values
+ *   control              device FP64 [5]: t_curr, t_end, eta, tau, status, as
+ *                        for tupan_hermite_step (a step at or past t_end
+ *                        changes nothing) */
+#define TUPAN_SIA_MASS 0
+#define TUPAN_SIA_EPS2 1
+#define TUPAN_SIA_RX 2     /* RX RY RZ */
+#define TUPAN_SIA_VX 5     /* VX VY VZ */
+#define TUPAN_SIA_AX 8     /* AX AY AZ */
+#define TUPAN_SIA_TSTEP 11
+#define TUPAN_SIA_TSTEPIJ 12
+#define TUPAN_SIA_TIME 13
+#define TUPAN_SIA_NSTEP 14
+#define TUPAN_SIA_NROLES 15
+int tupan_sia_step(int adaptive, int acc_mode, int tstep_mode, size_t n,
+                   void *const *arrays, double *scratch, double *control,
+                   double eta, int nops, const int *ops,
+                   const double *coef_bridge, const double *coef_evolve,
+                   void *stream);
+
 /* ---------------------------------------------------------------------------
  * Kepler-guess network (no counterpart in the reference): 4 -> 64 -> 64 -> 1
  * perceptron with SiLU, inputs x = (log tau, sigma, kappa, eps) of

@@ -24,14 +24,14 @@ def cpu_plummer(n, eps, seed):
     return ps.to_numpy()
 
 
-def run(method, eta, host, steps, backend, t_end=1.0, **kw):
+def run(method, eta, host, steps, backend, t_end=1.0, mode=None, **kw):
     from tupan_amd.integrator import Integrator
     from tupan_amd.particles import ParticleSystem
     if backend == "oracle":
         install_oracle("float64")
         dev = "cpu"
     else:
-        install_hip("float64")
+        install_hip("float64", **({"mode": mode} if mode else {}))
         dev = "cuda:0"
     try:
         ps = ParticleSystem.from_numpy(host, device=dev)
@@ -240,6 +240,56 @@ def test_device_hermite_graph_replays_equal_eager_steps():
     assert ta == tb == t_end
     for k in ("rx", "ry", "rz", "vx", "vy", "vz", "ax", "jz", "tstep",
               "tstepij", "time", "nstep"):
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "split32"])
+@pytest.mark.parametrize("method,eta,steps", [
+    ("sia21s.dkd", 1.0 / 256, 4), ("sia21a.dkd", 0.02, 4),
+    ("sia43a.kdk", 0.02, 3), ("sia45s.kdk", 1.0 / 256, 3),
+    ("sia22a.dkd", 0.02, 3),
+])
+def test_device_sia_steps_match_host_driver(method, eta, steps, mode):
+    """tupan_sia_step (the shared-step 's' and 'a' variants: step size,
+    drift/kick sequence and bookkeeping on the device) against the host
+    driver on the same device forces: the same step sizes and time and every
+    array bit for bit (the updates are the driver's x + v * h with
+    h = c_e * (c_b * tau), one rounding at a time)."""
+    host = cpu_plummer(128, 4.0 / 128, 13)
+    g, tg = run(method, eta, host, steps, "hip", mode=mode)
+    d, td = run(method, eta, host, steps, "hip", mode=mode,
+                device_steps=True)
+    assert tg == td
+    for k in ("rx", "ry", "rz", "vx", "vy", "vz", "ax", "ay", "az", "tstep",
+              "time", "nstep"):
+        np.testing.assert_array_equal(d[k], g[k], err_msg=k)
+    if method[5] == "a":
+        np.testing.assert_array_equal(d["tstepij"], g["tstepij"])
+
+
+def test_device_sia_graph_replays_equal_eager_steps():
+    """SIA.evolve replaying a graph of 8 device steps equals the same steps
+    launched one by one, bit for bit, and stops at t_end exactly."""
+    from tupan_amd.integrator import Integrator
+    from tupan_amd.particles import ParticleSystem
+    host = cpu_plummer(128, 4.0 / 128, 7)
+    t_end = 0.125
+    out = []
+    install_hip("float64")
+    for graph in (False, True):
+        ps = ParticleSystem.from_numpy(host, device="cuda:0")
+        type(ps).t_curr = 0.0
+        it = Integrator(0.02, 0.0, ps, method="sia21a.dkd", device_steps=True)
+        if graph:
+            assert it.integrator.evolve(t_end, steps_per_graph=8) >= 1
+        else:
+            while it.time < t_end:
+                it.evolve_step(t_end)
+        out.append((it.particle_system.to_numpy(), it.time))
+    (a, ta), (b, tb) = out
+    assert ta == tb == t_end
+    for k in ("rx", "ry", "rz", "vx", "vy", "vz", "ax", "tstep", "tstepij",
+              "time", "nstep"):
         np.testing.assert_array_equal(a[k], b[k], err_msg=k)
 
 

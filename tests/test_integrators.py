@@ -459,3 +459,23 @@ def test_hermite_device_steps_argument_checks():
         it.integrator.evolve(1.0, steps_per_graph=0)
     with pytest.raises(RuntimeError):
         it.evolve_step(1.0)
+
+
+def test_sia_device_steps_argument_checks():
+    """Device SIA steps exist for the shared-step 's' and 'a' variants:
+    evolve needs device_steps and at least one step per graph, a
+    hierarchical method or a host particle system refuses to start."""
+    from tupan_amd.integrator import Integrator
+    it = Integrator(1.0 / 64, 0.0, make_ps(kepler_pair()), method="sia21a.dkd")
+    with pytest.raises(RuntimeError):
+        it.integrator.evolve(1.0)
+    it = Integrator(1.0 / 64, 0.0, make_ps(kepler_pair()), method="sia21a.dkd",
+                    device_steps=True)
+    with pytest.raises(ValueError):
+        it.integrator.evolve(1.0, steps_per_graph=0)
+    with pytest.raises(RuntimeError):
+        it.evolve_step(1.0)
+    it = Integrator(1.0 / 64, 0.0, make_ps(kepler_pair()), method="sia21h.dkd",
+                    device_steps=True)
+    with pytest.raises(RuntimeError):
+        it.evolve_step(1.0)

@@ -74,6 +74,10 @@ def _declare(lib):
     lib.tupan_hermite_step.restype = ctypes.c_int
     lib.tupan_hermite_step.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        _S, _PP, _P, _P, ctypes.c_double, _P]
+    lib.tupan_sia_step.restype = ctypes.c_int
+    lib.tupan_sia_step.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   _S, _PP, _P, _P, ctypes.c_double,
+                                   ctypes.c_int, _P, _P, _P, _P]
     lib.tupan_pnacc.restype = ctypes.c_int
     lib.tupan_pnacc.argtypes = [ctypes.c_int, _S, _PP, _S, _PP, ctypes.c_ulong,
                                 ctypes.POINTER(ctypes.c_double), _PP, _P]
@@ -227,4 +231,4 @@ DEVICE_ABI = ("tupan_last_error", "tupan_version", "tupan_device_info",
               "tupan_mlp_pack", "tupan_mlp_infer", "tupan_sia_subtree",
               "tupan_pnacc", "tupan_pnacc_f32", "tupan_host_pnacc_f32",
               "tupan_hermite_step", "tupan_sakura_graph",
-              "tupan_sakura_step")
+              "tupan_sakura_step", "tupan_sia_step")

@@ -616,6 +616,14 @@ int tupan_hermite_step(int order, int adaptive, int mode, size_t n, void* const*
     return hermite_step_call(order, adaptive, mode, n, arrays, scratch, control, eta, stream);
 }
 
+int tupan_sia_step(int adaptive, int acc_mode, int tstep_mode, size_t n, void* const* arrays,
+                   double* scratch, double* control, double eta, int nops, const int* ops,
+                   const double* coef_bridge, const double* coef_evolve, void* stream)
+{
+    return sia_step_call(adaptive, acc_mode, tstep_mode, n, arrays, scratch, control, eta, nops,
+                         ops, coef_bridge, coef_evolve, stream);
+}
+
 int tupan_pnacc(int mode, size_t ni, const double* const* ifields, size_t nj,
                 const double* const* jfields, unsigned long order, const double* inv7,
                 double* const* out, void* stream)

@@ -7,7 +7,7 @@
 // one HIP graph and replayed with no host round trip.  Per step:
 //
 //   1. 'a' methods: Tstep (eta) into scratch arrays;
-//   2. hermite_tau_kernel: the base step min(|t_end| - |t|, |eta|) and, for
+//   2. block_tau_kernel: the base step min(|t_end| - |t|, |eta|) and, for
 //      'a' methods, the power-of-two block step below min |tstep| aligned
 //      with t (get_min_block_tstep); tau = 0 once t has reached t_end, which
 //      turns the rest of the step into a no-op, and also once the control
@@ -31,19 +31,11 @@
 // precision; its log2 is the device library's, which can differ from numpy's
 // in the last bit for a minimum time step within an ulp or two of a power of
 // two.
-#include <float.h>
-
-#include "tupan_internal.h"
+#include "block_step.h"
 
 namespace tupan {
 
 namespace {
-
-constexpr int kCtlT = 0, kCtlTEnd = 1, kCtlEta = 2, kCtlTau = 3, kCtlStatus = 4;
-// Halvings of the block step are bounded: from a step >= 2^-1075 down to
-// the spacing of t_next takes at most ~2100.
-constexpr int kMaxHalvings = 2200;
-constexpr int kTauThreads = 1024;
 
 struct HermiteArrays {
     double *m, *e2, *r[3], *v[3], *a[3], *j[3], *s[3], *c[3];
@@ -53,76 +45,6 @@ struct HermiteArrays {
     double *r0[3], *v0[3], *a0[3], *j0[3], *s0[3], *c0[3], *ts, *tsij;
     double *a1[3], *j1[3], *s1[3], *c1[3];
 };
-
-// min(a, b) that returns NaN when either is NaN, as numpy's min over an
-// array with a NaN does (fmin would drop it and pick a step from the rest).
-__device__ __forceinline__ double min_nan(double a, double b)
-{
-    return (a != a || b != b) ? __builtin_nan("") : fmin(a, b);
-}
-
-__global__ __launch_bounds__(kTauThreads) void hermite_tau_kernel(int n, const double* ts,
-                                                                   int adaptive, double* ctl)
-{
-    __shared__ double red[kTauThreads];
-    const int tid = static_cast<int>(threadIdx.x);
-    double lo = DBL_MAX;
-    if (adaptive)
-        for (int i = tid; i < n; i += kTauThreads) lo = min_nan(lo, fabs(ts[i]));
-    red[tid] = lo;
-    __syncthreads();
-    for (int w = kTauThreads / 2; w > 0; w >>= 1) {
-        if (tid < w) red[tid] = min_nan(red[tid], red[tid + w]);
-        __syncthreads();
-    }
-    if (tid != 0) return;
-    const double t = ctl[kCtlT], tend = ctl[kCtlTEnd], eta = ctl[kCtlEta];
-    if (ctl[kCtlStatus] != 0 || !isfinite(t) || !isfinite(tend) || !isfinite(eta)) {
-        ctl[kCtlTau] = 0.0;
-        if (ctl[kCtlStatus] == 0) ctl[kCtlStatus] = 1.0;
-        return;
-    }
-    if ((eta > 0 && t >= tend) || (eta < 0 && t <= tend) || eta == 0) {
-        ctl[kCtlTau] = 0.0;
-        return;
-    }
-    // integrator/__init__.py:48-57
-    double tau = fmin(fabs(tend) - fabs(t), fabs(eta));
-    tau = fmax(tau, fabs(tend) * (2 * DBL_EPSILON));
-    tau = copysign(tau, eta);
-    if (adaptive) {
-        // integrator/__init__.py:59-78
-        const double min_ts = red[0];
-        if (!(min_ts > 0) || !isfinite(min_ts)) {
-            // Tstep gave no positive finite step (NaN or zero inputs)
-            ctl[kCtlTau] = 0.0;
-            ctl[kCtlStatus] = 2.0;
-            return;
-        }
-        const double p = log2(min_ts) - 1;
-        if (p < -1070.0 || p > 1020.0) {
-            ctl[kCtlTau] = 0.0;
-            ctl[kCtlStatus] = 2.0;
-            return;
-        }
-        double bts = ldexp(1.0, static_cast<int>(p));
-        const double t_next = t + bts;
-        int halvings = 0;
-        while (fmod(t_next, bts) != 0 && halvings < kMaxHalvings) {
-            bts /= 2;
-            ++halvings;
-        }
-        if (halvings == kMaxHalvings || bts == 0) {
-            ctl[kCtlTau] = 0.0;
-            ctl[kCtlStatus] = 3.0;
-            return;
-        }
-        bts = copysign(bts, tau);
-        if (fabs(bts) > fabs(tau)) bts = tau;
-        tau = bts;
-    }
-    ctl[kCtlTau] = tau;
-}
 
 // hermite.py:20-410, predictor of H2/H4/H6/H8.
 template <int Order>
@@ -256,7 +178,7 @@ int step_order(int adaptive, int mode, int n, const HermiteArrays& h, double eta
         int rc = pairwise_call<double>(K_TSTEP, mode, n, in, n, in, out, eta, s);
         if (rc != TUPAN_OK) return rc;
     }
-    hipLaunchKernelGGL(hermite_tau_kernel, dim3(1), dim3(kTauThreads), 0, s, n, h.ts, adaptive,
+    hipLaunchKernelGGL(block_tau_kernel, dim3(1), dim3(kTauThreads), 0, s, n, h.ts, adaptive,
                        ctl);
     int rc = check_launch("hermite_tau");
     if (rc == TUPAN_OK) rc = forces(Order, mode, n, h, true, s);

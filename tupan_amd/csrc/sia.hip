@@ -26,6 +26,7 @@
 // host's elementwise updates are.
 //
 // Compiled with -ffp-contract=off.
+#include "block_step.h"
 #include "kepler.h"
 #include "ops.h"
 #include "tupan_internal.h"
@@ -338,6 +339,136 @@ int sia_subtree_call(int n, void* const* arrays, int narrays, const int* roles, 
     hipLaunchKernelGGL(sia_subtree_kernel, dim3(1), dim3(kSubMax), 0,
                        static_cast<hipStream_t>(stream), a);
     return check_launch("sia_subtree");
+}
+
+}  // namespace tupan
+
+// ---------------------------------------------------------------------------
+// Device-driven shared-step SIA steps: the 's' (fixed step) and 'a'
+// (adaptive power-of-two block step) variants of sia.py:1089-1123, in which
+// every particle is slow at every step, so a step is the scheme's bridge
+// sequence of evolve(all, c_b tau) calls -- each the scheme's drift/kick
+// sequence, a kick being Acc(all, all) and v += a h -- followed by the
+// bookkeeping.  One call enqueues a whole step with the step size in the
+// device control block (block_step.h), so a sequence of calls can be
+// captured as one HIP graph.  Acc goes to scratch and reaches the live
+// a arrays in the kick, and Tstep ('a') to scratch and the live tstepij in
+// the bookkeeping, so a step past t_end (tau = 0) changes no live array.
+// Every update is the host driver's elementwise operation, one rounding at
+// a time: h = c_e * (c_b * tau) as the driver forms it, then x + v * h.
+// ---------------------------------------------------------------------------
+namespace tupan {
+
+namespace {
+
+// Drifts and kicks of one shared step: the bridge's drifts times the evolve
+// sequence (sia69: 10 x 19 = 190).
+constexpr int kStepOps = 1024;
+
+struct SiaStepArrays {
+    double *m, *e2, *r[3], *v[3], *a[3];
+    double *tstep, *tstepij, *time;
+    long long* nstep;
+    double *as[3], *ts, *tsij;  // scratch
+};
+
+__global__ __launch_bounds__(256) void sia_update_kernel(int n, SiaStepArrays s, double cb,
+                                                         double ce, int kick,
+                                                         const double* ctl)
+{
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const double tau = ctl[kCtlTau];
+    if (i >= n || tau == 0) return;
+    const double h = ce * (cb * tau);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (kick) {
+            const double a = s.as[d][i];
+            s.a[d][i] = a;
+            s.v[d][i] = s.v[d][i] + a * h;
+        } else {
+            s.r[d][i] = s.r[d][i] + s.v[d][i] * h;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void sia_end_kernel(int n, SiaStepArrays s, int adaptive,
+                                                      double* ctl)
+{
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const double tau = ctl[kCtlTau];
+    if (tau == 0) return;
+    if (i < n) {
+        if (adaptive) s.tstepij[i] = s.tsij[i];
+        s.tstep[i] = tau;
+        s.time[i] = s.time[i] + tau;
+        s.nstep[i] = s.nstep[i] + 1;
+    }
+    if (i == 0) ctl[kCtlT] = ctl[kCtlT] + tau;
+}
+
+}  // namespace
+
+int sia_step_call(int adaptive, int acc_mode, int tstep_mode, size_t n, void* const* arrays,
+                  double* scratch, double* control, double eta, int nops, const int* ops,
+                  const double* coef_bridge, const double* coef_evolve, void* stream)
+{
+    for (int m : {acc_mode, tstep_mode})
+        if (m != TUPAN_MODE_FP64 && m != TUPAN_MODE_SPLIT32 && m != TUPAN_MODE_IEEE64)
+            return set_error(TUPAN_ERR_ARG, "unknown force mode");
+    if (nops < 1 || nops > kStepOps)
+        return set_error(TUPAN_ERR_ARG, "bad operator sequence length (1..1024)");
+    if (!arrays || !scratch || !control || !ops || !coef_bridge || !coef_evolve)
+        return set_error(TUPAN_ERR_ARG, "null argument");
+    for (int q = 0; q < nops; ++q)
+        if (ops[q] != 0 && ops[q] != 1) return set_error(TUPAN_ERR_ARG, "operator must be 0 or 1");
+    if (n > static_cast<size_t>(INT32_MAX)) return set_error(TUPAN_ERR_ARG, "n exceeds 2^31-1");
+    if (n < 3)
+        return set_error(TUPAN_ERR_ARG, "device SIA steps need at least 3 particles "
+                                        "(the reference evolves 1 or 2 by FewBody)");
+    for (int q = 0; q < TUPAN_SIA_NROLES; ++q)
+        if (!arrays[q]) return set_error(TUPAN_ERR_ARG, "null particle array");
+    SiaStepArrays s{};
+    auto d = [&](int q) { return static_cast<double*>(arrays[q]); };
+    s.m = d(TUPAN_SIA_MASS);
+    s.e2 = d(TUPAN_SIA_EPS2);
+    for (int k = 0; k < 3; ++k) {
+        s.r[k] = d(TUPAN_SIA_RX + k);
+        s.v[k] = d(TUPAN_SIA_VX + k);
+        s.a[k] = d(TUPAN_SIA_AX + k);
+        s.as[k] = scratch + k * n;
+    }
+    s.tstep = d(TUPAN_SIA_TSTEP);
+    s.tstepij = d(TUPAN_SIA_TSTEPIJ);
+    s.time = d(TUPAN_SIA_TIME);
+    s.nstep = static_cast<long long*>(arrays[TUPAN_SIA_NSTEP]);
+    s.ts = scratch + 3 * n;
+    s.tsij = scratch + 4 * n;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    const int ni = static_cast<int>(n);
+    const unsigned grid = static_cast<unsigned>((n + 255) / 256);
+    const double* in[8] = {s.m, s.r[0], s.r[1], s.r[2], s.e2, s.v[0], s.v[1], s.v[2]};
+    if (adaptive) {
+        double* out[2] = {s.ts, s.tsij};
+        const int rc = pairwise_call<double>(K_TSTEP, tstep_mode, n, in, n, in, out, eta, st);
+        if (rc != TUPAN_OK) return rc;
+    }
+    hipLaunchKernelGGL(block_tau_kernel, dim3(1), dim3(kTauThreads), 0, st, ni, s.ts, adaptive,
+                       control);
+    int rc = check_launch("block_tau");
+    for (int q = 0; q < nops && rc == TUPAN_OK; ++q) {
+        if (ops[q] == 1) {
+            double* out[3] = {s.as[0], s.as[1], s.as[2]};
+            rc = pairwise_call<double>(K_ACC, acc_mode, n, in, n, in, out, 0.0, st);
+            if (rc != TUPAN_OK) break;
+        }
+        hipLaunchKernelGGL(sia_update_kernel, dim3(grid), dim3(256), 0, st, ni, s,
+                           coef_bridge[q], coef_evolve[q], ops[q], control);
+        rc = check_launch("sia_update");
+    }
+    if (rc != TUPAN_OK) return rc;
+    hipLaunchKernelGGL(sia_end_kernel, dim3(grid), dim3(256), 0, st, ni, s, adaptive, control);
+    return check_launch("sia_end");
 }
 
 }  // namespace tupan

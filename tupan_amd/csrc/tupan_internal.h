@@ -139,6 +139,11 @@ int sakura_call_ex(size_t ni, const double* const* ip, size_t nj, const double* 
                    double dt, long flag, double* const* out, SakuraOptions& opt,
                    unsigned long long* counters, int* status, void* stream);
 
+// Device-driven shared-step SIA step ('s' and 'a' variants, sia.hip).
+int sia_step_call(int adaptive, int acc_mode, int tstep_mode, size_t n, void* const* arrays,
+                  double* scratch, double* control, double eta, int nops, const int* ops,
+                  const double* coef_bridge, const double* coef_evolve, void* stream);
+
 // Device SIA recursion for sets of at most 64 particles (sia.hip).
 int sia_subtree_call(int n, void* const* arrays, int narrays, const int* roles, int acc_mode,
                      int tstep_mode, double eta, double tau, int nbridge, const int* bridge_op,

@@ -260,17 +260,69 @@ class SIA(Base):
               "pn_amy", "pn_amz")
 
     def __init__(self, eta, time, ps, method, inplace=True,
-                 device_subtree=True, **kwargs):
+                 device_subtree=True, device_steps=False, **kwargs):
         super(SIA, self).__init__(eta, time, ps, **kwargs)
         self.inplace = inplace
         self.device_subtree = device_subtree
+        self.device_steps = device_steps
+        self._dev = None
         self.method = method
         self.scheme = Scheme(method[:5])
         variant = method[5]
         self.update_tstep = variant in "ah"
         self.shared_tstep = variant in "sa"
-        self.evolve = self.scheme.dkd if method.endswith("dkd") \
+        self.evolve_set = self.scheme.dkd if method.endswith("dkd") \
             else self.scheme.kdk
+
+    def _device(self):
+        if self._dev is None or self._dev.ps is not self.ps:
+            self._dev = DeviceSteps(self)
+        return self._dev
+
+    def evolve_step(self, t_end):
+        """integrator/__init__.py:80-99; with device_steps (the 's' and 'a'
+        variants) one device step and one read of the new time."""
+        if not self.device_steps:
+            return super(SIA, self).evolve_step(t_end)
+        if not self.is_initialized:
+            self.initialize(t_end)
+        dev = self._device()
+        dev.set_control(t_end)
+        dev.launch()
+        dev.read_time()
+        if self.reporter:
+            self.reporter.diagnostic_report(self.ps)
+
+    def evolve(self, t_end, steps_per_graph=32):
+        """Advance to t_end by device steps ('s' and 'a' variants),
+        replaying a HIP graph of steps_per_graph steps; steps past t_end
+        inside the last replay change nothing.  Returns the number of
+        replays."""
+        if not self.device_steps:
+            raise RuntimeError("evolve needs device_steps=True")
+        if steps_per_graph < 1:
+            raise ValueError("steps_per_graph must be at least 1")
+        if not self.is_initialized:
+            self.initialize(t_end)
+        dev = self._device()
+        dev.set_control(t_end)
+        dev.launch()                      # eager first step: sizes workspaces
+        replays = 0
+        graph = None
+        t = dev.read_time()
+        while (t < t_end) if self.eta > 0 else (t > t_end):
+            if graph is None:
+                graph = dev.capture(steps_per_graph)
+            graph.replay()
+            replays += 1
+            t_new = dev.read_time()
+            if t_new == t:
+                raise dev.native.TupanHipError(
+                    "device SIA steps made no progress at t = %r" % t)
+            t = t_new
+        if self.reporter:
+            self.reporter.diagnostic_report(self.ps)
+        return replays
 
     def do_step(self, ps, tau):
         names = self.AUX if self.inplace else ()
@@ -280,14 +332,15 @@ class SIA(Base):
             if name not in ps.__dict__:
                 ps.register_auxiliary_attribute(name, "real")
         self._subtree = _DeviceSubtree.make(self, ps)
-        return self.recurse(ps, tau, self.evolve, self.scheme.bridge_sf)
+        return self.recurse(ps, tau, self.evolve_set, self.scheme.bridge_sf)
 
     def recurse(self, ps, tau, evolve, bridge_sf):
         """sia.py:1089-1123."""
         if not ps.n:
             return ps
         sub = getattr(self, "_subtree", None)
-        if sub is not None and ps.n <= SUBTREE_MAX and evolve == self.evolve:
+        if (sub is not None and ps.n <= SUBTREE_MAX
+                and evolve == self.evolve_set):
             return sub.run(ps, tau)
         flag = -1
         if self.update_tstep:
@@ -305,6 +358,114 @@ class SIA(Base):
             slow.time += tau
             slow.nstep += 1
         return ps if self.inplace else join(slow, fast)
+
+
+class DeviceSteps(object):
+    """Shared-step SIA steps ('s' and 'a' variants) run by tupan_sia_step:
+    every particle is slow at every step, so a step is the bridge's
+    evolve(all, c_b tau) calls flattened into one drift/kick sequence with
+    h = c_e * (c_b * tau); the step size lives in a device control block
+    (t_curr, t_end, eta, tau, status) and a 5 n scratch array holds Acc and
+    Tstep outputs until the step commits them."""
+
+    ROLES = ("mass", "eps2", "rx", "ry", "rz", "vx", "vy", "vz", "ax", "ay",
+             "az", "tstep", "tstepij", "time", "nstep")
+    STATUS = {1: "non-finite time, t_end or eta",
+              2: "Tstep gave no positive finite time step",
+              3: "no power-of-two block step aligned with the time"}
+
+    def __init__(self, sia):
+        from .. import _native
+        from ..lib import extensions
+        from ..lib.hip_backend import HIPKernel
+        ps = sia.ps
+        if not sia.shared_tstep:
+            raise RuntimeError("device SIA steps run the shared-step 's' and "
+                               "'a' variants; %s is hierarchical" % sia.method)
+        if ps.include_pn_corrections:
+            raise RuntimeError("device SIA steps have no PN corrections")
+        modes = []
+        for ext in (extensions.acc, extensions.tstep):
+            k = getattr(ext, "kernel", None)
+            if not isinstance(k, HIPKernel) or k.prec != "float64":
+                raise RuntimeError("device SIA steps need the HIP backend in "
+                                   "float64")
+            modes.append(_native.MODES[k.mode])
+        if not ps.mass.is_cuda:
+            raise RuntimeError("device SIA steps need a device-resident "
+                               "particle system")
+        if ps.n < 3:
+            raise RuntimeError("device SIA steps need at least 3 particles "
+                               "(1 or 2 evolve by FewBody)")
+        for name in self.ROLES:
+            if name not in ps.__dict__:
+                ps.register_auxiliary_attribute(name, "real")
+        self.lib = _native.lib()
+        self.native = _native
+        self.sia = sia
+        self.ps = ps
+        self.n = ps.n
+        self.acc_mode, self.tstep_mode = modes
+        self.adaptive = 0 if sia.method[5] == "s" else 1
+        seq = palindrome(sia.scheme.d, sia.scheme.k)
+        if sia.method.endswith("dkd"):
+            ev = [(0 if w == 0 else 1, c) for w, c in seq]
+        else:
+            ev = [(1 if w == 0 else 0, c) for w, c in seq]
+        ops, cb, ce = [], [], []
+        for wb, c_b in seq:
+            if wb == 0:          # sf_drift: evolve(slow, c_b tau); no fast set
+                for we, c_e in ev:
+                    ops.append(we)
+                    cb.append(c_b)
+                    ce.append(c_e)
+            # wb == 1: sf_kick with an empty fast set does nothing
+        self.nops = len(ops)
+        self.ops = (ctypes.c_int * len(ops))(*ops)
+        self.cb = (ctypes.c_double * len(cb))(*cb)
+        self.ce = (ctypes.c_double * len(ce))(*ce)
+        dev = ps.mass.device
+        self.scratch = torch.empty(5 * self.n, dtype=torch.float64, device=dev)
+        self.ctl = torch.zeros(5, dtype=torch.float64, device=dev)
+        self.arrays = _native.ptr_array([ps.__dict__[r].data_ptr()
+                                         for r in self.ROLES])
+
+    def set_control(self, t_end):
+        self.ctl.copy_(torch.tensor([float(type(self.ps).t_curr),
+                                     float(t_end), float(self.sia.eta), 0.0,
+                                     0.0], dtype=torch.float64))
+
+    def launch(self):
+        from ..lib.hip_backend import current_stream_handle
+        rc = self.lib.tupan_sia_step(
+            self.adaptive, self.acc_mode, self.tstep_mode, self.n,
+            self.arrays, self.scratch.data_ptr(), self.ctl.data_ptr(),
+            float(self.sia.eta), self.nops, ctypes.addressof(self.ops),
+            ctypes.addressof(self.cb), ctypes.addressof(self.ce),
+            current_stream_handle())
+        self.native.check(rc, "tupan_sia_step")
+
+    def read_time(self):
+        """The device time (one synchronisation); raises if a step failed."""
+        c = self.ctl.cpu().tolist()
+        status = int(c[4])
+        if status:
+            raise self.native.TupanHipError(
+                "tupan_sia_step: %s" % self.STATUS.get(status, status))
+        type(self.ps).t_curr = c[0]
+        return c[0]
+
+    def capture(self, nsteps):
+        """HIP graph of nsteps device steps."""
+        graph = torch.cuda.CUDAGraph()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            with torch.cuda.graph(graph, stream=side):
+                for _ in range(nsteps):
+                    self.launch()
+        torch.cuda.current_stream().wait_stream(side)
+        return graph
 
 
 class _DeviceSubtree(object):
