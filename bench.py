@@ -493,7 +493,7 @@ def energy_quantiles(drv, e0):
             "max": float(err.max())}
 
 
-def block_step_rate(n, tau, steps, modes):
+def block_step_rate(n, tau, steps, modes, device_levels=True):
     """The block-step workload: Plummer sphere of n particles with masses
     dN/dm ~ m^-2.35 on [0.1, 10] (normalised to total mass 1), eps = 1e-4,
     seed 1, integrated by sia21h.dkd (eta = 0.02, hierarchical power-of-two
@@ -546,7 +546,59 @@ def block_step_rate(n, tau, steps, modes):
                      "wall_s": wall, "pairwise_kernel_s": kms / 1e3,
                      "pairwise_launches": launches,
                      "dE_E": abs((energy(ps) - e0) / e0)}
+        if device_levels:
+            log("bench: block steps, %s, device-scheduled levels" % mode)
+            out[mode]["device_levels"] = block_steps_device_levels(
+                ps0, mode, tau, steps, ps.to_numpy())
     return out
+
+
+def block_steps_device_levels(ps0, mode, tau, steps, host_final):
+    """The same steps with every level visit scheduled on the device
+    (SIA(..., device_levels=True)): one untimed step (it captures the two
+    slot graphs), then `steps` timed ones, each replaying the graphs until
+    the device reports the step done, no host read of level sizes.  Returns the
+    rate, the level visits above 64 particles and working slots per step,
+    the graph replays, and whether the final state equals the host-driven
+    levels' bit for bit."""
+    import torch
+    from tupan_amd.diagnostics import energy
+    from tupan_amd.integrator import Integrator
+    from tupan_amd.lib import extensions
+    from tupan_amd.lib.utils import ctype
+    ps = ps0.copy()
+    type(ps).t_curr = 0.0
+    old = ctype.force_mode
+    ctype.force_mode = mode
+    extensions.install("HIP", "float64")
+    try:
+        it = Integrator(0.02, 0.0, ps, method="sia21h.dkd", device_levels=True)
+        it.evolve_step(tau)
+        torch.cuda.synchronize()
+        lv = it.integrator._device_levels()
+        cnt = extensions.count_pairs(True)
+        replays = 0
+        visits = ops = 0
+        t0 = time.perf_counter()
+        for k in range(steps):
+            replays += lv.step(tau * (k + 2))
+            visits += lv.last_visits
+            ops += lv.last_ops
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        extensions.count_pairs(False)
+        energy(it.particle_system)        # phi, as the host leg's dE/E left it
+        final = it.particle_system.to_numpy()
+    finally:
+        ctype.force_mode = old
+        extensions.install("HIP", "float64")
+    pairs = int(sum(cnt.values()))
+    same = all(np.array_equal(final[k], host_final[k]) for k in host_final)
+    return {"gint_s": pairs / wall / 1e9, "pairs": pairs, "wall_s": wall,
+            "level_visits_above_64_per_step": visits / steps,
+            "slots_per_step": ops / steps, "graph_replays": replays,
+            "slots_per_replay": lv.slots,
+            "equals_host_levels": same}
 
 
 def _best(fn, reps):

@@ -494,6 +494,51 @@ int tupan_sia_step(int adaptive, int acc_mode, int tstep_mode, size_t n,
                    const double *coef_bridge, const double *coef_evolve,
                    void *stream);
 
+/* Device-scheduled hierarchical SIA steps: the 'h' variants of the SIA
+ * methods (tupan/integrator/sia.py:1089-1123) with every level visit decided
+ * on the device.  No counterpart in the reference, whose driver visits the
+ * levels from Python; the operations, their order and every pairwise call's
+ * launch plan are the host driver's, so the results are its bit for bit.
+ * A plan holds the walk's state in device memory (a stack of level frames)
+ * and is tied to the particle arrays, the scheme and the tuning table at
+ * creation.  tupan_sia_levels_enqueue enqueues `nslots` slots on `stream`
+ * (with begin != 0 first the step's start: the base step from the control
+ * time, t_end and eta, and the walk at the whole set); each slot performs
+ * one operation of the walk -- a Tstep or Acc call, a [slow | fast]
+ * partition, a drift, kick or bookkeeping update, a subset of at most 64
+ * particles in one launch (tupan_sia_subtree), a one- or two-particle evolve
+ * -- as a fixed sequence of launches whose workers read the operation from
+ * device memory, so any run of slots can be captured as one HIP graph.  The
+ * step is complete when the state's done flag is set (slots after that do
+ * nothing); a step takes roughly 12 slots per level visit above 64
+ * particles.
+ *   arrays[narrays]   every 8-byte per-particle array of the system (device;
+ *                     permuted together by the partitions), narrays <= 32
+ *   roles[15], acc_mode, tstep_mode, eta, bridge/evolve sequences: as for
+ *                     tupan_sia_subtree; n > 64
+ *   set:   control time t_curr, t_end and eta (synchronous); clear_pairs
+ *          zeroes the Tstep and Acc interaction counters
+ *   state: step[5] = t_curr, t_end, eta, tau, status of the step size;
+ *          state[4] = done, status (1 Kepler failure, 2 more than 64 levels,
+ *          3 more than 2^20 visits, 4 operation queue overflow, 16 scheduler
+ *          bound, 32 no valid step size), level visits above 64 particles,
+ *          slots that did work; pairs[2] = Tstep and Acc interactions
+ *          (synchronous) */
+typedef struct tupan_sia_levels tupan_sia_levels;
+int tupan_sia_levels_create(int n, void *const *arrays, int narrays,
+                            const int *roles, int acc_mode, int tstep_mode,
+                            double eta, int nbridge, const int *bridge_op,
+                            const double *bridge_c, int nevolve,
+                            const int *evolve_op, const double *evolve_c,
+                            tupan_sia_levels **plan);
+int tupan_sia_levels_destroy(tupan_sia_levels *plan);
+int tupan_sia_levels_set(tupan_sia_levels *plan, double t_curr, double t_end,
+                         double eta, int clear_pairs);
+int tupan_sia_levels_enqueue(tupan_sia_levels *plan, int begin, int nslots,
+                             void *stream);
+int tupan_sia_levels_state(tupan_sia_levels *plan, double step[5],
+                           int state[4], unsigned long long pairs[2]);
+
 /* ---------------------------------------------------------------------------
  * Kepler-guess network (no counterpart in the reference): 4 -> 64 -> 64 -> 1
  * perceptron with SiLU, inputs x = (log tau, sigma, kappa, eps) of

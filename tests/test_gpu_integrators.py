@@ -293,6 +293,84 @@ def test_device_sia_graph_replays_equal_eager_steps():
         np.testing.assert_array_equal(a[k], b[k], err_msg=k)
 
 
+def _levels_run(host, method, mode, steps, t_step, how):
+    """`steps` steps of base step t_step of an 'h' method: 'host' levels,
+    'device' (eagerly enqueued slots) or 'graph' (slots replayed from the
+    captured graphs), one evolve_step each, or 'evolve' (SIA.evolve to each
+    step's end)."""
+    from tupan_amd.integrator import Integrator
+    from tupan_amd.lib import extensions
+    from tupan_amd.particles import ParticleSystem
+    install_hip("float64", mode=mode)
+    try:
+        ps = ParticleSystem.from_numpy(host, device="cuda:0")
+        type(ps).t_curr = 0.0
+        kw = {"host": {}, "device": {"device_levels": "eager"},
+              "graph": {"device_levels": True},
+              "evolve": {"device_levels": True}}[how]
+        it = Integrator(0.02, 0.0, ps, method=method, **kw)
+        cnt = extensions.count_pairs(True)
+        for k in range(steps):
+            if how == "evolve":
+                it.integrator.evolve(t_step * (k + 1))
+            else:
+                it.evolve_step(t_step * (k + 1))
+        extensions.count_pairs(False)
+        out = it.particle_system.to_numpy()
+        t = type(ps).t_curr
+    finally:
+        install_hip("float64")
+    return out, t, dict(cnt)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "split32"])
+@pytest.mark.parametrize("method", ["sia21h.dkd", "sia22h.kdk"])
+def test_device_levels_match_host_driver(method, mode):
+    """Device-scheduled hierarchical steps (every level visit decided on the
+    device, no host read of level sizes) against the host-driven levels on a
+    2048-particle power-law Plummer sphere, several levels above 64
+    particles: every array in the same particle order, the time and the
+    Tstep/Acc interaction counts, bit for bit; graph-replayed slots equal
+    eagerly enqueued ones; SIA.evolve reaches the same state as evolve_step
+    when the steps end on t_end."""
+    from tupan_amd.ics.imf import IMF
+    from tupan_amd.ics.plummer import make_plummer
+    install_hip("float64")
+    host = make_plummer(2048, 1e-4, IMF.power_law(-2.35, 0.1, 10.0), seed=3,
+                        device="cuda:0").to_numpy()
+    g, tg, cg = _levels_run(host, method, mode, 2, 0.02 / 16, "host")
+    d, td, cd = _levels_run(host, method, mode, 2, 0.02 / 16, "device")
+    r, tr, cr = _levels_run(host, method, mode, 2, 0.02 / 16, "graph")
+    v, tv, cv = _levels_run(host, method, mode, 2, 0.02 / 16, "evolve")
+    print("%s %s: t %r, pairs host %s device %s" % (method, mode, tg, cg, cd))
+    assert tg == td == tr == tv
+    assert cg == cd == cr == cv
+    for k in g:
+        np.testing.assert_array_equal(d[k], g[k], err_msg=k)
+        np.testing.assert_array_equal(r[k], g[k], err_msg=k)
+        np.testing.assert_array_equal(v[k], g[k], err_msg=k)
+
+
+def test_device_levels_block_step_workload():
+    """The block-step workload (Plummer N = 262144, Salpeter masses,
+    eps = 1e-4, seed 1, sia21h.dkd, eta = 0.02): one step of base step
+    eta/64 with device-scheduled levels equals the host-driven levels bit
+    for bit (every array, time, interaction counts)."""
+    from tupan_amd.ics.imf import IMF
+    from tupan_amd.ics.plummer import make_plummer
+    install_hip("float64")
+    host = make_plummer(262144, 1e-4, IMF.power_law(-2.35, 0.1, 10.0), seed=1,
+                        device="cuda:0").to_numpy()
+    g, tg, cg = _levels_run(host, "sia21h.dkd", "split32", 1, 0.02 / 64,
+                            "host")
+    d, td, cd = _levels_run(host, "sia21h.dkd", "split32", 1, 0.02 / 64,
+                            "graph")
+    print("N = 262144: t %r, pairs %s" % (tg, cg))
+    assert tg == td and cg == cd
+    for k in g:
+        np.testing.assert_array_equal(d[k], g[k], err_msg=k)
+
+
 def test_torch_scalar_division_on_device():
     """Records how torch divides a device tensor by a Python number (true
     division or multiplication by the reciprocal); the device Hermite step

@@ -78,6 +78,21 @@ def _declare(lib):
     lib.tupan_sia_step.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                    _S, _PP, _P, _P, ctypes.c_double,
                                    ctypes.c_int, _P, _P, _P, _P]
+    lib.tupan_sia_levels_create.restype = ctypes.c_int
+    lib.tupan_sia_levels_create.argtypes = [
+        ctypes.c_int, _PP, ctypes.c_int, _P, ctypes.c_int, ctypes.c_int,
+        ctypes.c_double, ctypes.c_int, _P, _P, ctypes.c_int, _P, _P,
+        ctypes.POINTER(ctypes.c_void_p)]
+    lib.tupan_sia_levels_destroy.restype = ctypes.c_int
+    lib.tupan_sia_levels_destroy.argtypes = [_P]
+    lib.tupan_sia_levels_set.restype = ctypes.c_int
+    lib.tupan_sia_levels_set.argtypes = [_P, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_int]
+    lib.tupan_sia_levels_enqueue.restype = ctypes.c_int
+    lib.tupan_sia_levels_enqueue.argtypes = [_P, ctypes.c_int, ctypes.c_int,
+                                             _P]
+    lib.tupan_sia_levels_state.restype = ctypes.c_int
+    lib.tupan_sia_levels_state.argtypes = [_P, _P, _P, _P]
     lib.tupan_pnacc.restype = ctypes.c_int
     lib.tupan_pnacc.argtypes = [ctypes.c_int, _S, _PP, _S, _PP, ctypes.c_ulong,
                                 ctypes.POINTER(ctypes.c_double), _PP, _P]
@@ -231,4 +246,7 @@ DEVICE_ABI = ("tupan_last_error", "tupan_version", "tupan_device_info",
               "tupan_mlp_pack", "tupan_mlp_infer", "tupan_sia_subtree",
               "tupan_pnacc", "tupan_pnacc_f32", "tupan_host_pnacc_f32",
               "tupan_hermite_step", "tupan_sakura_graph",
-              "tupan_sakura_step", "tupan_sia_step")
+              "tupan_sakura_step", "tupan_sia_step",
+              "tupan_sia_levels_create", "tupan_sia_levels_destroy",
+              "tupan_sia_levels_set", "tupan_sia_levels_enqueue",
+              "tupan_sia_levels_state")

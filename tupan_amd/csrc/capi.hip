@@ -624,6 +624,38 @@ int tupan_sia_step(int adaptive, int acc_mode, int tstep_mode, size_t n, void* c
                          ops, coef_bridge, coef_evolve, stream);
 }
 
+int tupan_sia_levels_create(int n, void* const* arrays, int narrays, const int* roles,
+                            int acc_mode, int tstep_mode, double eta, int nbridge,
+                            const int* bridge_op, const double* bridge_c, int nevolve,
+                            const int* evolve_op, const double* evolve_c,
+                            tupan_sia_levels** plan)
+{
+    void* p = nullptr;
+    const int rc = sia_levels_create(n, arrays, narrays, roles, acc_mode, tstep_mode, eta, nbridge,
+                                     bridge_op, bridge_c, nevolve, evolve_op, evolve_c, &p);
+    if (plan) *plan = static_cast<tupan_sia_levels*>(p);
+    return rc;
+}
+
+int tupan_sia_levels_destroy(tupan_sia_levels* plan) { return sia_levels_destroy(plan); }
+
+int tupan_sia_levels_set(tupan_sia_levels* plan, double t_curr, double t_end, double eta,
+                         int clear_pairs)
+{
+    return sia_levels_set(plan, t_curr, t_end, eta, clear_pairs);
+}
+
+int tupan_sia_levels_enqueue(tupan_sia_levels* plan, int begin, int nslots, void* stream)
+{
+    return sia_levels_enqueue(plan, begin, nslots, stream);
+}
+
+int tupan_sia_levels_state(tupan_sia_levels* plan, double* step, int* state,
+                           unsigned long long* pairs)
+{
+    return sia_levels_state(plan, step, state, pairs);
+}
+
 int tupan_pnacc(int mode, size_t ni, const double* const* ifields, size_t nj,
                 const double* const* jfields, unsigned long order, const double* inv7,
                 double* const* out, void* stream)

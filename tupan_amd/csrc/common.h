@@ -66,12 +66,17 @@ struct GridPos {
     int jsplit;
 };
 
-__device__ __forceinline__ GridPos grid_pos(int nsplit)
+__device__ __forceinline__ GridPos grid_pos_of(int bid, int nsplit)
 {
     GridPos g;
-    g.jsplit = static_cast<int>(blockIdx.x) % nsplit;
-    g.iblock = static_cast<int>(blockIdx.x) / nsplit;
+    g.jsplit = bid % nsplit;
+    g.iblock = bid / nsplit;
     return g;
+}
+
+__device__ __forceinline__ GridPos grid_pos(int nsplit)
+{
+    return grid_pos_of(static_cast<int>(blockIdx.x), nsplit);
 }
 
 }  // namespace tupan

@@ -141,8 +141,10 @@ __device__ __forceinline__ void tile_pass(const typename Op::Slot* tile, int cnt
     }
 }
 
+// The tile engine's work for workgroup `bid` of a call (the body of
+// pair_kernel; also driven by the device-scheduled block steps).
 template <class Op, int BLOCK, int WPT, int TILE, int UNROLL>
-__global__ __launch_bounds__(BLOCK) void pair_kernel(typename Op::Args a)
+__device__ __forceinline__ void pair_body(const typename Op::Args& a, int bid)
 {
     using Slot = typename Op::Slot;
     using IReg = typename Op::IReg;
@@ -150,7 +152,7 @@ __global__ __launch_bounds__(BLOCK) void pair_kernel(typename Op::Args a)
     constexpr bool kUni = HasUniformE2<Op>::value;
     __shared__ Slot tile[TILE];
 
-    const GridPos g = grid_pos(a.nsplit);
+    const GridPos g = grid_pos_of(bid, a.nsplit);
     const int tid = static_cast<int>(threadIdx.x);
     const int ibase = g.iblock * (BLOCK * WPT);
     const typename Op::Frame fr = Op::frame(a, ibase);
@@ -223,6 +225,12 @@ __global__ __launch_bounds__(BLOCK) void pair_kernel(typename Op::Args a)
     }
 }
 
+template <class Op, int BLOCK, int WPT, int TILE, int UNROLL>
+__global__ __launch_bounds__(BLOCK) void pair_kernel(typename Op::Args a)
+{
+    pair_body<Op, BLOCK, WPT, TILE, UNROLL>(a, static_cast<int>(blockIdx.x));
+}
+
 // Small sets in one launch: G consecutive lanes (G divides 64) share an
 // i-particle, BLOCK / G of them per workgroup; of every LDS tile lane group
 // member g takes the slots k = g + G m (interleaved, so the group reads
@@ -231,7 +239,7 @@ __global__ __launch_bounds__(BLOCK) void pair_kernel(typename Op::Args a)
 // results are the same on every run and the call needs no j-split partials
 // or second kernel.
 template <class Op, int BLOCK, int G, int TILE>
-__global__ __launch_bounds__(BLOCK) void pair_small_kernel(typename Op::Args a)
+__device__ __forceinline__ void pair_small_body(const typename Op::Args& a, int bid)
 {
     using Slot = typename Op::Slot;
     static_assert(64 % G == 0 && TILE % G == 0 && BLOCK % 64 == 0,
@@ -241,7 +249,7 @@ __global__ __launch_bounds__(BLOCK) void pair_small_kernel(typename Op::Args a)
     const int tid = static_cast<int>(threadIdx.x);
     const int il = tid / G;
     const int g = tid % G;
-    const int ibase = static_cast<int>(blockIdx.x) * kIpb;
+    const int ibase = bid * kIpb;
     const typename Op::Frame fr = Op::frame(a, ibase);
     const int i = ibase + il;
     typename Op::IReg ir;
@@ -275,15 +283,21 @@ __global__ __launch_bounds__(BLOCK) void pair_small_kernel(typename Op::Args a)
     if (g == 0 && i < a.ni) Op::store(a, i, tot);
 }
 
+template <class Op, int BLOCK, int G, int TILE>
+__global__ __launch_bounds__(BLOCK) void pair_small_kernel(typename Op::Args a)
+{
+    pair_small_body<Op, BLOCK, G, TILE>(a, static_cast<int>(blockIdx.x));
+}
+
 // Combine the j-split partials of each i in split order and store.  The
 // partials are loaded kBatch splits at a time before they are combined, so a
 // thread has kBatch * kNacc loads in flight instead of kNacc: for small ni
 // (a few waves) the kernel is bound by memory latency, not bandwidth.
 template <class Op>
-__global__ __launch_bounds__(256) void reduce_kernel(typename Op::Args a)
+__device__ __forceinline__ void reduce_body(const typename Op::Args& a, int bid)
 {
     constexpr int kBatch = 8;
-    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const int i = bid * 256 + static_cast<int>(threadIdx.x);
     if (i >= a.ni) return;
     double tot[Op::kNacc];
     const size_t stride = static_cast<size_t>(Op::kNacc) * a.ni;
@@ -306,6 +320,12 @@ __global__ __launch_bounds__(256) void reduce_kernel(typename Op::Args a)
     Op::store(a, i, tot);
 }
 
+template <class Op>
+__global__ __launch_bounds__(256) void reduce_kernel(typename Op::Args a)
+{
+    reduce_body<Op>(a, static_cast<int>(blockIdx.x));
+}
+
 // The same combine for small ni, where one thread per i leaves most of the
 // chip idle and waits on memory latency once per batch: T consecutive lanes
 // share an i, lane t combining splits [t c, (t + 1) c) in order (c = ceil(
@@ -313,11 +333,11 @@ __global__ __launch_bounds__(256) void reduce_kernel(typename Op::Args a)
 // (lanes 2k and 2k + 1, then 4k and 4k + 2, ...), so the result is the same
 // on every run.
 template <class Op, int T>
-__global__ __launch_bounds__(256) void reduce_wide_kernel(typename Op::Args a)
+__device__ __forceinline__ void reduce_wide_body(const typename Op::Args& a, int bid)
 {
     constexpr int kBatch = 8;
     static_assert(T > 1 && (T & (T - 1)) == 0 && T <= 64, "T must be a power of two <= 64");
-    const int gid = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const int gid = bid * 256 + static_cast<int>(threadIdx.x);
     const int i = gid / T;
     const int t = gid % T;
     const bool valid = i < a.ni;
@@ -372,6 +392,12 @@ __global__ __launch_bounds__(256) void reduce_wide_kernel(typename Op::Args a)
     if (valid && t == 0) Op::store(a, i, tot);
 }
 
+template <class Op, int T>
+__global__ __launch_bounds__(256) void reduce_wide_kernel(typename Op::Args a)
+{
+    reduce_wide_body<Op, T>(a, static_cast<int>(blockIdx.x));
+}
+
 // Op::PostFallback64 after a call: every i whose stored outputs (the sums,
 // for the ops that have one) are non-finite, again in FP64 over every j.
 template <class Op>
@@ -389,6 +415,41 @@ __global__ __launch_bounds__(256) void fallback_kernel(typename Op::Args a)
     Op::store(a, i, tot);
 }
 
+// Launch constants shared by the host launcher (pairwise.hip) and the
+// device-scheduled block steps (sia.hip), which must plan every call alike.
+
+// Tile shapes {BLOCK, WPT, TILE, UNROLL} of the light interactions (Phi,
+// Acc, AccJerk, Tstep, split forms); the last, one wave per workgroup, is
+// for small systems.  The tuning table picks one per kernel, mode and size.
+constexpr int kNumShapes = 6;
+constexpr int kLightShapeDims[kNumShapes][4] = {
+    {256, 2, 256, 4}, {512, 2, 512, 2}, {256, 4, 256, 2}, {128, 2, 128, 4}, {512, 1, 512, 4},
+    {64, 1, 64, 4}};
+// Below this many i-particles the j-split combine uses 8 lanes per i.
+constexpr int kWideReduceMaxNi = 32768;
+
+// Calls of at most kSmallPairs pairs with kSmallMinJ < nj <= kSmallMaxJ
+// (in any mode but ieee64, whose sums keep the reference's order) take
+// pair_small_kernel: a few-thousand-particle system in one launch instead of
+// j-split partials and a combine kernel.  Sets of at most kSmallMinJ stay on
+// the tile engine, whose one-tile sums the device SIA subtrees reproduce bit
+// for bit; a few i against many j (block-step levels) keep the j-splits,
+// which spread j over the chip.
+constexpr long long kSmallPairs = 1ll << 21;
+constexpr int kSmallMinJ = 256;
+constexpr int kSmallMaxJ = 2048;
+constexpr int kSmallBlock = 256;
+constexpr int kSmallGroups = 64;
+// the largest power-of-two tile of at most 1024 slots within 64 KiB of LDS
+// (one tile, and one wait on global memory, for a 1024-particle set)
+template <class Op>
+constexpr int small_tile()
+{
+    int t = 1024;
+    while (t > kSmallGroups && t * static_cast<int>(sizeof(typename Op::Slot)) > 65536) t /= 2;
+    return t;
+}
+
 // Launch geometry of one all-pairs call.
 struct Launch {
     int nblock_i;  // i blocks
@@ -399,8 +460,8 @@ struct Launch {
 // Pick the j-split so that the grid has at least `target_wg` workgroups
 // (several per CU) without cutting j ranges below `min_chunk` j-particles
 // (a multiple of 16; the tile loop handles a partly filled last tile).
-inline Launch plan_launch(int ni, int nj, int block, int wpt, int tile, int target_wg,
-                          int max_split, int min_chunk)
+__host__ __device__ inline Launch plan_launch(int ni, int nj, int block, int wpt, int tile,
+                                              int target_wg, int max_split, int min_chunk)
 {
     Launch L;
     const int per_block = block * wpt;

@@ -20,30 +20,6 @@ namespace tupan {
 // step).  A run-time table, filled from the autotuner's cache, picks the
 // shape and the workgroups-per-CU target per kernel, mode and size bucket.
 // ---------------------------------------------------------------------------
-// Below this many i-particles the j-split combine uses 8 lanes per i.
-constexpr int kWideReduceMaxNi = 32768;
-
-// Calls of at most kSmallPairs pairs with kSmallMinJ < nj <= kSmallMaxJ
-// (in any mode but ieee64, whose sums keep the reference's order) take
-// pair_small_kernel: a few-thousand-particle system in one launch instead of
-// j-split partials and a combine kernel.  Sets of at most kSmallMinJ stay on
-// the tile engine, whose one-tile sums the device SIA subtrees reproduce bit
-// for bit; a few i against many j (block-step levels) keep the j-splits,
-// which spread j over the chip.
-constexpr long long kSmallPairs = 1ll << 21;
-constexpr int kSmallMinJ = 256;
-constexpr int kSmallMaxJ = 2048;
-constexpr int kSmallBlock = 256;
-constexpr int kSmallGroups = 64;
-// the largest power-of-two tile of at most 1024 slots within 64 KiB of LDS
-// (one tile, and one wait on global memory, for a 1024-particle set)
-template <class Op>
-constexpr int small_tile()
-{
-    int t = 1024;
-    while (t > kSmallGroups && t * static_cast<int>(sizeof(typename Op::Slot)) > 65536) t /= 2;
-    return t;
-}
 
 // Device memory a caller lends for the j-split partial sums instead of the
 // library's growable workspace: `doubles` values at `p`.  The launch plan
@@ -139,11 +115,16 @@ using Launcher = int (*)(typename Op::Args, hipStream_t, int, const FixedScratch
 
 // Shapes for the light interactions (Phi, Acc, AccJerk, Tstep, split); the
 // last, one wave per workgroup, is for small systems.
+template <class Op, int S>
+static int run_light(typename Op::Args a, hipStream_t s, int wg, const FixedScratch* f)
+{
+    return run_pairwise<Op, kLightShapeDims[S][0], kLightShapeDims[S][1], kLightShapeDims[S][2],
+                        kLightShapeDims[S][3]>(a, s, wg, f);
+}
 #define TUPAN_LIGHT_SHAPES(Op)                                                   \
     {                                                                            \
-        &run_pairwise<Op, 256, 2, 256, 4>, &run_pairwise<Op, 512, 2, 512, 2>,    \
-            &run_pairwise<Op, 256, 4, 256, 2>, &run_pairwise<Op, 128, 2, 128, 4>, \
-            &run_pairwise<Op, 512, 1, 512, 4>, &run_pairwise<Op, 64, 1, 64, 4>   \
+        &run_light<Op, 0>, &run_light<Op, 1>, &run_light<Op, 2>, &run_light<Op, 3>, \
+            &run_light<Op, 4>, &run_light<Op, 5>                                 \
     }
 // Shapes for SnapCrackle (twice the registers per i).
 #define TUPAN_HEAVY_SHAPES(Op)                                                   \
@@ -159,10 +140,8 @@ using Launcher = int (*)(typename Op::Args, hipStream_t, int, const FixedScratch
             &run_pairwise<Op, 256, 1, 256, 2>, &run_pairwise<Op, 64, 1, 64, 1>,  \
             &run_pairwise<Op, 512, 1, 256, 1>, &run_pairwise<Op, 64, 1, 32, 1>   \
     }
-constexpr int kShapes = 6;
-static const int kLightShape[kShapes][4] = {
-    {256, 2, 256, 4}, {512, 2, 512, 2}, {256, 4, 256, 2}, {128, 2, 128, 4}, {512, 1, 512, 4},
-    {64, 1, 64, 4}};
+constexpr int kShapes = kNumShapes;
+static const int (&kLightShape)[kShapes][4] = kLightShapeDims;
 static const int kHeavyShape[kShapes][4] = {
     {256, 1, 128, 2}, {128, 1, 128, 2}, {256, 2, 128, 1}, {64, 1, 64, 2}, {256, 1, 256, 4},
     {64, 1, 32, 1}};

@@ -144,6 +144,16 @@ int sia_step_call(int adaptive, int acc_mode, int tstep_mode, size_t n, void* co
                   double* scratch, double* control, double eta, int nops, const int* ops,
                   const double* coef_bridge, const double* coef_evolve, void* stream);
 
+// Device-scheduled hierarchical SIA steps (sia.hip).
+int sia_levels_create(int n, void* const* arrays, int narrays, const int* roles, int acc_mode,
+                      int tstep_mode, double eta, int nbridge, const int* bridge_op,
+                      const double* bridge_c, int nevolve, const int* evolve_op,
+                      const double* evolve_c, void** plan_out);
+int sia_levels_destroy(void* plan);
+int sia_levels_set(void* plan, double t_curr, double t_end, double eta, int clear_pairs);
+int sia_levels_enqueue(void* plan, int begin, int nslots, void* stream);
+int sia_levels_state(void* plan, double* step, int* state, unsigned long long* pairs);
+
 // Device SIA recursion for sets of at most 64 particles (sia.hip).
 int sia_subtree_call(int n, void* const* arrays, int narrays, const int* roles, int acc_mode,
                      int tstep_mode, double eta, double tau, int nbridge, const int* bridge_op,

@@ -260,12 +260,15 @@ class SIA(Base):
               "pn_amy", "pn_amz")
 
     def __init__(self, eta, time, ps, method, inplace=True,
-                 device_subtree=True, device_steps=False, **kwargs):
+                 device_subtree=True, device_steps=False, device_levels=False,
+                 **kwargs):
         super(SIA, self).__init__(eta, time, ps, **kwargs)
         self.inplace = inplace
         self.device_subtree = device_subtree
         self.device_steps = device_steps
+        self.device_levels = device_levels
         self._dev = None
+        self._levels = None
         self.method = method
         self.scheme = Scheme(method[:5])
         variant = method[5]
@@ -279,9 +282,25 @@ class SIA(Base):
             self._dev = DeviceSteps(self)
         return self._dev
 
+    def _device_levels(self):
+        if self._levels is None or not self._levels.valid_for(self.ps):
+            if self._levels is not None:
+                self._levels.close()
+            self._levels = DeviceLevels(self)
+        return self._levels
+
     def evolve_step(self, t_end):
         """integrator/__init__.py:80-99; with device_steps (the 's' and 'a'
-        variants) one device step and one read of the new time."""
+        variants) one device step and one read of the new time; with
+        device_levels (the 'h' variants) one device-scheduled step."""
+        if self.device_levels and not self.device_steps:
+            if not self.is_initialized:
+                self.initialize(t_end)
+            self._device_levels().step(t_end,
+                                       graphs=self.device_levels != "eager")
+            if self.reporter:
+                self.reporter.diagnostic_report(self.ps)
+            return
         if not self.device_steps:
             return super(SIA, self).evolve_step(t_end)
         if not self.is_initialized:
@@ -298,8 +317,18 @@ class SIA(Base):
         replaying a HIP graph of steps_per_graph steps; steps past t_end
         inside the last replay change nothing.  Returns the number of
         replays."""
+        if self.device_levels and not self.device_steps:
+            if steps_per_graph < 1:
+                raise ValueError("steps_per_graph must be at least 1")
+            if not self.is_initialized:
+                self.initialize(t_end)
+            replays = self._device_levels().evolve(t_end)
+            if self.reporter:
+                self.reporter.diagnostic_report(self.ps)
+            return replays
         if not self.device_steps:
-            raise RuntimeError("evolve needs device_steps=True")
+            raise RuntimeError("evolve needs device_steps=True or "
+                               "device_levels=True")
         if steps_per_graph < 1:
             raise ValueError("steps_per_graph must be at least 1")
         if not self.is_initialized:
@@ -466,6 +495,189 @@ class DeviceSteps(object):
                     self.launch()
         torch.cuda.current_stream().wait_stream(side)
         return graph
+
+
+class DeviceLevels(object):
+    """Hierarchical ('h') SIA steps walked on the device
+    (tupan_sia_levels_*): the level visits, partitions, drifts, kicks and
+    pairwise calls of the host driver, scheduled from device memory one
+    operation per slot, so a step needs no host read of level sizes.  A step
+    runs as slots in batches of `slots`, replayed from two captured HIP
+    graphs (the step's start plus a batch, and a batch) or, with
+    SIA(..., device_levels="eager"), enqueued call by call, until the device
+    reports the step done."""
+
+    STATUS = {1: "Kepler solver failure", 2: "more than 64 levels",
+              3: "more than 2^20 level visits",
+              4: "operation queue overflow", 16: "scheduler bound",
+              32: "no valid step size"}
+
+    def __init__(self, sia, slots=256):
+        from .. import _native
+        from ..lib import extensions
+        from ..lib.hip_backend import HIPKernel
+        ps = sia.ps
+        if not (sia.inplace and sia.update_tstep and not sia.shared_tstep):
+            raise RuntimeError("device levels run the hierarchical 'h' "
+                               "variants in place")
+        if ps.include_pn_corrections:
+            raise RuntimeError("device levels have no PN corrections")
+        modes = []
+        for ext in (extensions.acc, extensions.tstep):
+            k = getattr(ext, "kernel", None)
+            if not isinstance(k, HIPKernel) or k.prec != "float64" or \
+                    k.mode not in ("fp64", "split32"):
+                raise RuntimeError("device levels need the HIP backend in "
+                                   "float64, fp64 or split32 mode")
+            modes.append(_native.MODES[k.mode])
+        if ps.n <= SUBTREE_MAX:
+            raise RuntimeError("device levels need more than %d particles"
+                               % SUBTREE_MAX)
+        for name in sia.AUX:
+            if name not in ps.__dict__:
+                ps.register_auxiliary_attribute(name, "real")
+        arrays = ps.arrays()
+        if not all(t.is_cuda and t.element_size() == 8
+                   for t in arrays.values()):
+            raise RuntimeError("device levels need device arrays of 8-byte "
+                               "elements")
+        if any(r not in arrays for r in SUBTREE_ROLES) or len(arrays) > 32:
+            raise RuntimeError("device levels need the SIA arrays, at most "
+                               "32 of them")
+        self.lib = _native.lib()
+        self.native = _native
+        self.sia = sia
+        self.ps = ps
+        self.names = list(arrays)
+        self.ptrs = [t.data_ptr() for t in arrays.values()]
+        self.slots = slots
+        roles = (ctypes.c_int * len(SUBTREE_ROLES))(
+            *[self.names.index(r) for r in SUBTREE_ROLES])
+        seq = palindrome(sia.scheme.d, sia.scheme.k)
+        bop = (ctypes.c_int * len(seq))(*[w for w, _ in seq])
+        bc = (ctypes.c_double * len(seq))(*[c for _, c in seq])
+        if sia.method.endswith("dkd"):
+            ev = [(0 if w == 0 else 1, c) for w, c in seq]
+        else:
+            ev = [(1 if w == 0 else 0, c) for w, c in seq]
+        eop = (ctypes.c_int * len(ev))(*[w for w, _ in ev])
+        ec = (ctypes.c_double * len(ev))(*[c for _, c in ev])
+        self.plan = ctypes.c_void_p()
+        rc = self.lib.tupan_sia_levels_create(
+            ps.n, _native.ptr_array(self.ptrs), len(self.names),
+            ctypes.addressof(roles), modes[0], modes[1], float(sia.eta),
+            len(seq), ctypes.addressof(bop), ctypes.addressof(bc), len(ev),
+            ctypes.addressof(eop), ctypes.addressof(ec),
+            ctypes.byref(self.plan))
+        _native.check(rc, "tupan_sia_levels_create")
+        self.graphs = None
+
+    def valid_for(self, ps):
+        arrays = ps.arrays()
+        return (ps is self.ps and list(arrays) == self.names
+                and [t.data_ptr() for t in arrays.values()] == self.ptrs)
+
+    def close(self):
+        if self.plan:
+            self.lib.tupan_sia_levels_destroy(self.plan)
+            self.plan = ctypes.c_void_p()
+        self.graphs = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_control(self, t_end, clear_pairs=True):
+        rc = self.lib.tupan_sia_levels_set(
+            self.plan, float(type(self.ps).t_curr), float(t_end),
+            float(self.sia.eta), 1 if clear_pairs else 0)
+        self.native.check(rc, "tupan_sia_levels_set")
+
+    def enqueue(self, begin):
+        from ..lib.hip_backend import current_stream_handle
+        rc = self.lib.tupan_sia_levels_enqueue(
+            self.plan, 1 if begin else 0, self.slots, current_stream_handle())
+        self.native.check(rc, "tupan_sia_levels_enqueue")
+
+    def state(self):
+        """(t_curr, done, visits, (Tstep, Acc) interactions); raises if the
+        walk failed.  One synchronisation.  Also keeps the step's level
+        visits above 64 particles and slots that did work in
+        last_visits and last_ops."""
+        step = (ctypes.c_double * 5)()
+        st = (ctypes.c_int * 4)()
+        pairs = (ctypes.c_ulonglong * 2)()
+        rc = self.lib.tupan_sia_levels_state(self.plan, step, st, pairs)
+        self.native.check(rc, "tupan_sia_levels_state")
+        if st[1]:
+            bits = [m for b, m in sorted(self.STATUS.items()) if st[1] & b]
+            raise self.native.TupanHipError(
+                "device SIA levels failed (status %d: %s)"
+                % (st[1], ", ".join(bits) or "?"))
+        self.last_visits, self.last_ops = st[2], st[3]
+        return step[0], bool(st[0]), st[2], (pairs[0], pairs[1])
+
+    def _run(self, first, more):
+        """One step: `first` enqueues or replays the start and a batch,
+        `more` a further batch, until the device reports the step done."""
+        from ..lib import extensions
+        first()
+        t, done, _, pairs = self.state()
+        batches = 1
+        while not done:
+            more()
+            t, done, _, pairs = self.state()
+            batches += 1
+        type(self.ps).t_curr = t
+        if extensions.pair_counts is not None:
+            extensions.pair_counts["Tstep"] += pairs[0]
+            extensions.pair_counts["Acc"] += pairs[1]
+        return batches
+
+    def step(self, t_end, graphs=True):
+        """One step: slots replayed from the two captured graphs (captured
+        on first use), or enqueued eagerly.  Returns the replays or
+        batches."""
+        self.set_control(t_end)
+        if not graphs:
+            return self._run(lambda: self.enqueue(True),
+                             lambda: self.enqueue(False))
+        if self.graphs is None:
+            self.graphs = self.capture()
+        return self._run(self.graphs[0].replay, self.graphs[1].replay)
+
+    def capture(self):
+        graphs = []
+        for begin in (True, False):
+            g = torch.cuda.CUDAGraph()
+            side = torch.cuda.Stream()
+            side.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(side):
+                with torch.cuda.graph(g, stream=side):
+                    self.enqueue(begin)
+            torch.cuda.current_stream().wait_stream(side)
+            graphs.append(g)
+        return graphs
+
+    def evolve(self, t_end):
+        """Steps to t_end from the captured graphs; returns the number of
+        graph replays."""
+        if self.graphs is None:
+            self.graphs = self.capture()
+        g_first, g_more = self.graphs
+        replays = 0
+        t = type(self.ps).t_curr
+        while (t < t_end) if self.sia.eta > 0 else (t > t_end):
+            self.set_control(t_end)
+            replays += self._run(g_first.replay, g_more.replay)
+            t_new = type(self.ps).t_curr
+            if t_new == t:
+                raise self.native.TupanHipError(
+                    "device SIA levels made no progress at t = %r" % t)
+            t = t_new
+        return replays
 
 
 class _DeviceSubtree(object):
