@@ -344,7 +344,7 @@ def sakura_forms(n, net_path, reps=3):
     import torch
     from tupan_amd import _native
     from tupan_amd.ics.plummer import make_plummer
-    from tupan_amd.learned import KeplerNet, sakura_kernel
+    from tupan_amd.learned import KeplerNet, SolveLogBuffer, sakura_kernel
     ps = make_plummer(n, 0.0, ("equalmass",), seed=1, device="cuda:0")
     forms = [("one_pass", "one-pass", None), ("two_pass", "two-pass", None)]
     if net_path and os.path.exists(net_path):
@@ -377,6 +377,19 @@ def sakura_forms(n, net_path, reps=3):
                      "guess_fallbacks": c[4], "near_pairs":
                          ext.kernel.near_pairs,
                      "solves_per_s": c[0] / (best[0] / 1e3)}
+        if path == "two-pass":
+            # one more call with a solve log: the Laguerre steps of every
+            # near pair, and the mean over waves of 64 pairs of the slowest
+            # pair's steps (a wave runs as long as its slowest lane)
+            log = SolveLogBuffer(ext.kernel.near_pairs + 64)
+            ext.kernel.solve_log = log
+            ext.calc(ps, ps, 1.0 / 64, -1)
+            ext.kernel.solve_log = None
+            k = ext.kernel.near_pairs
+            steps = log.iterations[:k].cpu().numpy()
+            waves = steps[:(k // 64) * 64].reshape(-1, 64)
+            out[name]["wave_max_iterations"] = float(waves.max(1).mean())
+            out[name]["max_iterations"] = int(steps.max())
     ref = results["one_pass"]
 
     def rel(a):
@@ -395,6 +408,15 @@ def sakura_forms(n, net_path, reps=3):
     if "two_pass_learned" in out:
         out["learned_speedup_vs_two_pass"] = out["two_pass"]["kernel_s"] / \
             out["two_pass_learned"]["kernel_s"]
+        out["learned_target"] = {
+            "speedup": 1.5,
+            "stopping_rule": "a solve ends after a Laguerre step whose "
+                             "correction is at most 2^-42 of s, so it takes "
+                             "at least 2 steps unless the guess is already "
+                             "that close",
+            "near_pass_share": "Kepler solves are the near pass only; the "
+                               "far-pair leapfrog kernel is the rest of the "
+                               "call"}
     return out
 
 

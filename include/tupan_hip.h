@@ -394,9 +394,10 @@ int tupan_kepler_pairs(size_t npairs, const int *idx,
  *   path   0: automatic (two-pass for ni * nj >= 2^22), 1: one-pass,
  *          2: two-pass (far pairs in the tile loop, near pairs compacted
  *          into a dense list and solved one per thread)
- *   net    packed Kepler-guess network (tupan_mlp_pack) or NULL: the first
- *          solve of each near pair starts from the network's guess and
- *          falls back to the plain guess after 8 Laguerre steps
+ *   net    packed Kepler-guess network (tupan_mlp_pack; device, 16-byte
+ *          aligned) or NULL: the near pass evaluates the network on each
+ *          near pair's first solve, which starts from the network's guess
+ *          and falls back to the plain guess after 8 Laguerre steps
  *   log    NULL or a device log of the near pairs' first solves
  *   counters  NULL or 5 device counters: solves, Laguerre steps,
  *          subdivisions, leapfrog pairs, guess fallbacks

@@ -754,6 +754,8 @@ int tupan_sakura_ex(size_t ni, const double* const* ifields, size_t nj,
 {
     if (path < 0 || path > 2) return set_error(TUPAN_ERR_ARG, "path must be 0, 1 or 2");
     if (!ifields || !jfields || !out) return set_error(TUPAN_ERR_ARG, "null field list");
+    if (reinterpret_cast<uintptr_t>(net) % 16 != 0)
+        return set_error(TUPAN_ERR_ARG, "the packed network must be 16-byte aligned");
     SakuraOptions opt;
     opt.path = path;
     opt.net = net;

@@ -20,12 +20,11 @@
 //   mlp_infer_kernel     BF16 inference with FP32 accumulation; the 64x64
 //       layer runs on the matrix cores (v_mfma_f32_32x32x8_bf16), the two
 //       thin layers on the vector ALUs
+#include "mlp_wave.h"
 #include "tupan_internal.h"
 
 namespace tupan {
 
-constexpr int kHidden = 64;
-constexpr int kIn = 4;
 constexpr int kOffW1 = 0;
 constexpr int kOffB1 = kOffW1 + kHidden * kIn;       // 256
 constexpr int kOffW2 = kOffB1 + kHidden;             // 320
@@ -35,48 +34,12 @@ constexpr int kOffB3 = kOffW3 + kHidden;             // 4544
 constexpr int kNetParams = kOffB3 + 1;               // 4545
 constexpr int kNormFloats = 10;
 
-// Packed BF16 inference block.
-struct NetPacked {
-    float mean[4];
-    float inv_std[4];
-    float out_scale, out_shift;
-    float pad0[6];
-    float b1[kHidden];
-    float b2[kHidden];
-    float b3;
-    float pad1[15];
-    unsigned short w1[kHidden * kIn];
-    unsigned short w2[kHidden * kHidden];
-    unsigned short w3[kHidden];
-};
-
-// Round to nearest even BF16 (finite inputs; NaN stays NaN).
-__host__ __device__ __forceinline__ unsigned short to_bf16(float x)
-{
-    unsigned int u = __builtin_bit_cast(unsigned int, x);
-    if ((u & 0x7f800000u) == 0x7f800000u) return static_cast<unsigned short>(u >> 16 | ((u & 0xffffu) ? 0x40u : 0u));
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return static_cast<unsigned short>(u >> 16);
-}
-
-__device__ __forceinline__ float from_bf16(unsigned short h)
-{
-    return __builtin_bit_cast(float, static_cast<unsigned int>(h) << 16);
-}
-
-__device__ __forceinline__ float sigmoid(float z) { return 1.0f / (1.0f + __expf(-z)); }
-
-// x = (log tau, sigma, kappa, eps), standardised.
+// x = (log tau, sigma, kappa, eps) of item t of the SoA features, standardised.
 __device__ __forceinline__ void prepare(const float* feat, size_t n, size_t t,
                                         const float* mean, const float* inv_std, float x[4])
 {
-    const float tau = feat[0 * n + t];
-    x[0] = __logf(fmaxf(tau, 1e-30f));
-    x[1] = feat[1 * n + t];
-    x[2] = feat[2 * n + t];
-    x[3] = feat[3 * n + t];
-#pragma unroll
-    for (int c = 0; c < 4; ++c) x[c] = (x[c] - mean[c]) * inv_std[c];
+    prepare_features(feat[0 * n + t], feat[1 * n + t], feat[2 * n + t], feat[3 * n + t], mean,
+                     inv_std, x);
 }
 
 // ---------------------------------------------------------------------------
@@ -289,92 +252,36 @@ __global__ __launch_bounds__(256) void mlp_pack_kernel(const float* params, cons
 }
 
 // ---------------------------------------------------------------------------
-// BF16 inference.  One wave handles 64 items per step: lanes l and l + 32
-// share item (l mod 32) (+ 32 for the second column tile) and each computes
-// half of its 64 layer-1 activations, exactly the layer-2 B operand it
-// supplies:
-//   v_mfma_f32_32x32x8_bf16:  A[m][k] from lane m + 32 (k / 4), element k % 4
-//                             B[k][n] from lane n + 32 (k / 4), element k % 4
-//                             D[m][n] in lane n + 32 ((m / 4) % 2),
-//                                     element (m % 4) + 4 (m / 8)
-// Layer 2 is D = W2 (64 x 64, hidden x k) times H1^T (64 x 64, k x item):
-// 2 x 2 output tiles of 32 x 32, 8 k-steps each.  D's column is the item,
-// so layer 3 (a dot over the hidden rows) sums within the lane and across
-// the lane pair (l, l ^ 32).
+// BF16 inference: one wave per 64 items (mlp_wave.h: mlp_wave_eval), the
+// packed network staged in LDS.  No loop over item groups: the weights are
+// then read where they are used instead of being hoisted into registers.
 // ---------------------------------------------------------------------------
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
-typedef float f32x16 __attribute__((ext_vector_type(16)));
-
 __global__ __launch_bounds__(256) void mlp_infer_kernel(int n, const float* feat,
-                                                        const NetPacked* net, float* delta)
+                                                        const NetPacked* gnet, float* delta)
 {
+    __shared__ __align__(16) NetPacked lnet;
+    mlp_stage(gnet, &lnet);
+    const NetPacked* net = &lnet;
     const int lane = static_cast<int>(threadIdx.x) & 63;
-    const int wave = static_cast<int>(threadIdx.x) >> 6;
-    const int half = lane >> 5;  // k / 4 parity of this lane's operand slices
+    const int half = lane >> 5;
     const int col = lane & 31;
-    // A operands (W2) for both row tiles and all eight k-steps
-    bf16x4 A[2][8];
-#pragma unroll
-    for (int r = 0; r < 2; ++r) {
-#pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            const int m = 32 * r + col;
-            const int k0 = 8 * s + 4 * half;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) A[r][s][e] = static_cast<short>(net->w2[m * kHidden + k0 + e]);
-        }
-    }
     const size_t nn = static_cast<size_t>(n);
-    const int waves_total = static_cast<int>(gridDim.x) * 4;
-    for (int grp = static_cast<int>(blockIdx.x) * 4 + wave; grp * 64 < n; grp += waves_total) {
-        bf16x4 B[2][8];
+    const size_t grp = static_cast<size_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    float x[2][4];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const size_t item = grp * 64 + 32 * c + col;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) x[c][q] = 0.f;
+        if (item < nn) prepare(feat, nn, item, net->mean, net->inv_std, x[c]);
+    }
+    float y[2];
+    mlp_wave_eval(net, x, lane, y);
+    if (half == 0) {
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
-            const int item = grp * 64 + 32 * c + col;
-            float x[4] = {0.f, 0.f, 0.f, 0.f};
-            if (item < n) prepare(feat, nn, static_cast<size_t>(item), net->mean, net->inv_std, x);
-            float xb[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) xb[q] = from_bf16(to_bf16(x[q]));
-#pragma unroll
-            for (int s = 0; s < 8; ++s) {
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int k = 8 * s + 4 * half + e;
-                    float z = net->b1[k];
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) z = fmaf(from_bf16(net->w1[k * kIn + q]), xb[q], z);
-                    B[c][s][e] = static_cast<short>(to_bf16(z * sigmoid(z)));
-                }
-            }
-        }
-        float y[2] = {0.f, 0.f};
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-#pragma unroll
-            for (int r = 0; r < 2; ++r) {
-                f32x16 acc;
-#pragma unroll
-                for (int v = 0; v < 16; ++v) acc[v] = 0.f;
-#pragma unroll
-                for (int s = 0; s < 8; ++s)
-                    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(A[r][s], B[c][s], acc, 0, 0, 0);
-#pragma unroll
-                for (int v = 0; v < 16; ++v) {
-                    const int m = 32 * r + (v % 4) + 8 * (v / 4) + 4 * half;
-                    const float z = acc[v] + net->b2[m];
-                    const float h = from_bf16(to_bf16(z * sigmoid(z)));
-                    y[c] = fmaf(from_bf16(net->w3[m]), h, y[c]);
-                }
-            }
-            y[c] += __shfl_xor(y[c], 32, 64);
-        }
-        if (half == 0) {
-#pragma unroll
-            for (int c = 0; c < 2; ++c) {
-                const int item = grp * 64 + 32 * c + col;
-                if (item < n) delta[item] = (y[c] + net->b3) * net->out_scale + net->out_shift;
-            }
+            const size_t item = grp * 64 + 32 * c + col;
+            if (item < nn) delta[item] = mlp_delta(net, y[c]);
         }
     }
 }
@@ -383,10 +290,10 @@ int mlp_infer(int n, const float* feat, const void* net, float* delta, hipStream
 {
     if (n <= 0) return TUPAN_OK;
     if (!feat || !net || !delta) return set_error(TUPAN_ERR_ARG, "null array");
+    if (reinterpret_cast<uintptr_t>(net) % 16 != 0)
+        return set_error(TUPAN_ERR_ARG, "the packed network must be 16-byte aligned");
     const int groups = (n + 63) / 64;
-    int grid = (groups + 3) / 4;
-    const int cap = 4 * current_device().cu_count;
-    if (grid > cap) grid = cap;
+    const int grid = (groups + 3) / 4;
     hipLaunchKernelGGL(mlp_infer_kernel, dim3(grid), dim3(256), 0, stream, n, feat,
                        static_cast<const NetPacked*>(net), delta);
     return check_launch("mlp_infer");

@@ -13,6 +13,7 @@
 // Compiled with -ffp-contract=off (reference operation order).
 #include "engine.h"
 #include "kepler.h"
+#include "mlp_wave.h"
 #include "tupan_internal.h"
 
 namespace tupan {
@@ -411,37 +412,76 @@ __global__ void near_capacity_kernel(const int* total, int cap, int* status)
     if (threadIdx.x == 0 && total[0] > cap && status) atomicOr(status, 2);
 }
 
-// Features of each near pair's first stage, SoA [4][nitems] (kepler.h:
-// SolveFeatures); NaN for a pair whose first stage has no solve setup.
-__global__ __launch_bounds__(128) void near_features_kernel(SakuraArgs a, int nitems,
-                                                            const int2* items, float* feat)
+// One near pair, item t of the list: the pair's drift (from the learned
+// guess when `guess` is set and usable), its six contributions (contrib is
+// SoA [6][nitems]) and, with a log, the first stage's features, the learned
+// target and the pair's Laguerre steps.
+__device__ __forceinline__ void near_pair(const SakuraArgs& a, int t, int nitems,
+                                          const int2* items, double* contrib, bool guess,
+                                          float d, const SolveLogDev& log, kep::Counters& c)
 {
-    const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    if (t >= nitems) return;
     const int2 it = items[t];
     const int i = it.x, j = it.y;
-    double q[3] = {a.ip[1][i] - a.jp[1][j], a.ip[2][i] - a.jp[2][j], a.ip[3][i] - a.jp[3][j]};
+    const double p0[3] = {a.ip[1][i] - a.jp[1][j], a.ip[2][i] - a.jp[2][j],
+                          a.ip[3][i] - a.jp[3][j]};
     const double e2 = a.ip[4][i] + a.jp[4][j];
     const double v0[3] = {a.ip[5][i] - a.jp[5][j], a.ip[6][i] - a.jp[6][j],
                           a.ip[7][i] - a.jp[7][j]};
-    const double m = a.ip[0][i] + a.jp[0][j];
-    const double dts = kep::first_stage(sakura_dt(a), a.flag, q, v0);
-    double k[5], s0;
-    kep::SolveFeatures f;
-    if (!kep::solve_setup(dts, m, e2, q, v0, k, s0, &f)) {
-        f.tau = f.sigma = f.kappa = f.eps = __builtin_nanf("");
+    const double mj = a.jp[0][j];
+    const double m = a.ip[0][i] + mj;
+    double p[3] = {p0[0], p0[1], p0[2]};
+    double v[3] = {v0[0], v0[1], v0[2]};
+    kep::Guess gs;
+    if (guess) {
+        gs.use = d == d && d > -0.9f && d < 8.0f;
+        gs.delta = static_cast<double>(d);
     }
+    const bool logging = log.target != nullptr && t < log.capacity;
+    double s_conv = __builtin_nan("");
+    kep::drift_twobody(sakura_dt(a), a.flag, m, e2, p, v, c, guess ? &gs : nullptr,
+                       logging ? &s_conv : nullptr);
+    const double mu = mj / m;
     const size_t n = static_cast<size_t>(nitems);
-    feat[0 * n + t] = f.tau;
-    feat[1 * n + t] = f.sigma;
-    feat[2 * n + t] = f.kappa;
-    feat[3 * n + t] = f.eps;
+    contrib[0 * n + t] = mu * (p[0] - p0[0]);
+    contrib[1 * n + t] = mu * (p[1] - p0[1]);
+    contrib[2 * n + t] = mu * (p[2] - p0[2]);
+    contrib[3 * n + t] = mu * (v[0] - v0[0]);
+    contrib[4 * n + t] = mu * (v[1] - v0[1]);
+    contrib[5 * n + t] = mu * (v[2] - v0[2]);
+    if (logging) {
+        double q[3] = {p0[0], p0[1], p0[2]};
+        const double dts = kep::first_stage(sakura_dt(a), a.flag, q, v0);
+        double k[5], s0;
+        kep::SolveFeatures f{};
+        float target = __builtin_nanf("");
+        if (kep::solve_setup(dts, m, e2, q, v0, k, s0, &f) && s_conv == s_conv)
+            target = static_cast<float>(s_conv * k[1] / k[0] - 1);
+        const size_t cap = static_cast<size_t>(log.capacity);
+        log.features[0 * cap + t] = f.tau;
+        log.features[1 * cap + t] = f.sigma;
+        log.features[2 * cap + t] = f.kappa;
+        log.features[3 * cap + t] = f.eps;
+        log.target[t] = target;
+        log.iterations[t] = static_cast<int>(c.iters);
+    }
 }
 
-// Pass 2: one near pair per thread.  contrib is SoA [6][nitems].  With
-// `delta` the first solve of the pair starts from the learned guess
-// (kepler.h: Guess).  With a log, the first stage's features, the learned
-// target and the pair's Laguerre steps are recorded per item.
+// The near pass's counters, fallbacks and failure status.
+__device__ __forceinline__ void near_flush(const SakuraArgs& a, bool valid,
+                                           const kep::Counters& c,
+                                           unsigned long long* fallbacks)
+{
+    flush_counters(a.counters, valid, c, static_cast<int>(threadIdx.x));
+    if (fallbacks) {
+        const unsigned int fb = wave_sum(valid ? c.fallbacks : 0u);
+        if ((threadIdx.x & (kWave - 1)) == 0 && fb)
+            atomicAdd(fallbacks, static_cast<unsigned long long>(fb));
+    }
+    if (a.status && valid && c.fail) atomicOr(a.status, 1);
+}
+
+// Pass 2: one near pair per thread.  With `delta` the first solve of the
+// pair starts from the learned guess (kepler.h: Guess).
 __global__ __launch_bounds__(128) void sakura_near_kernel(SakuraArgs a, int nitems,
                                                           const int* nitems_dev,
                                                           const int2* items, double* contrib,
@@ -454,60 +494,62 @@ __global__ __launch_bounds__(128) void sakura_near_kernel(SakuraArgs a, int nite
     const int count = nitems_dev ? min(*nitems_dev, nitems) : nitems;
     const bool valid = t < count;
     kep::Counters c;
+    if (valid)
+        near_pair(a, t, nitems, items, contrib, delta != nullptr, delta ? delta[t] : 0.f, log,
+                  c);
+    near_flush(a, valid, c, fallbacks);
+}
+
+// Pass 2 with the learned guess evaluated in the same kernel: each wave
+// forms its 64 pairs' first-stage features (kepler.h: SolveFeatures; NaN for
+// a pair whose first stage has no solve setup), runs the network on them
+// with the stand-alone inference kernel's arithmetic (mlp_wave.h), then
+// solves its pairs from the guesses.  The packed network is staged in LDS.
+__global__ __launch_bounds__(128, 2) void sakura_near_learned_kernel(SakuraArgs a, int nitems,
+                                                                  const int2* items,
+                                                                  double* contrib,
+                                                                  const NetPacked* net,
+                                                                  SolveLogDev log,
+                                                                  unsigned long long* fallbacks)
+{
+    __shared__ __align__(16) NetPacked lnet;
+    mlp_stage(net, &lnet);
+    const int t = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const int lane = static_cast<int>(threadIdx.x) & (kWave - 1);
+    const bool valid = t < nitems;
+    float xs[4] = {0.f, 0.f, 0.f, 0.f};
     if (valid) {
         const int2 it = items[t];
         const int i = it.x, j = it.y;
-        const double p0[3] = {a.ip[1][i] - a.jp[1][j], a.ip[2][i] - a.jp[2][j],
-                              a.ip[3][i] - a.jp[3][j]};
+        double q[3] = {a.ip[1][i] - a.jp[1][j], a.ip[2][i] - a.jp[2][j],
+                       a.ip[3][i] - a.jp[3][j]};
         const double e2 = a.ip[4][i] + a.jp[4][j];
         const double v0[3] = {a.ip[5][i] - a.jp[5][j], a.ip[6][i] - a.jp[6][j],
                               a.ip[7][i] - a.jp[7][j]};
-        const double mj = a.jp[0][j];
-        const double m = a.ip[0][i] + mj;
-        double p[3] = {p0[0], p0[1], p0[2]};
-        double v[3] = {v0[0], v0[1], v0[2]};
-        kep::Guess gs;
-        if (delta) {
-            const float d = delta[t];
-            gs.use = d == d && d > -0.9f && d < 8.0f;
-            gs.delta = static_cast<double>(d);
-        }
-        const bool logging = log.target != nullptr && t < log.capacity;
-        double s_conv = __builtin_nan("");
-        kep::drift_twobody(sakura_dt(a), a.flag, m, e2, p, v, c, delta ? &gs : nullptr,
-                           logging ? &s_conv : nullptr);
-        const double mu = mj / m;
-        const size_t n = static_cast<size_t>(nitems);
-        contrib[0 * n + t] = mu * (p[0] - p0[0]);
-        contrib[1 * n + t] = mu * (p[1] - p0[1]);
-        contrib[2 * n + t] = mu * (p[2] - p0[2]);
-        contrib[3 * n + t] = mu * (v[0] - v0[0]);
-        contrib[4 * n + t] = mu * (v[1] - v0[1]);
-        contrib[5 * n + t] = mu * (v[2] - v0[2]);
-        if (logging) {
-            double q[3] = {p0[0], p0[1], p0[2]};
-            const double dts = kep::first_stage(sakura_dt(a), a.flag, q, v0);
-            double k[5], s0;
-            kep::SolveFeatures f{};
-            float target = __builtin_nanf("");
-            if (kep::solve_setup(dts, m, e2, q, v0, k, s0, &f) && s_conv == s_conv)
-                target = static_cast<float>(s_conv * k[1] / k[0] - 1);
-            const size_t cap = static_cast<size_t>(log.capacity);
-            log.features[0 * cap + t] = f.tau;
-            log.features[1 * cap + t] = f.sigma;
-            log.features[2 * cap + t] = f.kappa;
-            log.features[3 * cap + t] = f.eps;
-            log.target[t] = target;
-            log.iterations[t] = static_cast<int>(c.iters);
-        }
+        const double m = a.ip[0][i] + a.jp[0][j];
+        const double dts = kep::first_stage(sakura_dt(a), a.flag, q, v0);
+        double k[5], s0;
+        kep::SolveFeatures f;
+        if (!kep::solve_setup(dts, m, e2, q, v0, k, s0, &f))
+            f.tau = f.sigma = f.kappa = f.eps = __builtin_nanf("");
+        prepare_features(f.tau, f.sigma, f.kappa, f.eps, lnet.mean, lnet.inv_std, xs);
     }
-    flush_counters(a.counters, valid, c, static_cast<int>(threadIdx.x));
-    if (fallbacks) {
-        const unsigned int fb = wave_sum(valid ? c.fallbacks : 0u);
-        if ((threadIdx.x & (kWave - 1)) == 0 && fb)
-            atomicAdd(fallbacks, static_cast<unsigned long long>(fb));
+    // lanes l and l + 32 supply the B operands of items l mod 32 and 32 + l mod 32
+    float x[2][4];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) x[c][q] = __shfl(xs[q], 32 * c + (lane & 31), kWave);
     }
-    if (a.status && valid && c.fail) atomicOr(a.status, 1);
+    float y[2];
+    mlp_wave_eval(&lnet, x, lane, y);
+    const float d = mlp_delta(&lnet, lane < 32 ? y[0] : y[1]);
+    // the solve reloads the pair rather than keep the feature pass's values
+    // live across the network (register pressure)
+    __asm__ volatile("" ::: "memory");
+    kep::Counters c;
+    if (valid) near_pair(a, t, nitems, items, contrib, true, d, log, c);
+    near_flush(a, valid, c, fallbacks);
 }
 
 // Pass 3: add each (split, i)'s near contributions, in list order, to its
@@ -782,29 +824,25 @@ int sakura_call_ex(size_t ni, const double* const* ip, size_t nj, const double* 
     const int total = counts[0];
     opt.near_pairs = total;
     if (total > 0) {
-        // workspace slot 2: items, contributions, features and guesses
+        // workspace slot 2: items and contributions
         const size_t nt = static_cast<size_t>(total);
         const size_t b_items = up(nt * sizeof(int2));
         const size_t b_con = up(nt * 6 * sizeof(double));
-        const size_t b_feat = opt.net ? up(nt * 4 * sizeof(float)) : 0;
-        const size_t b_delta = opt.net ? up(nt * sizeof(float)) : 0;
-        char* w2 = static_cast<char*>(workspace(b_items + b_con + b_feat + b_delta, s, 2));
+        char* w2 = static_cast<char*>(workspace(b_items + b_con, s, 2));
         if (!w2) return set_error(TUPAN_ERR_ALLOC, "workspace allocation failed");
         int2* items = reinterpret_cast<int2*>(w2);
         double* contrib = reinterpret_cast<double*>(w2 + b_items);
-        float* feat = opt.net ? reinterpret_cast<float*>(w2 + b_items + b_con) : nullptr;
-        float* delta = opt.net ? reinterpret_cast<float*>(w2 + b_items + b_con + b_feat) : nullptr;
         hipLaunchKernelGGL(near_compact_kernel, dim3(eg), dim3(256), 0, s, n, a.ni,
                            a.near_count, near_off, a.near_j, items, total);
         const unsigned ig = static_cast<unsigned>((total + 127) / 128);
         if (opt.net) {
-            hipLaunchKernelGGL(near_features_kernel, dim3(ig), dim3(128), 0, s, a, total, items,
-                               feat);
-            rc = mlp_infer(total, feat, opt.net, delta, s);
-            if (rc != TUPAN_OK) return rc;
+            hipLaunchKernelGGL(sakura_near_learned_kernel, dim3(ig), dim3(128), 0, s, a, total,
+                               items, contrib, static_cast<const NetPacked*>(opt.net), opt.log,
+                               opt.fallbacks);
+        } else {
+            hipLaunchKernelGGL(sakura_near_kernel, dim3(ig), dim3(128), 0, s, a, total, nullptr,
+                               items, contrib, nullptr, opt.log, opt.fallbacks);
         }
-        hipLaunchKernelGGL(sakura_near_kernel, dim3(ig), dim3(128), 0, s, a, total, nullptr,
-                           items, contrib, delta, opt.log, opt.fallbacks);
         hipLaunchKernelGGL(near_add_kernel, dim3(eg), dim3(256), 0, s, n, a.ni, total,
                            a.near_count, near_off, contrib, a.partial);
     }
