@@ -314,6 +314,19 @@ int tupan_pairwise_f32(int kernel, int mode, size_t ni,
                        const float *const *jfields, float *const *out,
                        float eta, void *stream);
 
+/* Frees the scratch workspaces of the current device after a device
+ * synchronisation: the current ones and those retired when they grew (no
+ * counterpart in the reference, whose C backend allocates nothing on a
+ * device).  The library's pairwise calls keep one growable workspace per
+ * device; an outgrown buffer stays allocated until this call because HIP
+ * graphs captured earlier still address it, so graphs that contain library
+ * launches must be captured again after it.  The next call allocates
+ * afresh.  Calls on different streams that use the same workspace are
+ * ordered on the device (the later stream waits for the earlier one's work)
+ * unless a stream is capturing a graph: a capture must not overlap other
+ * streams' calls. */
+int tupan_release_workspaces(void);
+
 /* tupan_pairwise without allocation (no counterpart in the reference; for
  * callers that cannot allocate device memory: code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc).  Same

@@ -49,6 +49,8 @@ def _declare(lib):
     lib.tupan_timing_read.restype = ctypes.c_int
     lib.tupan_timing_read.argtypes = [ctypes.POINTER(ctypes.c_double),
                                       ctypes.POINTER(ctypes.c_int)]
+    lib.tupan_release_workspaces.restype = ctypes.c_int
+    lib.tupan_release_workspaces.argtypes = []
     lib.tupan_timing_read_each.restype = ctypes.c_int
     lib.tupan_timing_read_each.argtypes = [ctypes.POINTER(ctypes.c_double),
                                            ctypes.c_int,
@@ -209,6 +211,13 @@ def timing_read_each():
     return list(out)[:cnt.value]
 
 
+def release_workspaces():
+    """Free the library's scratch workspaces on the current device (after a
+    device synchronisation); HIP graphs captured earlier that contain
+    library launches must be captured again."""
+    check(lib().tupan_release_workspaces(), "tupan_release_workspaces")
+
+
 def device_info():
     cu = ctypes.c_int()
     wg = ctypes.c_int()
@@ -236,7 +245,7 @@ DEVICE_ABI = ("tupan_last_error", "tupan_version", "tupan_device_info",
               "tupan_timing", "tupan_timing_read", "tupan_timing_read_each",
               "tupan_phi", "tupan_acc", "tupan_acc_jerk", "tupan_snap_crackle",
               "tupan_tstep", "tupan_pairwise", "tupan_pairwise_f32",
-              "tupan_pairwise_noalloc",
+              "tupan_pairwise_noalloc", "tupan_release_workspaces",
               "tupan_sakura", "tupan_kepler_pairs", "tupan_host_pairwise_f32",
               "tupan_host_sakura_f32", "tupan_host_kepler_f32",
               "tupan_ensemble_create", "tupan_ensemble_destroy",

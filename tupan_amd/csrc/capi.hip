@@ -20,7 +20,8 @@ std::mutex g_dev_mutex;
 std::vector<Device> g_devices;
 std::vector<void*> g_stage;       // host-layer staging buffers, per device
 std::vector<size_t> g_stage_bytes;
-std::vector<void*> g_retired;     // outgrown workspaces, kept until exit
+// outgrown workspaces (device, pointer), kept until released
+std::vector<std::pair<int, void*>> g_retired;
 
 Device& device_slot()
 {
@@ -51,10 +52,41 @@ Device& device_slot()
 
 const Device& current_device() { return device_slot(); }
 
-void* workspace(size_t bytes, hipStream_t, int slot)
+namespace {
+
+// Orders this use of a workspace slot after its last use when that was on
+// another stream (event recorded there, waited on here).  Skipped while
+// either stream is capturing: a capture must not overlap other streams' use
+// of the library.
+void order_slot(Device& d, int slot, hipStream_t stream)
+{
+    if (d.ws_used[slot] && d.ws_stream[slot] != stream) {
+        hipStreamCaptureStatus here = hipStreamCaptureStatusNone;
+        hipStreamCaptureStatus there = hipStreamCaptureStatusNone;
+        bool ok = hipStreamIsCapturing(stream, &here) == hipSuccess &&
+                  hipStreamIsCapturing(d.ws_stream[slot], &there) == hipSuccess;
+        if (ok && here == hipStreamCaptureStatusNone && there == hipStreamCaptureStatusNone) {
+            if (!d.ws_event[slot] &&
+                hipEventCreateWithFlags(&d.ws_event[slot], hipEventDisableTiming) != hipSuccess) {
+                d.ws_event[slot] = nullptr;
+                ok = false;
+            }
+            ok = ok && hipEventRecord(d.ws_event[slot], d.ws_stream[slot]) == hipSuccess &&
+                 hipStreamWaitEvent(stream, d.ws_event[slot], 0) == hipSuccess;
+        }
+        if (!ok) (void)hipGetLastError();
+    }
+    d.ws_stream[slot] = stream;
+    d.ws_used[slot] = true;
+}
+
+}  // namespace
+
+void* workspace(size_t bytes, hipStream_t stream, int slot)
 {
     if (slot < 0 || slot > 3) return nullptr;
     Device& d = device_slot();
+    order_slot(d, slot, stream);
     if (d.ws_bytes[slot] >= bytes) return d.ws[slot];
     // at least twice the old size, so a slot's retired buffers never add up
     // to more than its current one
@@ -71,7 +103,7 @@ void* workspace(size_t bytes, hipStream_t, int slot)
         // (and launches still in flight may use it), so it stays valid until
         // the process ends.
         std::lock_guard<std::mutex> lock(g_dev_mutex);
-        g_retired.push_back(d.ws[slot]);
+        g_retired.emplace_back(d.id, d.ws[slot]);
     }
     d.ws[slot] = p;
     d.ws_bytes[slot] = want;
@@ -252,6 +284,35 @@ extern "C" {
 const char* tupan_last_error(void) { return g_error.c_str(); }
 
 int tupan_version(void) { return 100; }
+
+int tupan_release_workspaces(void)
+{
+    Device& d = device_slot();
+    int rc = check_hip(hipDeviceSynchronize(), "synchronise before release");
+    if (rc != TUPAN_OK) return rc;
+    for (int k = 0; k < 4; ++k) {
+        if (d.ws[k]) (void)hipFree(d.ws[k]);
+        d.ws[k] = nullptr;
+        d.ws_bytes[k] = 0;
+        d.ws_used[k] = false;
+        d.ws_stream[k] = nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_dev_mutex);
+    std::vector<std::pair<int, void*>> keep;
+    for (const auto& r : g_retired) {
+        if (r.first == d.id)
+            (void)hipFree(r.second);
+        else
+            keep.push_back(r);
+    }
+    g_retired.swap(keep);
+    if (g_stage[d.id]) {
+        (void)hipFree(g_stage[d.id]);
+        g_stage[d.id] = nullptr;
+        g_stage_bytes[d.id] = 0;
+    }
+    return check_hip(hipGetLastError(), "release");
+}
 
 int tupan_timing(int enable)
 {

@@ -60,17 +60,25 @@ struct Device {
     // scratch workspaces (j-split partials, near-pair lists), by slot
     void* ws[4] = {nullptr, nullptr, nullptr, nullptr};
     size_t ws_bytes[4] = {0, 0, 0, 0};
+    // the stream of each slot's last use, and an event that orders a use on
+    // another stream after it
+    hipStream_t ws_stream[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool ws_used[4] = {false, false, false, false};
+    hipEvent_t ws_event[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
 // State of the current device (initialised on first use).
 const Device& current_device();
 
-// Scratch workspace `slot` (0..3) of at least `bytes` on the current device.
-// Slots are independent buffers, so one call can hold several at once.
-// Growing one allocates a new buffer of at least twice the size and keeps the
-// old one allocated until the process ends (captured HIP graphs may still
-// address it); no synchronisation.  Returns null on allocation failure (the
-// old buffer then stays current).
+// Scratch workspace `slot` (0..3) of at least `bytes` on the current device,
+// for work enqueued on `stream`.  Slots are independent buffers, so one call
+// can hold several at once.  When the slot's last use was on another stream
+// and neither stream is capturing a graph, `stream` is made to wait (no host
+// synchronisation) for the work enqueued on that stream so far.  Growing a
+// slot allocates a new buffer of at least twice the size and keeps the old
+// one allocated until tupan_release_workspaces or the end of the process
+// (captured HIP graphs may still address it).  Returns null on allocation
+// failure (the old buffer then stays current).
 void* workspace(size_t bytes, hipStream_t stream, int slot = 0);
 
 // Error state (thread-local message).  set_error returns `code`.
