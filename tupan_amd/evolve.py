@@ -2,7 +2,8 @@
 
     python -m tupan_amd.evolve --n 16384 --method ahermite4 --eta 0.01 \\
         --t-end 1 --mode split32 [--eps 1e-4] [--imf equalmass] [--seed 1] \\
-        [--graph-steps 64] [--chunk 0.0625] [--max-wall 300]
+        [--graph-steps 64] [--chunk 0.0625] [--max-wall 300] \\
+        [--save state.npz] [--resume state.npz]
 
 Hermite methods run as device steps replayed in HIP graphs of
 --graph-steps steps (--graph-steps 0: the host-driven driver); other
@@ -12,6 +13,13 @@ time units and stops at --t-end or after the first chunk that ends past
 potential energies to stderr.  At the end it prints one JSON line: the time reached, steps, wall
 time, |dE/E| with E from FP64 Phi and exactly rounded sums, and the force
 mode.
+
+--save writes every particle array, the time, the initial energy and the
+wall time so far to an .npz file after every chunk; --resume continues a run
+from such a file (the other options must be the run's own).  A step depends
+only on the particle arrays and the time, and power-of-two block steps
+aligned with the time never cross a chunk boundary, so a resumed run takes
+the same steps as an uninterrupted one.
 """
 import argparse
 import json
@@ -35,29 +43,59 @@ def main(argv=None):
     ap.add_argument("--max-wall", type=float, default=0.0,
                     help="stop after the chunk that passes this many "
                          "seconds (0: no limit)")
+    ap.add_argument("--save", default=None,
+                    help="write the run's state here after every chunk")
+    ap.add_argument("--resume", default=None,
+                    help="continue the run saved in this file")
     args = ap.parse_args(argv)
 
+    import os
+    import numpy as np
     import torch
     from .diagnostics import energy
     from .ics.plummer import make_plummer
     from .integrator import Integrator
     from .lib import extensions
     from .lib.utils import ctype
+    from .particles import ParticleSystem
 
-    ps = make_plummer(args.n, args.eps, (args.imf,), seed=args.seed,
-                      device="cuda:0")
-    type(ps).t_curr = 0.0
-    e0 = energy(ps)
-    print("t = 0  K = %.15e  U = %.15e" % (ps.kinetic_energy, ps.potential_energy),
-          file=sys.stderr, flush=True)
+    wall_before = 0.0
+    if args.resume:
+        with np.load(args.resume, allow_pickle=False) as z:
+            meta = json.loads(str(z["__meta__"]))
+            arrays = {k: z[k] for k in z.files if k != "__meta__"}
+        ps = ParticleSystem.from_numpy(arrays, device="cuda:0")
+        t = float(meta["t"])
+        e0 = float(meta["e0"])
+        wall_before = float(meta["wall_s"])
+        print("resumed at t = %.6f (%.1f s before)" % (t, wall_before),
+              file=sys.stderr, flush=True)
+    else:
+        ps = make_plummer(args.n, args.eps, (args.imf,), seed=args.seed,
+                          device="cuda:0")
+        type(ps).t_curr = 0.0
+        t = 0.0
+        e0 = energy(ps)
+        print("t = 0  K = %.15e  U = %.15e"
+              % (ps.kinetic_energy, ps.potential_energy),
+              file=sys.stderr, flush=True)
     ctype.force_mode = args.mode
     extensions.install("HIP", "float64")
     hermite = "hermite" in args.method
     kw = {"device_steps": True} if hermite and args.graph_steps > 0 else {}
-    it = Integrator(args.eta, 0.0, ps, method=args.method, **kw)
+    it = Integrator(args.eta, t, ps, method=args.method, **kw)
+
+    def save(t_now, wall_now):
+        arrays = it.particle_system.to_numpy()
+        meta = {"t": t_now, "e0": e0, "wall_s": wall_now, "n": args.n,
+                "eps": args.eps, "imf": args.imf, "seed": args.seed,
+                "method": args.method, "eta": args.eta, "mode": args.mode}
+        tmp = args.save + ".tmp.npz"
+        np.savez(tmp, __meta__=np.array(json.dumps(meta)), **arrays)
+        os.replace(tmp, args.save)
+
     torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    t = 0.0
+    t0 = time.perf_counter() - wall_before
     while t < args.t_end:
         target = min(args.t_end, t + args.chunk)
         if kw:
@@ -72,7 +110,10 @@ def main(argv=None):
         print("t = %.6f  %.1f s  |dE/E| = %.3e  K = %.15e  U = %.15e"
               % (t, time.perf_counter() - t0, abs((ke + pe - e0) / e0), ke, pe),
               file=sys.stderr, flush=True)
-        if args.max_wall and time.perf_counter() - t0 > args.max_wall:
+        if args.save:
+            save(t, time.perf_counter() - t0)
+        if args.max_wall and time.perf_counter() - t0 - wall_before > \
+                args.max_wall:
             break
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
