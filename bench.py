@@ -91,6 +91,10 @@ def parse():
     ap.add_argument("--sia-tau", type=float, default=0.02 / 64,
                     help="base step of the timed sia21h.dkd block steps")
     ap.add_argument("--sia-steps", type=int, default=2)
+    ap.add_argument("--sia-eta-step", default="",
+                    help="force mode of one extra whole eta step of the "
+                         "block-step workload, host and device levels "
+                         "(default: none)")
     ap.add_argument("--hermite-n", type=int, default=1024,
                     help="Plummer N of the ahermite4 leg (0: skip)")
     ap.add_argument("--hermite-t-end", type=float, default=1.0,
@@ -623,6 +627,64 @@ def block_steps_device_levels(ps0, mode, tau, steps, host_final):
             "equals_host_levels": same}
 
 
+def block_eta_step(n, mode):
+    """One whole base step (eta = 0.02, every level of the hierarchy) of the
+    block-step workload from t = 0, with host-driven levels and with
+    device-scheduled levels (graphs captured inside the timed step), forces
+    and Tstep in `mode`: interactions, rate, level visits, slots and graph
+    replays, and whether the two final states are equal bit for bit.  About
+    two minutes per driver in split32; not part of the default run."""
+    import torch
+    from tupan_amd.ics.imf import IMF
+    from tupan_amd.ics.plummer import make_plummer
+    from tupan_amd.integrator import Integrator
+    from tupan_amd.lib import extensions
+    from tupan_amd.lib.utils import ctype
+    ps0 = make_plummer(n, 1e-4, IMF.power_law(-2.35, 0.1, 10.0), seed=1,
+                       device="cuda:0")
+    out = {"n": n, "method": "sia21h.dkd", "eta": 0.02, "tau": 0.02,
+           "mode": mode}
+    finals = {}
+    old = ctype.force_mode
+    for how in ("host_levels", "device_levels"):
+        log("bench: one eta step, %s, %s" % (mode, how))
+        ps = ps0.copy()
+        type(ps).t_curr = 0.0
+        ctype.force_mode = mode
+        extensions.install("HIP", "float64")
+        try:
+            kw = {"device_levels": True} if how == "device_levels" else {}
+            it = Integrator(0.02, 0.0, ps, method="sia21h.dkd", **kw)
+            torch.cuda.synchronize()
+            cnt = extensions.count_pairs(True)
+            t0 = time.perf_counter()
+            if how == "device_levels":
+                it.integrator.is_initialized = True
+                lv = it.integrator._device_levels()
+                replays = lv.step(0.02)
+            else:
+                it.evolve_step(0.02)
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+            extensions.count_pairs(False)
+            finals[how] = it.particle_system.to_numpy()
+        finally:
+            ctype.force_mode = old
+            extensions.install("HIP", "float64")
+        pairs = int(sum(cnt.values()))
+        res = {"gint_s": pairs / wall / 1e9, "pairs": pairs, "wall_s": wall,
+               "t": type(ps).t_curr}
+        if how == "device_levels":
+            res.update({"level_visits_above_64": lv.last_visits,
+                        "slots": lv.last_ops, "graph_replays": replays,
+                        "slots_per_replay": lv.slots})
+        out[how] = res
+    out["device_equals_host"] = all(
+        np.array_equal(finals["host_levels"][k], finals["device_levels"][k])
+        for k in finals["host_levels"])
+    return out
+
+
 def _best(fn, reps):
     best = None
     for _ in range(reps):
@@ -951,6 +1013,10 @@ def main():
         out["block_steps"] = block_step_rate(
             args.sia_n, args.sia_tau, args.sia_steps,
             [m for m in args.sia_modes.split(",") if m])
+
+    if args.sia_eta_step:
+        out["block_eta_step"] = block_eta_step(args.sia_n or 262144,
+                                               args.sia_eta_step)
 
     if args.hermite_n > 0:
         log("bench: ahermite4 N=%d to t=%g, host, device and graph steps"
