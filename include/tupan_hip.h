@@ -322,9 +322,9 @@ int tupan_pairwise_f32(int kernel, int mode, size_t ni,
  * graphs captured earlier still address it, so graphs that contain library
  * launches must be captured again after it.  The next call allocates
  * afresh.  Calls on different streams that use the same workspace are
- * ordered on the device (the later stream waits for the earlier one's work)
- * unless a stream is capturing a graph: a capture must not overlap other
- * streams' calls. */
+ * ordered on the device (the later stream waits for the earlier one's work,
+ * so the earlier stream must not have been destroyed) unless a stream is
+ * capturing a graph: a capture must not overlap other streams' calls. */
 int tupan_release_workspaces(void);
 
 /* tupan_pairwise without allocation (no counterpart in the reference; for
