@@ -182,7 +182,8 @@ __global__ __launch_bounds__(BLOCK) void sakura_kernel_dev(SakuraArgs a)
     if (a.status && valid && acc.c.fail) atomicOr(a.status, 1);
 }
 
-// Counter flush shared by the kernels: wave sums, one atomic per wave.
+// Counter flush shared by the kernels: wave sums, one atomic per wave and
+// nonzero counter.
 __device__ __forceinline__ void flush_counters(unsigned long long* counters, bool valid,
                                                const kep::Counters& c, int tid)
 {
@@ -192,10 +193,35 @@ __device__ __forceinline__ void flush_counters(unsigned long long* counters, boo
     const unsigned int rt = wave_sum(valid ? c.retries : 0u);
     const unsigned int lf = wave_sum(valid ? c.leapfrog : 0u);
     if ((tid & (kWave - 1)) == 0) {
-        atomicAdd(counters + 0, static_cast<unsigned long long>(s));
-        atomicAdd(counters + 1, static_cast<unsigned long long>(it));
-        atomicAdd(counters + 2, static_cast<unsigned long long>(rt));
-        atomicAdd(counters + 3, static_cast<unsigned long long>(lf));
+        if (s) atomicAdd(counters + 0, static_cast<unsigned long long>(s));
+        if (it) atomicAdd(counters + 1, static_cast<unsigned long long>(it));
+        if (rt) atomicAdd(counters + 2, static_cast<unsigned long long>(rt));
+        if (lf) atomicAdd(counters + 3, static_cast<unsigned long long>(lf));
+    }
+}
+
+// Counter bank of the near pass.  Its thousands of short waves all finish
+// within a few microseconds, and their atomics on the caller's five
+// counters serialise in the cache (≈200 µs for 4 570 waves); each workgroup
+// adds into one of kBankSlots 128-byte slots instead, and
+// near_bank_flush_kernel sums the slots into the counters.  Slot entries:
+// solves, Laguerre steps, retries, leapfrog pairs, guess fallbacks.
+constexpr int kBankSlots = 64;
+constexpr int kBankStride = 16;  // unsigned long longs per slot
+
+__global__ void near_bank_flush_kernel(const unsigned long long* bank,
+                                       unsigned long long* counters,
+                                       unsigned long long* fallbacks)
+{
+    const int k = static_cast<int>(threadIdx.x);
+    if (k >= 5) return;
+    unsigned long long v = 0;
+    for (int b = 0; b < kBankSlots; ++b) v += bank[b * kBankStride + k];
+    if (v == 0) return;
+    if (k < 4) {
+        if (counters) atomicAdd(counters + k, v);
+    } else if (fallbacks) {
+        atomicAdd(fallbacks, v);
     }
 }
 
@@ -343,9 +369,16 @@ __global__ __launch_bounds__(kScanThreads) void near_block_sums_kernel(int n, co
 
 __global__ __launch_bounds__(kScanThreads) void near_block_offsets_kernel(int nb, int* bsum,
                                                                           const int* bover,
-                                                                          int* total)
+                                                                          int* total,
+                                                                          unsigned long long* bank)
 {
     __shared__ int lds[kScanThreads];
+    if (bank) {
+        // the near pass's counter bank starts from zero
+        for (int k = static_cast<int>(threadIdx.x); k < kBankSlots * kBankStride;
+             k += kScanThreads)
+            bank[k] = 0;
+    }
     int carry = 0, over = 0;
     for (int b0 = 0; b0 < nb; b0 += kScanThreads) {
         const int b = b0 + static_cast<int>(threadIdx.x);
@@ -466,16 +499,17 @@ __device__ __forceinline__ void near_pair(const SakuraArgs& a, int t, int nitems
     }
 }
 
-// The near pass's counters, fallbacks and failure status.
+// The near pass's counters (into the workgroup's slot of the counter bank,
+// when counting) and failure status.
 __device__ __forceinline__ void near_flush(const SakuraArgs& a, bool valid,
-                                           const kep::Counters& c,
-                                           unsigned long long* fallbacks)
+                                           const kep::Counters& c, unsigned long long* bank)
 {
-    flush_counters(a.counters, valid, c, static_cast<int>(threadIdx.x));
-    if (fallbacks) {
+    if (bank) {
+        unsigned long long* slot = bank + (blockIdx.x % kBankSlots) * kBankStride;
+        flush_counters(slot, valid, c, static_cast<int>(threadIdx.x));
         const unsigned int fb = wave_sum(valid ? c.fallbacks : 0u);
         if ((threadIdx.x & (kWave - 1)) == 0 && fb)
-            atomicAdd(fallbacks, static_cast<unsigned long long>(fb));
+            atomicAdd(slot + 4, static_cast<unsigned long long>(fb));
     }
     if (a.status && valid && c.fail) atomicOr(a.status, 1);
 }
@@ -486,7 +520,7 @@ __global__ __launch_bounds__(128) void sakura_near_kernel(SakuraArgs a, int nite
                                                           const int* nitems_dev,
                                                           const int2* items, double* contrib,
                                                           const float* delta, SolveLogDev log,
-                                                          unsigned long long* fallbacks)
+                                                          unsigned long long* bank)
 {
     // nitems: the list's capacity (contrib's stride); nitems_dev, when
     // given, the number of items listed on the device
@@ -497,7 +531,7 @@ __global__ __launch_bounds__(128) void sakura_near_kernel(SakuraArgs a, int nite
     if (valid)
         near_pair(a, t, nitems, items, contrib, delta != nullptr, delta ? delta[t] : 0.f, log,
                   c);
-    near_flush(a, valid, c, fallbacks);
+    near_flush(a, valid, c, bank);
 }
 
 // Pass 2 with the learned guess evaluated in the same kernel: each wave
@@ -510,7 +544,7 @@ __global__ __launch_bounds__(128, 2) void sakura_near_learned_kernel(SakuraArgs 
                                                                   double* contrib,
                                                                   const NetPacked* net,
                                                                   SolveLogDev log,
-                                                                  unsigned long long* fallbacks)
+                                                                  unsigned long long* bank)
 {
     __shared__ __align__(16) NetPacked lnet;
     mlp_stage(net, &lnet);
@@ -549,7 +583,7 @@ __global__ __launch_bounds__(128, 2) void sakura_near_learned_kernel(SakuraArgs 
     __asm__ volatile("" ::: "memory");
     kep::Counters c;
     if (valid) near_pair(a, t, nitems, items, contrib, true, d, log, c);
-    near_flush(a, valid, c, fallbacks);
+    near_flush(a, valid, c, bank);
 }
 
 // Pass 3: add each (split, i)'s near contributions, in list order, to its
@@ -726,7 +760,8 @@ int sakura_call_ex(size_t ni, const double* const* ip, size_t nj, const double* 
     const unsigned fgrid = grid;
 
     // workspace slot 1: partials [nsplit][6][ni], counts, offsets, the
-    // overflow list, block sums, totals and the near j slots
+    // overflow list, block sums, totals, the near pass's counter bank and
+    // the near j slots
     auto up = [](size_t b) { return (b + 255) & ~static_cast<size_t>(255); };
     const int n = static_cast<int>(nent);
     const int nb = (n + kScanPer - 1) / kScanPer;
@@ -734,9 +769,10 @@ int sakura_call_ex(size_t ni, const double* const* ip, size_t nj, const double* 
     const size_t b_cnt = up(nent * sizeof(int));
     const size_t b_blk = up(static_cast<size_t>(nb) * sizeof(int));
     const size_t b_tot = 256;
+    const size_t b_bank = up(kBankSlots * kBankStride * sizeof(unsigned long long));
     const size_t b_slots = up(nent * kNearSlots * sizeof(int));
     char* w1 = static_cast<char*>(
-        workspace(b_part + 3 * b_cnt + 2 * b_blk + b_tot + b_slots, s, 1));
+        workspace(b_part + 3 * b_cnt + 2 * b_blk + b_tot + b_bank + b_slots, s, 1));
     if (!w1) return set_error(TUPAN_ERR_ALLOC, "workspace allocation failed");
     char* q = w1;
     a.partial = reinterpret_cast<double*>(q);
@@ -753,6 +789,10 @@ int sakura_call_ex(size_t ni, const double* const* ip, size_t nj, const double* 
     q += b_blk;
     int* total_dev = reinterpret_cast<int*>(q);
     q += b_tot;
+    unsigned long long* bank = (a.counters || opt.fallbacks)
+                                   ? reinterpret_cast<unsigned long long*>(q)
+                                   : nullptr;
+    q += b_bank;
     a.near_j = reinterpret_cast<int*>(q);
 
     timing_begin(s);
@@ -785,7 +825,7 @@ int sakura_call_ex(size_t ni, const double* const* ip, size_t nj, const double* 
     hipLaunchKernelGGL(near_block_sums_kernel, dim3(nb), dim3(kScanThreads), 0, s, n,
                        a.near_count, bsum, bover);
     hipLaunchKernelGGL(near_block_offsets_kernel, dim3(1), dim3(kScanThreads), 0, s, nb, bsum,
-                       bover, total_dev);
+                       bover, total_dev, bank);
     hipLaunchKernelGGL(near_offsets_kernel, dim3(nb), dim3(kScanThreads), 0, s, n,
                        a.near_count, bsum, near_off, total_dev, over_list);
     const unsigned eg = static_cast<unsigned>((n + 255) / 256);
@@ -805,7 +845,10 @@ int sakura_call_ex(size_t ni, const double* const* ip, size_t nj, const double* 
         hipLaunchKernelGGL(near_compact_kernel, dim3(eg), dim3(256), 0, s, n, a.ni, a.near_count,
                            near_off, a.near_j, items, cap);
         hipLaunchKernelGGL(sakura_near_kernel, dim3((cap + 127) / 128), dim3(128), 0, s, a, cap,
-                           total_dev, items, contrib, nullptr, opt.log, opt.fallbacks);
+                           total_dev, items, contrib, nullptr, opt.log, bank);
+        if (bank)
+            hipLaunchKernelGGL(near_bank_flush_kernel, dim3(1), dim3(64), 0, s, bank, a.counters,
+                               opt.fallbacks);
         hipLaunchKernelGGL(near_add_kernel, dim3(eg), dim3(256), 0, s, n, a.ni, cap,
                            a.near_count, near_off, contrib, a.partial);
         hipLaunchKernelGGL(near_overflow_kernel, dim3(kOverflowWaves), dim3(kWave), 0, s, a,
@@ -838,11 +881,14 @@ int sakura_call_ex(size_t ni, const double* const* ip, size_t nj, const double* 
         if (opt.net) {
             hipLaunchKernelGGL(sakura_near_learned_kernel, dim3(ig), dim3(128), 0, s, a, total,
                                items, contrib, static_cast<const NetPacked*>(opt.net), opt.log,
-                               opt.fallbacks);
+                               bank);
         } else {
             hipLaunchKernelGGL(sakura_near_kernel, dim3(ig), dim3(128), 0, s, a, total, nullptr,
-                               items, contrib, nullptr, opt.log, opt.fallbacks);
+                               items, contrib, nullptr, opt.log, bank);
         }
+        if (bank)
+            hipLaunchKernelGGL(near_bank_flush_kernel, dim3(1), dim3(64), 0, s, bank, a.counters,
+                               opt.fallbacks);
         hipLaunchKernelGGL(near_add_kernel, dim3(eg), dim3(256), 0, s, n, a.ni, total,
                            a.near_count, near_off, contrib, a.partial);
     }
