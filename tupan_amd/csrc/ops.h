@@ -476,7 +476,9 @@ struct OpTstep {
         T v2 = dot3<P>(ux, uy, uz, ux, uy, uz);
         const bool mask = r2 > T(0);
         T q1, q2;
-        P::q12(r2, e2, mask, q1, q2);
+        // a masked pair's term is zeroed at the end, so the fast policies
+        // skip masking 1/R here (the reference-order policy keeps it)
+        P::q12(r2, e2, P::kFused ? true : mask, q1, q2);
         // a = 2, b = (1 + a/2) * q2 = 2 q2
         T b = T(2) * q2;
         T phi = mt * q1;
@@ -798,8 +800,8 @@ struct OpSplitTstep {
         float rv = __builtin_fmaf(dz, uz, __builtin_fmaf(dy, uy, dx * ux));
         float v2 = __builtin_fmaf(uz, uz, __builtin_fmaf(uy, uy, ux * ux));
         const bool mask = R2 > e2;
+        // no mask on 1/R: a masked pair's term is zeroed at the end
         float q1 = __builtin_amdgcn_rsqf(R2);
-        q1 = mask ? q1 : 0.0f;
         float q2 = q1 * q1;
         float phi = mt * q1;
         float w2 = q2 * __builtin_fmaf(2.0f, phi, v2);
