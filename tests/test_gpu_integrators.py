@@ -351,6 +351,24 @@ def test_device_levels_match_host_driver(method, mode):
         np.testing.assert_array_equal(v[k], g[k], err_msg=k)
 
 
+def test_device_levels_small_set_takes_host_step():
+    """A set of at most 64 particles with device_levels=True takes the host
+    driver's step (one subtree launch): the same state as the host path,
+    through evolve_step and through SIA.evolve."""
+    from tupan_amd.ics.imf import IMF
+    from tupan_amd.ics.plummer import make_plummer
+    install_hip("float64")
+    host = make_plummer(48, 1e-4, IMF.power_law(-2.35, 0.1, 10.0), seed=3,
+                        device="cuda:0").to_numpy()
+    g, tg, cg = _levels_run(host, "sia21h.dkd", "fp64", 2, 0.02 / 16, "host")
+    d, td, cd = _levels_run(host, "sia21h.dkd", "fp64", 2, 0.02 / 16, "graph")
+    v, tv, cv = _levels_run(host, "sia21h.dkd", "fp64", 2, 0.02 / 16, "evolve")
+    assert tg == td == tv and cg == cd == cv
+    for k in g:
+        np.testing.assert_array_equal(d[k], g[k], err_msg=k)
+        np.testing.assert_array_equal(v[k], g[k], err_msg=k)
+
+
 def test_device_levels_block_step_workload():
     """The block-step workload (Plummer N = 262144, Salpeter masses,
     eps = 1e-4, seed 1, sia21h.dkd, eta = 0.02): one step of base step

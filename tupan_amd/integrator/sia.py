@@ -292,8 +292,11 @@ class SIA(Base):
     def evolve_step(self, t_end):
         """integrator/__init__.py:80-99; with device_steps (the 's' and 'a'
         variants) one device step and one read of the new time; with
-        device_levels (the 'h' variants) one device-scheduled step."""
-        if self.device_levels and not self.device_steps:
+        device_levels (the 'h' variants) one device-scheduled step; a set of
+        at most 64 particles, which the host driver advances in one subtree
+        launch, takes the host driver's step."""
+        if self.device_levels and not self.device_steps and \
+                self.ps.n > SUBTREE_MAX:
             if not self.is_initialized:
                 self.initialize(t_end)
             self._device_levels().step(t_end,
@@ -314,12 +317,20 @@ class SIA(Base):
 
     def evolve(self, t_end, steps_per_graph=32):
         """Advance to t_end by device steps ('s' and 'a' variants),
-        replaying a HIP graph of steps_per_graph steps; steps past t_end
-        inside the last replay change nothing.  Returns the number of
-        replays."""
+        replaying a HIP graph of steps_per_graph steps (steps past t_end
+        inside the last replay change nothing), or by device-scheduled
+        level steps ('h' variants with device_levels), each replaying the
+        level graphs until the device reports the step done.  Returns the
+        number of replays."""
         if self.device_levels and not self.device_steps:
             if steps_per_graph < 1:
                 raise ValueError("steps_per_graph must be at least 1")
+            if self.ps.n <= SUBTREE_MAX:
+                # one subtree launch per step already
+                while (self.ps.t_curr < t_end) if self.eta > 0 else \
+                        (self.ps.t_curr > t_end):
+                    self.evolve_step(t_end)
+                return 0
             if not self.is_initialized:
                 self.initialize(t_end)
             replays = self._device_levels().evolve(t_end)
