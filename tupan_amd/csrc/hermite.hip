@@ -23,6 +23,16 @@
 // Force evaluations write only scratch arrays and every elementwise kernel
 // does nothing when tau = 0, so a step after t_end changes no live array.
 //
+// Order 4 on sets the small-set kernel takes (257 to 2048 particles, fp64
+// or split32) fuses steps 3-5 into the three force launches: each launch's
+// per-particle store runs that particle's predictor, corrector or corrector
+// and bookkeeping.  Those epilogues cannot write the positions the same
+// launch reads as j-particles, so the predicted state goes to the r0, v0
+// scratch, the once-corrected state to the s0, c0 scratch (unused at order
+// 4), and the live r, v keep the start-of-step state until the last
+// corrector replaces them: five launches per step instead of nine, with the
+// same operations in the same order, so the same results bit for bit.
+//
 // The predictor and corrector evaluate the reference's polynomials operation
 // by operation as the host driver does on its arrays (one rounding per
 // operation, IEEE division, no contraction: this file is compiled with
@@ -32,6 +42,7 @@
 // in the last bit for a minimum time step within an ulp or two of a power of
 // two.
 #include "block_step.h"
+#include "ops.h"
 
 namespace tupan {
 
@@ -46,20 +57,22 @@ struct HermiteArrays {
     double *a1[3], *j1[3], *s1[3], *c1[3];
 };
 
-// hermite.py:20-410, predictor of H2/H4/H6/H8.
+// hermite.py:20-410, predictor of H2/H4/H6/H8 for particle i: reads the
+// start-of-step r, v (live arrays) and a0 (j0, s0, c0), saves r, v to
+// rs, vs when given, writes the predicted r, v to ro, vo.
 template <int Order>
-__global__ __launch_bounds__(256) void hermite_predict_kernel(int n, HermiteArrays h,
-                                                              const double* ctl)
+__device__ __forceinline__ void predict_one(const HermiteArrays& h, int i, double tau,
+                                            double* const* rs, double* const* vs,
+                                            double* const* ro, double* const* vo)
 {
-    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
-    const double tau = ctl[kCtlTau];
-    if (i >= n || tau == 0) return;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         const double r0 = h.r[d][i], v0 = h.v[d][i];
         const double a0 = h.a0[d][i];
-        h.r0[d][i] = r0;
-        h.v0[d][i] = v0;
+        if (rs) {
+            rs[d][i] = r0;
+            vs[d][i] = v0;
+        }
         double dr, dv;
         if constexpr (Order == 2) {
             dr = (a0 * tau / 2 + v0) * tau;
@@ -77,22 +90,33 @@ __global__ __launch_bounds__(256) void hermite_predict_kernel(int n, HermiteArra
             dr = ((((c0 * tau / 5 + s0) * tau / 4 + j0) * tau / 3 + a0) * tau / 2 + v0) * tau;
             dv = (((c0 * tau / 4 + s0) * tau / 3 + j0) * tau / 2 + a0) * tau;
         }
-        h.r[d][i] = r0 + dr;
-        h.v[d][i] = v0 + dv;
+        ro[d][i] = r0 + dr;
+        vo[d][i] = v0 + dv;
     }
 }
 
-// hermite.py:20-410, corrector of H2/H4/H6/H8 (v first, then r with the new v).
 template <int Order>
-__global__ __launch_bounds__(256) void hermite_correct_kernel(int n, HermiteArrays h,
+__global__ __launch_bounds__(256) void hermite_predict_kernel(int n, HermiteArrays h,
                                                               const double* ctl)
 {
     const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     const double tau = ctl[kCtlTau];
     if (i >= n || tau == 0) return;
+    predict_one<Order>(h, i, tau, h.r0, h.v0, h.r, h.v);
+}
+
+// hermite.py:20-410, corrector of H2/H4/H6/H8 for particle i (v first, then
+// r with the new v): the start-of-step r, v from r0s, v0s, the forces a0..
+// and a1.., which it also copies into the live a (j, s, c); the corrected
+// r, v go to ro, vo.
+template <int Order>
+__device__ __forceinline__ void correct_one(const HermiteArrays& h, int i, double tau,
+                                            double* const* r0s, double* const* v0s,
+                                            double* const* ro, double* const* vo)
+{
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        const double r0 = h.r0[d][i], v0 = h.v0[d][i];
+        const double r0 = r0s[d][i], v0 = v0s[d][i];
         const double a0 = h.a0[d][i], a1 = h.a1[d][i];
         h.a[d][i] = a1;
         if constexpr (Order >= 4) h.j[d][i] = h.j1[d][i];
@@ -122,27 +146,120 @@ __global__ __launch_bounds__(256) void hermite_correct_kernel(int n, HermiteArra
             r1 = ((((s0 - s1) * tau / 20 + (j0 + j1)) * tau / 3 + 3 * (a0 - a1)) * tau / 14 +
                   (v0 + v1)) * tau / 2 + r0;
         }
-        h.v[d][i] = v1;
-        h.r[d][i] = r1;
+        vo[d][i] = v1;
+        ro[d][i] = r1;
     }
 }
 
-// hermite.py:390-410 (do_step's bookkeeping).  Thread 0 of block 0 also
-// advances t; every thread reads tau before that write is visible to it or
-// not at all, since tau is not changed here.
+template <int Order>
+__global__ __launch_bounds__(256) void hermite_correct_kernel(int n, HermiteArrays h,
+                                                              const double* ctl)
+{
+    const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+    const double tau = ctl[kCtlTau];
+    if (i >= n || tau == 0) return;
+    correct_one<Order>(h, i, tau, h.r0, h.v0, h.r, h.v);
+}
+
+// hermite.py:390-410 (do_step's bookkeeping) for particle i.
+__device__ __forceinline__ void end_one(const HermiteArrays& h, int i, double tau, int adaptive)
+{
+    h.tstep[i] = tau;
+    h.time[i] = h.time[i] + tau;
+    h.nstep[i] = h.nstep[i] + 1;
+    if (adaptive) h.tstepij[i] = h.tsij[i];
+}
+
+// Thread 0 of block 0 also advances t; every thread reads tau before that
+// write is visible to it or not at all, since tau is not changed here.
 __global__ __launch_bounds__(256) void hermite_end_kernel(int n, HermiteArrays h, int adaptive,
                                                           double* ctl)
 {
     const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
     const double tau = ctl[kCtlTau];
     if (tau == 0) return;
-    if (i < n) {
-        h.tstep[i] = tau;
-        h.time[i] = h.time[i] + tau;
-        h.nstep[i] = h.nstep[i] + 1;
-        if (adaptive) h.tstepij[i] = h.tsij[i];
-    }
+    if (i < n) end_one(h, i, tau, adaptive);
     if (i == 0) ctl[kCtlT] = ctl[kCtlT] + tau;
+}
+
+// AccJerk of the small-set kernel with a Hermite-4 epilogue in its
+// per-particle store.  Stage 0: forces of the start state into a0, j0, then
+// the predictor into the r0, v0 scratch; stage 1: forces of the predicted
+// state into a1, j1, then the corrector into the s0, c0 scratch; stage 2:
+// forces of that state into a1, j1, then the corrector into the live r, v
+// and the bookkeeping.
+template <class Base, int Stage>
+struct OpHermite4 : Base {
+    struct Args : Base::Args {
+        HermiteArrays h;
+        double* ctl;
+        int adaptive;
+    };
+    __device__ static void store(const Args& a, int i, const double* t)
+    {
+        Base::store(a, i, t);
+        const double tau = a.ctl[kCtlTau];
+        if (tau == 0) return;
+        const HermiteArrays& h = a.h;
+        if constexpr (Stage == 0) {
+            predict_one<4>(h, i, tau, nullptr, nullptr, h.r0, h.v0);
+        } else if constexpr (Stage == 1) {
+            correct_one<4>(h, i, tau, h.r, h.v, h.s0, h.c0);
+        } else {
+            correct_one<4>(h, i, tau, h.r, h.v, h.r, h.v);
+            end_one(h, i, tau, a.adaptive);
+            if (i == 0) a.ctl[kCtlT] = a.ctl[kCtlT] + tau;
+        }
+    }
+};
+
+// Whether an order-4 step of n particles in `mode` takes the fused launches:
+// the small-set kernel's range (as pairwise_call decides it for AccJerk).
+bool fused4(int mode, int n)
+{
+    return mode != TUPAN_MODE_IEEE64 && n > kSmallMinJ && n <= kSmallMaxJ &&
+           static_cast<long long>(n) * n <= kSmallPairs;
+}
+
+template <class Base, int Stage>
+int force_stage4(int n, const HermiteArrays& h, double* ctl, int adaptive, hipStream_t s)
+{
+    using Op = OpHermite4<Base, Stage>;
+    typename Op::Args a{};
+    double* const* R = Stage == 0 ? h.r : (Stage == 1 ? h.r0 : h.s0);
+    double* const* V = Stage == 0 ? h.v : (Stage == 1 ? h.v0 : h.c0);
+    const double* in[8] = {h.m, R[0], R[1], R[2], h.e2, V[0], V[1], V[2]};
+    double* const* A = Stage == 0 ? h.a0 : h.a1;
+    double* const* J = Stage == 0 ? h.j0 : h.j1;
+    for (int f = 0; f < 8; ++f) {
+        a.ip[f] = in[f];
+        a.jp[f] = in[f];
+    }
+    for (int k = 0; k < 3; ++k) {
+        a.out[k] = A[k];
+        a.out[3 + k] = J[k];
+    }
+    a.ni = n;
+    a.nj = n;
+    a.nsplit = 1;
+    a.jchunk = n;
+    a.h = h;
+    a.ctl = ctl;
+    a.adaptive = adaptive;
+    constexpr int ipb = kSmallBlock / kSmallGroups;
+    const unsigned grid = static_cast<unsigned>((n + ipb - 1) / ipb);
+    hipLaunchKernelGGL((pair_small_kernel<Op, kSmallBlock, kSmallGroups, small_tile<Base>()>),
+                       dim3(grid), dim3(kSmallBlock), 0, s, a);
+    return check_launch("hermite force stage");
+}
+
+template <class Base>
+int fused_step4(int n, const HermiteArrays& h, double* ctl, int adaptive, hipStream_t s)
+{
+    int rc = force_stage4<Base, 0>(n, h, ctl, adaptive, s);
+    if (rc == TUPAN_OK) rc = force_stage4<Base, 1>(n, h, ctl, adaptive, s);
+    if (rc == TUPAN_OK) rc = force_stage4<Base, 2>(n, h, ctl, adaptive, s);
+    return rc;
 }
 
 int forces(int order, int mode, size_t n, const HermiteArrays& h, bool start, hipStream_t s)
@@ -181,7 +298,15 @@ int step_order(int adaptive, int mode, int n, const HermiteArrays& h, double eta
     hipLaunchKernelGGL(block_tau_kernel, dim3(1), dim3(kTauThreads), 0, s, n, h.ts, adaptive,
                        ctl);
     int rc = check_launch("hermite_tau");
-    if (rc == TUPAN_OK) rc = forces(Order, mode, n, h, true, s);
+    if (rc != TUPAN_OK) return rc;
+    if constexpr (Order == 4) {
+        if (fused4(mode, n)) {
+            return mode == TUPAN_MODE_SPLIT32
+                       ? fused_step4<OpSplitAccJerk>(n, h, ctl, adaptive, s)
+                       : fused_step4<OpAccJerk<Fast64>>(n, h, ctl, adaptive, s);
+        }
+    }
+    rc = forces(Order, mode, n, h, true, s);
     if (rc != TUPAN_OK) return rc;
     hipLaunchKernelGGL((hermite_predict_kernel<Order>), dim3(grid), dim3(256), 0, s, n, h, ctl);
     rc = check_launch("hermite_predict");
