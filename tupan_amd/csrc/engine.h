@@ -460,6 +460,16 @@ struct Launch {
 // Pick the j-split so that the grid has at least `target_wg` workgroups
 // (several per CU) without cutting j ranges below `min_chunk` j-particles
 // (a multiple of 16; the tile loop handles a partly filled last tile).
+// Smallest j-chunk of a split: whole tiles for a set that fits one tile
+// (such a set is always one split, so its sums are the one-tile order) and,
+// in the reference-order mode, never below 256 particles; 32 particles
+// otherwise, so that a set of a few tiles still spreads over the chip.
+__host__ __device__ inline int min_jchunk(int nj, int tile, bool exact)
+{
+    if (exact) return tile > 256 ? tile : 256;
+    return nj > tile ? 32 : tile;
+}
+
 __host__ __device__ inline Launch plan_launch(int ni, int nj, int block, int wpt, int tile,
                                               int target_wg, int max_split, int min_chunk)
 {

@@ -63,13 +63,12 @@ static int run_pairwise(typename Op::Args a, hipStream_t stream, int wg_per_cu,
     }
     const Device& dev = current_device();
     const int target = (wg_per_cu > 0 ? wg_per_cu : 4) * dev.cu_count;
-    // j-chunks of whole tiles, except for sets well above a tile in the
-    // modes that sum in any order: there 32-particle chunks let a small
-    // system (N ~ 1e3) fill the chip with j-splits.  ieee64 never cuts a
-    // j-range below 256 particles, so any set of at most 256 is one split
-    // (the reference's summation order) whatever shape is tuned.
-    const int min_chunk = a.exact ? (TILE > 256 ? TILE : 256)
-                                  : (a.nj > 4 * TILE ? 32 : TILE);
+    // j-chunks of whole tiles for a set of one tile, 32-particle chunks for
+    // larger sets in the modes that sum in any order, so that a small system
+    // (N ~ 1e3) fills the chip with j-splits.  ieee64 never cuts a j-range
+    // below 256 particles, so any set of at most 256 is one split (the
+    // reference's summation order) whatever shape is tuned.
+    const int min_chunk = min_jchunk(a.nj, TILE, a.exact != 0);
     // at most 64 j-splits; with lent scratch, at most as many as it holds
     int max_split = 64;
     if (fixed) {

@@ -631,7 +631,7 @@ __device__ void lvl_plan(LvlCtl* c, int op, int ilo, int ni, int jlo, int nj)
     const int BLOCK = kLightShapeDims[shape][0], WPT = kLightShapeDims[shape][1],
               TILE = kLightShapeDims[shape][2];
     const int target = (wg > 0 ? wg : 4) * c->cu_count;
-    const int min_chunk = nj > 4 * TILE ? 32 : TILE;
+    const int min_chunk = min_jchunk(nj, TILE, false);
     const Launch L = plan_launch(ni, nj, BLOCK, WPT, TILE, target, 64, min_chunk);
     a.nsplit = L.nsplit;
     a.jchunk = L.jchunk;
