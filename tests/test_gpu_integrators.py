@@ -215,13 +215,39 @@ def test_device_hermite_steps_match_host_driver(method, eta, steps):
         np.testing.assert_allclose(d[k], g[k], rtol=1e-13, atol=1e-15)
 
 
-def test_device_hermite_graph_replays_equal_eager_steps():
+@pytest.mark.parametrize("mode", ["fp64", "split32"])
+@pytest.mark.parametrize("method,eta,steps", [
+    ("ahermite4", 0.01, 6), ("hermite4", 1.0 / 512, 4),
+])
+def test_device_hermite4_fused_launches_match_host_driver(method, eta, steps,
+                                                          mode):
+    """At 257 to 1448 particles the order-4 device step runs its predictor,
+    correctors and bookkeeping in the per-particle stores of its three
+    AccJerk launches (predicted and once-corrected states in scratch): at
+    N = 1024 against the host driver on the same device forces, the same
+    step sizes and time and states within a few ulps."""
+    host = cpu_plummer(1024, 4.0 / 1024, 11)
+    g, tg = run(method, eta, host, steps, "hip", mode=mode)
+    d, td = run(method, eta, host, steps, "hip", mode=mode, device_steps=True)
+    assert tg == td
+    np.testing.assert_array_equal(g["tstep"], d["tstep"])
+    np.testing.assert_array_equal(g["nstep"], d["nstep"])
+    np.testing.assert_array_equal(g["time"], d["time"])
+    for k in ("rx", "ry", "rz", "vx", "vy", "vz", "ax", "ay", "az", "jx",
+              "jy", "jz"):
+        np.testing.assert_allclose(d[k], g[k], rtol=1e-13, atol=1e-15,
+                                   err_msg=k)
+
+
+@pytest.mark.parametrize("n", [256, 1024])
+def test_device_hermite_graph_replays_equal_eager_steps(n):
     """evolve() replaying a graph of 16 device steps equals the same steps
     launched one by one, bit for bit, and stops at t_end exactly (the steps
-    of the last replay past t_end change nothing)."""
+    of the last replay past t_end change nothing); N = 1024 takes the fused
+    order-4 launches."""
     from tupan_amd.integrator import Integrator
     from tupan_amd.particles import ParticleSystem
-    host = cpu_plummer(256, 4.0 / 256, 5)
+    host = cpu_plummer(n, 4.0 / n, 5)
     t_end = 0.0625
     out = []
     install_hip("float64")
