@@ -70,6 +70,12 @@ template <class Op, class = void>
 struct HasPostFallback : std::false_type {};
 template <class Op>
 struct HasPostFallback<Op, std::void_t<typename Op::PostFallback64>> : std::true_type {};
+// Ops whose per-particle store is spread over the lanes of the small-set
+// kernel's group (store_group(a, i, totals, lane)) instead of one lane.
+template <class Op, class = void>
+struct HasGroupStore : std::false_type {};
+template <class Op>
+struct HasGroupStore<Op, std::void_t<decltype(&Op::store_group)>> : std::true_type {};
 
 template <int N>
 __device__ __forceinline__ bool all_finite(const double (&t)[N])
@@ -280,7 +286,14 @@ __device__ __forceinline__ void pair_small_body(const typename Op::Args& a, int 
         for (int k = 0; k < Op::kNacc; ++k) other[k] = __shfl_xor(tot[k], off);
         if ((g & (2 * off - 1)) == 0) Op::combine(tot, other);
     }
-    if (g == 0 && i < a.ni) Op::store(a, i, tot);
+    if constexpr (HasGroupStore<Op>::value) {
+        // the group's totals (lane 0) to every lane of the group
+#pragma unroll
+        for (int k = 0; k < Op::kNacc; ++k) tot[k] = __shfl(tot[k], 0, G);
+        if (i < a.ni) Op::store_group(a, i, tot, g);
+    } else {
+        if (g == 0 && i < a.ni) Op::store(a, i, tot);
+    }
 }
 
 template <class Op, int BLOCK, int G, int TILE>

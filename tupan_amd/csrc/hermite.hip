@@ -57,42 +57,50 @@ struct HermiteArrays {
     double *a1[3], *j1[3], *s1[3], *c1[3];
 };
 
-// hermite.py:20-410, predictor of H2/H4/H6/H8 for particle i: reads the
-// start-of-step r, v (live arrays) and a0 (j0, s0, c0), saves r, v to
-// rs, vs when given, writes the predicted r, v to ro, vo.
+// hermite.py:20-410, predictor of H2/H4/H6/H8 for component d of particle
+// i: the start-of-step r, v (live arrays) and the start forces a0, j0
+// (passed in) and s0, c0, the predicted r, v to ro, vo and, when rs is
+// given, the start r, v to rs, vs.
+template <int Order>
+__device__ __forceinline__ void predict_dim(const HermiteArrays& h, int i, int d, double tau,
+                                            double a0, double j0, double* const* rs,
+                                            double* const* vs, double* const* ro,
+                                            double* const* vo)
+{
+    const double r0 = h.r[d][i], v0 = h.v[d][i];
+    if (rs) {
+        rs[d][i] = r0;
+        vs[d][i] = v0;
+    }
+    double dr, dv;
+    if constexpr (Order == 2) {
+        dr = (a0 * tau / 2 + v0) * tau;
+        dv = a0 * tau;
+    } else if constexpr (Order == 4) {
+        dr = ((j0 * tau / 3 + a0) * tau / 2 + v0) * tau;
+        dv = (j0 * tau / 2 + a0) * tau;
+    } else if constexpr (Order == 6) {
+        const double s0 = h.s0[d][i];
+        dr = (((s0 * tau / 4 + j0) * tau / 3 + a0) * tau / 2 + v0) * tau;
+        dv = ((s0 * tau / 3 + j0) * tau / 2 + a0) * tau;
+    } else {
+        const double s0 = h.s0[d][i], c0 = h.c0[d][i];
+        dr = ((((c0 * tau / 5 + s0) * tau / 4 + j0) * tau / 3 + a0) * tau / 2 + v0) * tau;
+        dv = (((c0 * tau / 4 + s0) * tau / 3 + j0) * tau / 2 + a0) * tau;
+    }
+    ro[d][i] = r0 + dr;
+    vo[d][i] = v0 + dv;
+}
+
 template <int Order>
 __device__ __forceinline__ void predict_one(const HermiteArrays& h, int i, double tau,
                                             double* const* rs, double* const* vs,
                                             double* const* ro, double* const* vo)
 {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const double r0 = h.r[d][i], v0 = h.v[d][i];
-        const double a0 = h.a0[d][i];
-        if (rs) {
-            rs[d][i] = r0;
-            vs[d][i] = v0;
-        }
-        double dr, dv;
-        if constexpr (Order == 2) {
-            dr = (a0 * tau / 2 + v0) * tau;
-            dv = a0 * tau;
-        } else if constexpr (Order == 4) {
-            const double j0 = h.j0[d][i];
-            dr = ((j0 * tau / 3 + a0) * tau / 2 + v0) * tau;
-            dv = (j0 * tau / 2 + a0) * tau;
-        } else if constexpr (Order == 6) {
-            const double j0 = h.j0[d][i], s0 = h.s0[d][i];
-            dr = (((s0 * tau / 4 + j0) * tau / 3 + a0) * tau / 2 + v0) * tau;
-            dv = ((s0 * tau / 3 + j0) * tau / 2 + a0) * tau;
-        } else {
-            const double j0 = h.j0[d][i], s0 = h.s0[d][i], c0 = h.c0[d][i];
-            dr = ((((c0 * tau / 5 + s0) * tau / 4 + j0) * tau / 3 + a0) * tau / 2 + v0) * tau;
-            dv = (((c0 * tau / 4 + s0) * tau / 3 + j0) * tau / 2 + a0) * tau;
-        }
-        ro[d][i] = r0 + dr;
-        vo[d][i] = v0 + dv;
-    }
+    for (int d = 0; d < 3; ++d)
+        predict_dim<Order>(h, i, d, tau, h.a0[d][i], Order >= 4 ? h.j0[d][i] : 0.0, rs, vs,
+                           ro, vo);
 }
 
 template <int Order>
@@ -105,50 +113,60 @@ __global__ __launch_bounds__(256) void hermite_predict_kernel(int n, HermiteArra
     predict_one<Order>(h, i, tau, h.r0, h.v0, h.r, h.v);
 }
 
-// hermite.py:20-410, corrector of H2/H4/H6/H8 for particle i (v first, then
-// r with the new v): the start-of-step r, v from r0s, v0s, the forces a0..
-// and a1.., which it also copies into the live a (j, s, c); the corrected
-// r, v go to ro, vo.
+// hermite.py:20-410, corrector of H2/H4/H6/H8 for component d of particle
+// i (v first, then r with the new v): the start-of-step r, v from r0s, v0s,
+// the start forces a0.. and the new forces a1 (j1 for order 4 passed in,
+// higher derivatives read from s1, c1), which it also copies into the live
+// a (j, s, c); the corrected r, v go to ro, vo.
+template <int Order>
+__device__ __forceinline__ void correct_dim(const HermiteArrays& h, int i, int d, double tau,
+                                            double a1, double j1, double* const* r0s,
+                                            double* const* v0s, double* const* ro,
+                                            double* const* vo)
+{
+    const double r0 = r0s[d][i], v0 = v0s[d][i];
+    const double a0 = h.a0[d][i];
+    h.a[d][i] = a1;
+    if constexpr (Order >= 4) h.j[d][i] = j1;
+    if constexpr (Order >= 6) {
+        h.s[d][i] = h.s1[d][i];
+        h.c[d][i] = h.c1[d][i];
+    }
+    double v1, r1;
+    if constexpr (Order == 2) {
+        v1 = (a0 + a1) * tau / 2 + v0;
+        r1 = (v0 + v1) * tau / 2 + r0;
+    } else if constexpr (Order == 4) {
+        const double j0 = h.j0[d][i];
+        v1 = ((j0 - j1) * tau / 6 + (a0 + a1)) * tau / 2 + v0;
+        r1 = ((a0 - a1) * tau / 6 + (v0 + v1)) * tau / 2 + r0;
+    } else if constexpr (Order == 6) {
+        const double j0 = h.j0[d][i];
+        const double s0 = h.s0[d][i], s1 = h.s1[d][i];
+        v1 = (((s0 + s1) * tau / 12 + (j0 - j1)) * tau / 5 + (a0 + a1)) * tau / 2 + v0;
+        r1 = (((j0 + j1) * tau / 12 + (a0 - a1)) * tau / 5 + (v0 + v1)) * tau / 2 + r0;
+    } else {
+        const double j0 = h.j0[d][i];
+        const double s0 = h.s0[d][i], s1 = h.s1[d][i];
+        const double c0 = h.c0[d][i], c1 = h.c1[d][i];
+        v1 = ((((c0 - c1) * tau / 20 + (s0 + s1)) * tau / 3 + 3 * (j0 - j1)) * tau / 14 +
+              (a0 + a1)) * tau / 2 + v0;
+        r1 = ((((s0 - s1) * tau / 20 + (j0 + j1)) * tau / 3 + 3 * (a0 - a1)) * tau / 14 +
+              (v0 + v1)) * tau / 2 + r0;
+    }
+    vo[d][i] = v1;
+    ro[d][i] = r1;
+}
+
 template <int Order>
 __device__ __forceinline__ void correct_one(const HermiteArrays& h, int i, double tau,
                                             double* const* r0s, double* const* v0s,
                                             double* const* ro, double* const* vo)
 {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const double r0 = r0s[d][i], v0 = v0s[d][i];
-        const double a0 = h.a0[d][i], a1 = h.a1[d][i];
-        h.a[d][i] = a1;
-        if constexpr (Order >= 4) h.j[d][i] = h.j1[d][i];
-        if constexpr (Order >= 6) {
-            h.s[d][i] = h.s1[d][i];
-            h.c[d][i] = h.c1[d][i];
-        }
-        double v1, r1;
-        if constexpr (Order == 2) {
-            v1 = (a0 + a1) * tau / 2 + v0;
-            r1 = (v0 + v1) * tau / 2 + r0;
-        } else if constexpr (Order == 4) {
-            const double j0 = h.j0[d][i], j1 = h.j1[d][i];
-            v1 = ((j0 - j1) * tau / 6 + (a0 + a1)) * tau / 2 + v0;
-            r1 = ((a0 - a1) * tau / 6 + (v0 + v1)) * tau / 2 + r0;
-        } else if constexpr (Order == 6) {
-            const double j0 = h.j0[d][i], j1 = h.j1[d][i];
-            const double s0 = h.s0[d][i], s1 = h.s1[d][i];
-            v1 = (((s0 + s1) * tau / 12 + (j0 - j1)) * tau / 5 + (a0 + a1)) * tau / 2 + v0;
-            r1 = (((j0 + j1) * tau / 12 + (a0 - a1)) * tau / 5 + (v0 + v1)) * tau / 2 + r0;
-        } else {
-            const double j0 = h.j0[d][i], j1 = h.j1[d][i];
-            const double s0 = h.s0[d][i], s1 = h.s1[d][i];
-            const double c0 = h.c0[d][i], c1 = h.c1[d][i];
-            v1 = ((((c0 - c1) * tau / 20 + (s0 + s1)) * tau / 3 + 3 * (j0 - j1)) * tau / 14 +
-                  (a0 + a1)) * tau / 2 + v0;
-            r1 = ((((s0 - s1) * tau / 20 + (j0 + j1)) * tau / 3 + 3 * (a0 - a1)) * tau / 14 +
-                  (v0 + v1)) * tau / 2 + r0;
-        }
-        vo[d][i] = v1;
-        ro[d][i] = r1;
-    }
+    for (int d = 0; d < 3; ++d)
+        correct_dim<Order>(h, i, d, tau, h.a1[d][i], Order >= 4 ? h.j1[d][i] : 0.0, r0s, v0s,
+                           ro, vo);
 }
 
 template <int Order>
@@ -183,11 +201,14 @@ __global__ __launch_bounds__(256) void hermite_end_kernel(int n, HermiteArrays h
 }
 
 // AccJerk of the small-set kernel with a Hermite-4 epilogue in its
-// per-particle store.  Stage 0: forces of the start state into a0, j0, then
-// the predictor into the r0, v0 scratch; stage 1: forces of the predicted
-// state into a1, j1, then the corrector into the s0, c0 scratch; stage 2:
-// forces of that state into a1, j1, then the corrector into the live r, v
-// and the bookkeeping.
+// per-particle store, spread over the particle's lane group: lane d < 3
+// runs component d, lane 3 the bookkeeping.  Stage 0: forces of the start
+// state into a0, j0, then the predictor into the r0, v0 scratch; stage 1:
+// forces of the predicted state into a1, j1, then the corrector into the
+// s0, c0 scratch; stage 2: forces of that state into a1, j1, then the
+// corrector into the live r, v and the bookkeeping.  The new forces come
+// from the lanes' registers (the values the store writes), never from the
+// arrays lane 0 is writing.
 template <class Base, int Stage>
 struct OpHermite4 : Base {
     struct Args : Base::Args {
@@ -195,18 +216,24 @@ struct OpHermite4 : Base {
         double* ctl;
         int adaptive;
     };
-    __device__ static void store(const Args& a, int i, const double* t)
+    __device__ static void store_group(const Args& a, int i, const double* t, int lane)
     {
-        Base::store(a, i, t);
+        if (lane == 0) Base::store(a, i, t);
         const double tau = a.ctl[kCtlTau];
         if (tau == 0) return;
         const HermiteArrays& h = a.h;
-        if constexpr (Stage == 0) {
-            predict_one<4>(h, i, tau, nullptr, nullptr, h.r0, h.v0);
-        } else if constexpr (Stage == 1) {
-            correct_one<4>(h, i, tau, h.r, h.v, h.s0, h.c0);
-        } else {
-            correct_one<4>(h, i, tau, h.r, h.v, h.r, h.v);
+        if (lane < 3) {
+            const int d = lane;
+            const double f = d == 0 ? t[0] : (d == 1 ? t[1] : t[2]);
+            const double g = d == 0 ? t[3] : (d == 1 ? t[4] : t[5]);
+            if constexpr (Stage == 0) {
+                predict_dim<4>(h, i, d, tau, f, g, nullptr, nullptr, h.r0, h.v0);
+            } else if constexpr (Stage == 1) {
+                correct_dim<4>(h, i, d, tau, f, g, h.r, h.v, h.s0, h.c0);
+            } else {
+                correct_dim<4>(h, i, d, tau, f, g, h.r, h.v, h.r, h.v);
+            }
+        } else if (lane == 3 && Stage == 2) {
             end_one(h, i, tau, a.adaptive);
             if (i == 0) a.ctl[kCtlT] = a.ctl[kCtlT] + tau;
         }
