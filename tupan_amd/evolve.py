@@ -3,7 +3,8 @@
     python -m tupan_amd.evolve --n 16384 --method ahermite4 --eta 0.01 \\
         --t-end 1 --mode split32 [--eps 1e-4] [--imf equalmass] [--seed 1] \\
         [--graph-steps 64] [--chunk 0.0625] [--max-wall 300] \\
-        [--save state.npz] [--resume state.npz]
+        [--save state.npz] [--resume state.npz] [--input ic.hdf5] \
+        [--output worldline.hdf5 --dump-freq 1] [--snapshot final.hdf5]
 
 Hermite methods run as device steps replayed in HIP graphs of
 --graph-steps steps (--graph-steps 0: the host-driven driver); other
@@ -20,6 +21,13 @@ from such a file (the other options must be the run's own).  A step depends
 only on the particle arrays and the time, and power-of-two block steps
 aligned with the time never cross a chunk boundary, so a resumed run takes
 the same steps as an uninterrupted one.
+
+--input takes the initial particles from the ``Snapshot`` group of an HDF5
+file in the reference's layout (tupan/simulation.py:146-147) instead of a
+Plummer sphere; --output writes the worldline there, as the reference's
+dumpper does (the initial state, then every dump_freq-th step of every
+particle; host-driven steps only, --graph-steps 0); --snapshot writes the
+final state as a snapshot.
 """
 import argparse
 import json
@@ -47,7 +55,16 @@ def main(argv=None):
                     help="write the run's state here after every chunk")
     ap.add_argument("--resume", default=None,
                     help="continue the run saved in this file")
+    ap.add_argument("--input", default=None,
+                    help="initial conditions from this HDF5 snapshot")
+    ap.add_argument("--output", default=None,
+                    help="write the worldline to this HDF5 file")
+    ap.add_argument("--dump-freq", type=int, default=1)
+    ap.add_argument("--snapshot", default=None,
+                    help="write the final state to this HDF5 file")
     args = ap.parse_args(argv)
+    if args.output and "hermite" in args.method and args.graph_steps > 0:
+        ap.error("--output needs host-driven steps (--graph-steps 0)")
 
     import os
     import numpy as np
@@ -71,8 +88,13 @@ def main(argv=None):
         print("resumed at t = %.6f (%.1f s before)" % (t, wall_before),
               file=sys.stderr, flush=True)
     else:
-        ps = make_plummer(args.n, args.eps, (args.imf,), seed=args.seed,
-                          device="cuda:0")
+        if args.input:
+            from .io import IO
+            ps = IO(args.input, "r").load_snapshot(device="cuda:0")
+            args.n = ps.n
+        else:
+            ps = make_plummer(args.n, args.eps, (args.imf,), seed=args.seed,
+                              device="cuda:0")
         type(ps).t_curr = 0.0
         t = 0.0
         e0 = energy(ps)
@@ -83,6 +105,9 @@ def main(argv=None):
     extensions.install("HIP", "float64")
     hermite = "hermite" in args.method
     kw = {"device_steps": True} if hermite and args.graph_steps > 0 else {}
+    if args.output:
+        from .io import IO
+        kw.update(dumpper=IO(args.output, "a"), dump_freq=args.dump_freq)
     it = Integrator(args.eta, t, ps, method=args.method, **kw)
 
     def save(t_now, wall_now):
@@ -98,7 +123,7 @@ def main(argv=None):
     t0 = time.perf_counter() - wall_before
     while t < args.t_end:
         target = min(args.t_end, t + args.chunk)
-        if kw:
+        if kw.get("device_steps"):
             it.integrator.evolve(target, steps_per_graph=args.graph_steps)
         else:
             while it.time < target:
@@ -122,6 +147,9 @@ def main(argv=None):
     extensions.install("HIP", "float64")
     e1 = energy(q)
     steps = int(q.nstep.max().item())
+    if args.snapshot:
+        from .io import IO
+        IO(args.snapshot, "w").dump_snapshot(q)
     print(json.dumps({"n": args.n, "eps": args.eps, "imf": args.imf,
                       "seed": args.seed, "method": args.method,
                       "eta": args.eta, "mode": args.mode, "t": t,

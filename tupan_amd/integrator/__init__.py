@@ -11,6 +11,7 @@ import math
 import sys
 
 import numpy as np
+import torch
 
 
 class Base(object):
@@ -68,21 +69,46 @@ class Base(object):
         return min_bts
 
     def evolve_step(self, t_end):
-        """integrator/__init__.py:80-99."""
+        """integrator/__init__.py:80-99: with a dumpper, the worldline
+        samples that this step collects are written after it."""
         if not self.is_initialized:
             self.initialize(t_end)
         ps = self.ps
-        self.wl = None
+        self.wl = type(ps)(device=ps.device) if self.dumpper else None
         tau = self.get_base_tstep(t_end)
         ps = self.do_step(ps, tau)
         if self.reporter:
             self.reporter.diagnostic_report(ps)
+        if self.dumpper:
+            self.dumpper.dump_worldline(self.wl)
         self.ps = ps
 
     def initialize(self, t_end):
+        """integrator/__init__.py:142-144 with hermite.py:312-328: report
+        and dump the initial state."""
         if self.reporter:
             self.reporter.diagnostic_report(self.ps)
+        if self.dumpper:
+            self.dumpper.dump_worldline(self.ps)
         self.is_initialized = True
+
+    def record_worldline(self, ps, tau):
+        """hermite.py:402-405 (sakura.py:151-154, nreg.py:167-170,
+        sia.py:1114-1117): with a dumpper, the particles of ``ps`` whose
+        time is a multiple of dump_freq * tau join this step's worldline.
+        fmod is exact, so its zeros are those of the reference's numpy
+        ``%``."""
+        if not self.dumpper:
+            return
+        slc = torch.fmod(ps.time, self.dump_freq * tau) == 0
+        if bool(slc.any()):
+            self.wl.append(ps[slc])
+
+    def no_dumpper(self, what):
+        """Device-driven steps do not collect worldlines."""
+        if self.dumpper:
+            raise ValueError("%s does not collect worldlines; use the "
+                             "host-driven steps with a dumpper" % what)
 
     def finalize(self, t_end):
         pass

@@ -290,6 +290,7 @@ class Hermite(Base):
         (and one read of the new time)."""
         if not self.device_steps:
             return super(Hermite, self).evolve_step(t_end)
+        self.no_dumpper("device_steps")
         if not self.is_initialized:
             self.initialize(t_end)
         dev = self._device()
@@ -305,6 +306,7 @@ class Hermite(Base):
         nothing.  Returns the number of replays."""
         if not self.device_steps:
             raise RuntimeError("evolve needs device_steps=True")
+        self.no_dumpper("device_steps")
         if steps_per_graph < 1:
             raise ValueError("steps_per_graph must be at least 1")
         if not self.is_initialized:
@@ -345,4 +347,5 @@ class Hermite(Base):
         ps.tstep[...] = tau
         ps.time += tau
         ps.nstep += 1
+        self.record_worldline(ps, tau)
         return ps

@@ -77,6 +77,8 @@ class NREG(Base):
         type(ps).S = -pe
         if self.reporter:
             self.reporter.diagnostic_report(ps)
+        if self.dumpper:
+            self.dumpper.dump_worldline(ps)
         self.is_initialized = True
 
     def do_step(self, ps, tau):
@@ -89,4 +91,5 @@ class NREG(Base):
         ps.tstep[...] = dt
         ps.time += tau
         ps.nstep += 1
+        self.record_worldline(ps, tau)
         return ps

@@ -205,6 +205,7 @@ class Sakura(Base):
         (and one read of the new time)."""
         if not self.device_steps:
             return super(Sakura, self).evolve_step(t_end)
+        self.no_dumpper("device_steps")
         if not self.is_initialized:
             self.initialize(t_end)
         dev = self._device()
@@ -223,6 +224,7 @@ class Sakura(Base):
         captured again.  Returns the number of replays."""
         if not self.device_steps:
             raise RuntimeError("evolve needs device_steps=True")
+        self.no_dumpper("device_steps")
         if steps_per_graph < 1:
             raise ValueError("steps_per_graph must be at least 1")
         if not self.is_initialized:
@@ -269,4 +271,5 @@ class Sakura(Base):
         ps.tstep[...] = tau
         ps.time += tau
         ps.nstep += 1
+        self.record_worldline(ps, tau)
         return ps

@@ -297,6 +297,7 @@ class SIA(Base):
         launch, takes the host driver's step."""
         if self.device_levels and not self.device_steps and \
                 self.ps.n > SUBTREE_MAX:
+            self.no_dumpper("device_levels")
             if not self.is_initialized:
                 self.initialize(t_end)
             self._device_levels().step(t_end,
@@ -306,6 +307,7 @@ class SIA(Base):
             return
         if not self.device_steps:
             return super(SIA, self).evolve_step(t_end)
+        self.no_dumpper("device_steps")
         if not self.is_initialized:
             self.initialize(t_end)
         dev = self._device()
@@ -323,6 +325,7 @@ class SIA(Base):
         level graphs until the device reports the step done.  Returns the
         number of replays."""
         if self.device_levels and not self.device_steps:
+            self.no_dumpper("device_levels")
             if steps_per_graph < 1:
                 raise ValueError("steps_per_graph must be at least 1")
             if self.ps.n <= SUBTREE_MAX:
@@ -340,6 +343,7 @@ class SIA(Base):
         if not self.device_steps:
             raise RuntimeError("evolve needs device_steps=True or "
                                "device_levels=True")
+        self.no_dumpper("device_steps")
         if steps_per_graph < 1:
             raise ValueError("steps_per_graph must be at least 1")
         if not self.is_initialized:
@@ -371,7 +375,10 @@ class SIA(Base):
         for name in names:
             if name not in ps.__dict__:
                 ps.register_auxiliary_attribute(name, "real")
-        self._subtree = _DeviceSubtree.make(self, ps)
+        # the one-launch subtree runs its levels on the device, where no
+        # worldline samples are taken
+        self._subtree = (None if self.dumpper
+                         else _DeviceSubtree.make(self, ps))
         return self.recurse(ps, tau, self.evolve_set, self.scheme.bridge_sf)
 
     def recurse(self, ps, tau, evolve, bridge_sf):
@@ -397,6 +404,7 @@ class SIA(Base):
             slow.tstep[...] = tau
             slow.time += tau
             slow.nstep += 1
+            self.record_worldline(slow, tau)
         return ps if self.inplace else join(slow, fast)
 
 

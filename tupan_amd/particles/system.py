@@ -36,6 +36,15 @@ ATTRS = (("id", "uint"), ("mass", "real"), ("eps2", "real"),
          ("vx", "real"), ("vy", "real"), ("vz", "real"),
          ("time", "real"), ("nstep", "uint"), ("tstep", "real"))
 MEMBER_TYPES = ("bodies", "stars", "blackholes", "sphs")
+# Attributes each member type adds to the base ones (star.py:21-28,
+# blackhole.py:21-26, sph.py:21-23); all "real".  They exist on a system
+# only when registered (or loaded from a file that holds them).
+MEMBER_EXTRAS = {
+    "bodies": (),
+    "stars": ("spinx", "spiny", "spinz", "radius", "age", "metallicity"),
+    "blackholes": ("spinx", "spiny", "spinz", "radius"),
+    "sphs": ("density",),
+}
 
 
 def default_device():
@@ -59,9 +68,10 @@ class Member(object):
     """View of one member type: attribute access returns the slice of the
     system's arrays that belongs to it."""
 
-    def __init__(self, system, start, stop):
+    def __init__(self, system, start, stop, name="bodies"):
         self.__dict__["_ps"] = system
         self.__dict__["_sl"] = slice(start, stop)
+        self.__dict__["name"] = name
 
     @property
     def n(self):
@@ -78,6 +88,20 @@ class Member(object):
 
     def __setattr__(self, name, value):
         getattr(self, name)[...] = value
+
+    def get_state(self):
+        """Structured array of this member's particles in its type's record
+        layout: the base attributes, then the type's own (body.py:664-668
+        with the attrs lists of star.py, blackhole.py, sph.py); an own
+        attribute the system does not hold is written as zeros."""
+        dt = ([(name, ctype.ctypedict[sct]) for name, sct in ATTRS]
+              + [(name, ctype.REAL) for name in MEMBER_EXTRAS[self.name]])
+        out = np.zeros(self.n, dtype=dt)
+        arrays = self.__dict__["_ps"].__dict__
+        for name in out.dtype.names:
+            if name in arrays and isinstance(arrays[name], torch.Tensor):
+                out[name] = arrays[name][self._sl].detach().cpu().numpy()
+        return out
 
 
 class ParticleSystem(object):
@@ -115,7 +139,7 @@ class ParticleSystem(object):
         start = 0
         for name, k in self._member_counts:
             if k:
-                out[name] = Member(self, start, start + k)
+                out[name] = Member(self, start, start + k, name)
             start += k
         return out
 
@@ -501,9 +525,14 @@ class ParticleSystem(object):
 
 
 def _merge_counts(counts):
+    """Drop empty ranges and join neighbouring ranges of one member type
+    (a worldline appends many slices of the same type)."""
     out = []
     for name, k in counts:
         if k == 0:
             continue
-        out.append((name, k))
+        if out and out[-1][0] == name:
+            out[-1] = (name, out[-1][1] + k)
+        else:
+            out.append((name, k))
     return out or [("bodies", 0)]
