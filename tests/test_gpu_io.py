@@ -92,3 +92,38 @@ def test_evolve_reads_and_writes_hdf5(tmp_path, capsys):
     final = IO(fin, "r").load_snapshot(device="cpu").get_state()
     check_worldline(wl, IO(ic, "r").load_snapshot(device="cpu").get_state(),
                     final)
+
+
+def test_gpu_diagnostic_report(tmp_path):
+    """The reference's report file (simulation.py:40-129) from device
+    Hermite steps: the header, then one 13-column line per report_freq
+    reports, te = ke + pe as printed, eerr = (te - te0) / (-pe) against the
+    energies the system reports at the same time."""
+    from tupan_amd.diagnostics import Diagnostic
+    from tupan_amd.integrator import Integrator
+    from tupan_amd.lib import extensions
+    extensions.install("HIP", "float64")
+    print("Diagnostic from device ahermite4 steps, N = 256", flush=True)
+    ps = plummer()
+    fname = str(tmp_path / "diag.txt")
+    rep = Diagnostic(fname, 0.0, ps, report_freq=2)
+    it = Integrator(1.0 / 64, 0.0, ps, method="ahermite4", reporter=rep,
+                    device_steps=True)
+    for k in range(6):
+        it.evolve_step(1.0)
+    torch.cuda.synchronize()
+    lines = open(fname).read().splitlines()
+    assert lines[0].split() == list(Diagnostic.HEADER)
+    body = [[float(x) for x in ln.split()] for ln in lines[1:]]
+    assert len(body) == 4 and all(len(r) == 13 for r in body)
+    assert body[0][0] == 0.0 and body[0][6] == 0.0
+    for r in body:
+        t, dt, ke, pe, te, vir, eerr = r[:7]
+        assert abs(te - (ke + pe)) < 1e-3 * abs(te)
+    last = rep.rows[-1]
+    q = it.particle_system
+    assert last["time"] == it.time
+    assert abs(last["te"] - (q.kinetic_energy + q.potential_energy)) < \
+        1e-12 * abs(last["te"])
+    assert last["eerr"] == (last["te"] - rep.te0) / (-last["pe"])
+    assert abs(last["eerr"]) < 1e-6
