@@ -39,6 +39,17 @@ struct HasUniformE2 : std::false_type {};
 template <class Op>
 struct HasUniformE2<Op, std::void_t<decltype(Op::kUniformE2)>>
     : std::integral_constant<bool, Op::kUniformE2> {};
+// Ops that declare unmask_allowed may run a uniform tile without their
+// pair mask when the call allows it, and, for ops that also declare
+// self_safe, every j of the tile passes that (see pair_body).
+template <class Op, class = void>
+struct HasUnmask : std::false_type {};
+template <class Op>
+struct HasUnmask<Op, std::void_t<decltype(&Op::unmask_allowed)>> : std::true_type {};
+template <class Op, class = void>
+struct HasSelfSafe : std::false_type {};
+template <class Op>
+struct HasSelfSafe<Op, std::void_t<decltype(&Op::self_safe)>> : std::true_type {};
 template <class Op, bool = HasUniformE2<Op>::value>
 struct SoftType {
     using type = double;
@@ -106,9 +117,24 @@ __device__ void fallback64(const typename F::Args& a, int i, int j0, int j1, int
     for (int k = 0; k < F::kNacc; ++k) tot[k] = F::get(acc, k);
 }
 
+// One pair of a tile pass: uniform (kU) or not, masked (kM) or not.
+template <class Op, bool kU, bool kM, class E2>
+__device__ __forceinline__ void tile_pair(const typename Op::IReg& r,
+                                          const typename Op::Slot& s, typename Op::Acc& c,
+                                          E2 e2u)
+{
+    if constexpr (kU && !kM) {
+        Op::template interact<true, false>(r, s, c, e2u);
+    } else if constexpr (kU) {
+        Op::template interact<true>(r, s, c, e2u);
+    } else {
+        Op::interact(r, s, c);
+    }
+}
+
 // One tile's interactions: UNROLL slots at a time, each applied to the WPT
 // i-particles, accumulators flushed every CH slots.
-template <class Op, int WPT, int UNROLL, int CH, bool kU, class E2>
+template <class Op, int WPT, int UNROLL, int CH, bool kU, bool kM, class E2>
 __device__ __forceinline__ void tile_pass(const typename Op::Slot* tile, int cnt,
                                           const typename Op::IReg (&ir)[WPT],
                                           typename Op::Acc (&acc)[WPT], E2 e2u)
@@ -122,25 +148,13 @@ __device__ __forceinline__ void tile_pass(const typename Op::Slot* tile, int cnt
             for (int u = 0; u < UNROLL; ++u) {
                 const Slot s = tile[k + u];
 #pragma unroll
-                for (int w = 0; w < WPT; ++w) {
-                    if constexpr (kU) {
-                        Op::template interact<true>(ir[w], s, acc[w], e2u);
-                    } else {
-                        Op::interact(ir[w], s, acc[w]);
-                    }
-                }
+                for (int w = 0; w < WPT; ++w) tile_pair<Op, kU, kM>(ir[w], s, acc[w], e2u);
             }
         }
         for (; k < kend; ++k) {
             const Slot s = tile[k];
 #pragma unroll
-            for (int w = 0; w < WPT; ++w) {
-                if constexpr (kU) {
-                    Op::template interact<true>(ir[w], s, acc[w], e2u);
-                } else {
-                    Op::interact(ir[w], s, acc[w]);
-                }
-            }
+            for (int w = 0; w < WPT; ++w) tile_pair<Op, kU, kM>(ir[w], s, acc[w], e2u);
         }
 #pragma unroll
         for (int w = 0; w < WPT; ++w) Op::flush(acc[w]);
@@ -176,8 +190,15 @@ __device__ __forceinline__ void pair_body(const typename Op::Args& a, int bid)
     // uniform path when every i of the workgroup and every j of the tile
     // carry exactly that value; the pair value E + E is then formed once,
     // the same number the per-pair sum gives, so results do not change.
+    // Ops with unmask_allowed run a uniform tile unmasked when the call
+    // allows it (and, with self_safe, every j of the tile passes that):
+    // their sums are then the masked ones.
+    constexpr bool kUnmask = kUni && HasUnmask<Op>::value;
+    constexpr bool kSelf = kUnmask && HasSelfSafe<Op>::value;
     bool iuni = true;
     [[maybe_unused]] typename SoftType<Op>::type e2_first{};
+    [[maybe_unused]] bool unmask = false;
+    [[maybe_unused]] float y_self = 0.0f;
     if constexpr (kUni) {
         e2_first = Op::e2_of_i(a, ibase);
 #pragma unroll
@@ -185,6 +206,14 @@ __device__ __forceinline__ void pair_body(const typename Op::Args& a, int bid)
             const int i = ibase + w * BLOCK + tid;
             iuni = iuni && (i >= a.ni || Op::e2_of_i(a, i) == e2_first);
         }
+    }
+    if constexpr (kUnmask) unmask = Op::unmask_allowed(a);
+    if constexpr (kSelf) {
+        // the same for every thread: held in a scalar register, so the pair
+        // loop keeps its vector registers
+        y_self = __builtin_bit_cast(
+            float, __builtin_amdgcn_readfirstlane(
+                       __builtin_bit_cast(int, Op::self_y(e2_first + e2_first))));
     }
 
     const int j0 = g.jsplit * a.jchunk;
@@ -195,19 +224,30 @@ __device__ __forceinline__ void pair_body(const typename Op::Args& a, int bid)
         const int cnt = min(TILE, j1 - jt);
         __syncthreads();
         bool tuni = iuni;
+        [[maybe_unused]] bool tsafe = unmask;
         for (int k = tid; k < cnt; k += BLOCK) {
             Op::load_j(a, fr, jt + k, tile[k]);
             if constexpr (kUni) tuni = tuni && Op::slot_e2(tile[k]) == e2_first;
+            if constexpr (kSelf) tsafe = tsafe && Op::self_safe(y_self, tile[k]);
         }
-        if constexpr (kUni) {
-            if (__syncthreads_and(tuni)) {
-                tile_pass<Op, WPT, UNROLL, CH, true>(tile, cnt, ir, acc, e2_first + e2_first);
+        if constexpr (kUnmask) {
+            // a uniform tile that may not run unmasked takes the general
+            // loop (the same sums; one more add per pair), so the kernel
+            // holds two pair loops, not three
+            if (__syncthreads_and(tuni && tsafe)) {
+                tile_pass<Op, WPT, UNROLL, CH, true, false>(tile, cnt, ir, acc, e2_first + e2_first);
             } else {
-                tile_pass<Op, WPT, UNROLL, CH, false>(tile, cnt, ir, acc, e2_first);
+                tile_pass<Op, WPT, UNROLL, CH, false, true>(tile, cnt, ir, acc, e2_first);
+            }
+        } else if constexpr (kUni) {
+            if (__syncthreads_and(tuni)) {
+                tile_pass<Op, WPT, UNROLL, CH, true, true>(tile, cnt, ir, acc, e2_first + e2_first);
+            } else {
+                tile_pass<Op, WPT, UNROLL, CH, false, true>(tile, cnt, ir, acc, e2_first);
             }
         } else {
             __syncthreads();
-            tile_pass<Op, WPT, UNROLL, CH, false>(tile, cnt, ir, acc, 0);
+            tile_pass<Op, WPT, UNROLL, CH, false, true>(tile, cnt, ir, acc, 0);
         }
     }
 

@@ -135,6 +135,9 @@ struct PairArgs {
     T clight[5];  // PNAcc: 1/c^2, 1/c^4, 1/c^5, 1/c^6, 1/c^7
     int order;    // PNAcc: post-Newtonian order
     int exact;    // host-side launch planning: keep whole-tile j-chunks
+    // split AccJerk: 0 here when no i and j share a position with different
+    // velocities (pairwise.hip: the coincidence scan); null: not scanned
+    const int* coincide;
 };
 
 // ---------------------------------------------------------------------------
@@ -543,12 +546,23 @@ struct SplitFrame {
 // is 1.
 constexpr double kRsqBias = -7.35832e-9;
 
-template <bool kRefine>
+// kMask false: no R^2 > e2 test (the unmasked tiles of the split Acc and
+// AccJerk, see their self_safe).
+template <bool kRefine, bool kMask = true>
 __device__ __forceinline__ float split_rsqrt(float dx, float dy, float dz, float e2)
 {
     const float R2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, __builtin_fmaf(dx, dx, e2)));
     const float y = kRefine ? rsqrt_refined(R2) : __builtin_amdgcn_rsqf(R2);
+    if constexpr (!kMask) return y;
     return R2 > e2 ? y : 0.0f;
+}
+
+// 1/R of a pair at zero separation under the uniform softening e2u: what
+// split_rsqrt gives for dx = dy = dz = 0 (its R^2 is then e2u exactly).
+template <bool kRefine>
+__device__ __forceinline__ float split_self_rsqrt(float e2u)
+{
+    return kRefine ? rsqrt_refined(e2u) : __builtin_amdgcn_rsqf(e2u);
 }
 
 template <bool kRefine>
@@ -597,19 +611,35 @@ struct OpSplitAccT {
         s.v[3] = static_cast<float>(a.jp[0][j]);
         s.v[7] = static_cast<float>(a.jp[4][j]);
     }
+    // Unmasked uniform tiles.  The mask R^2 > e2 drops the pairs at zero
+    // separation (the reference's r^2 > 0).  Under a uniform softening whose
+    // zero-separation term y^3 m_j is finite for every j of the tile
+    // (self_safe, checked per tile), such a pair adds -q3 * 0 = 0 to each
+    // sum, so the tile runs without the test: the same sums, two
+    // instructions fewer per pair.  (A pair with 0 < r^2 below half an FP32
+    // ulp of e2, which the mask also drops, then contributes, as in the
+    // reference.)
+    __device__ static bool unmask_allowed(const Args&) { return true; }
+    __device__ static float self_y(float e2u) { return split_self_rsqrt<kRefine>(e2u); }
+    __device__ static bool self_safe(float y0, const Slot& s)
+    {
+        const float q3 = (y0 * y0) * (y0 * s.v[3]);
+        return __builtin_isfinite(q3);
+    }
     __device__ static void zero(Acc& c)
     {
         c.ax = c.ay = c.az = 0.0f;
         c.tx = c.ty = c.tz = 0.0;
     }
-    template <bool kU = false>
+    template <bool kU = false, bool kM = true>
     __device__ static __forceinline__ void interact(const IReg& r, const Slot& s, Acc& c,
                                                     float e2u = float(0))
     {
         float dx = (r.xh - s.v[0]) + (r.xl - s.v[4]);
         float dy = (r.yh - s.v[1]) + (r.yl - s.v[5]);
         float dz = (r.zh - s.v[2]) + (r.zl - s.v[6]);
-        const float y = split_rsqrt<kRefine>(dx, dy, dz, kU ? e2u : r.e2 + s.v[7]);
+        const float y =
+            split_rsqrt<kRefine, kM || !kU>(dx, dy, dz, kU ? e2u : r.e2 + s.v[7]);
         float q3 = (y * y) * (y * s.v[3]);
         c.ax = __builtin_fmaf(-q3, dx, c.ax);
         c.ay = __builtin_fmaf(-q3, dy, c.ay);
@@ -681,12 +711,36 @@ struct OpSplitAccJerkT {
         s.v[10] = static_cast<float>(a.jp[7][j] - f.cvz);
         s.v[11] = 0.0f;
     }
+    // Unmasked uniform tiles, as OpSplitAccT's: a pair at zero separation
+    // then has rv = 0, alpha = 0 and adds -q3 * 0 to the accelerations and
+    // -q3 * (v_i - v_j) to the jerks, zero for a particle with itself.  Two
+    // particles at one position with different velocities would add a
+    // non-zero jerk, which the reference drops (r^2 = 0).  The call's
+    // coincidence scan (pairwise.hip) clears the flag only when no such
+    // pair exists and every j passes self_safe_j, and then every uniform
+    // tile goes unmasked.
+    static constexpr bool kCoincideScan = true;
+    __device__ static bool unmask_allowed(const Args& a)
+    {
+        return a.coincide != nullptr && *a.coincide == 0;
+    }
+    // j's zero-separation term is finite in any uniform tile it belongs to:
+    // there every pair softening is e2_j + e2_j (in FP32, as load_j and
+    // e2_of_i round it).
+    __device__ static bool self_safe_j(const Args& a, int j)
+    {
+        const float e2 = static_cast<float>(a.jp[4][j]);
+        const float y0 = split_self_rsqrt<kRefine>(e2 + e2);
+        const float q2 = y0 * y0;
+        const float q3 = q2 * (y0 * static_cast<float>(a.jp[0][j]));
+        return __builtin_isfinite(q2) && __builtin_isfinite(q3);
+    }
     __device__ static void zero(Acc& c)
     {
 #pragma unroll
         for (int k = 0; k < 6; ++k) { c.s[k] = 0.0f; c.t[k] = 0.0; }
     }
-    template <bool kU = false>
+    template <bool kU = false, bool kM = true>
     __device__ static __forceinline__ void interact(const IReg& r, const Slot& s, Acc& c,
                                                     float e2u = float(0))
     {
@@ -698,7 +752,7 @@ struct OpSplitAccJerkT {
         float uy = r.vy - s.v[9];
         float uz = r.vz - s.v[10];
         float rv = __builtin_fmaf(dz, uz, __builtin_fmaf(dy, uy, dx * ux));
-        const float y = split_rsqrt<kRefine>(dx, dy, dz, e2);
+        const float y = split_rsqrt<kRefine, kM || !kU>(dx, dy, dz, e2);
         float q2 = y * y;
         float q3 = q2 * (y * s.v[3]);
         float alpha = 3.0f * rv * q2;
@@ -772,10 +826,14 @@ struct OpSplitTstep {
         r.vy = static_cast<float>(a.ip[6][i] - f.cvy);
         r.vz = static_cast<float>(a.ip[7][i] - f.cvz);
         r.eta = static_cast<float>(a.eta);
+        r.m *= 2.0f;
     }
+    // masses held doubled (exact): the pair's 2 phi = (2 m_i + 2 m_j) / R
+    // comes out of one product, and 2 q2 phi = q2 (2 phi) exactly
     __device__ static void load_j(const Args& a, const Frame& f, int j, Slot& s)
     {
         OpSplitAccJerkT<false>::load_j(a, f, j, s);
+        s.v[3] *= 2.0f;
     }
     __device__ static void zero(Acc& c)
     {
@@ -803,9 +861,9 @@ struct OpSplitTstep {
         // no mask on 1/R: a masked pair's term is zeroed at the end
         float q1 = __builtin_amdgcn_rsqf(R2);
         float q2 = q1 * q1;
-        float phi = mt * q1;
-        float w2 = q2 * __builtin_fmaf(2.0f, phi, v2);
-        float ga = q2 * __builtin_fmaf(2.0f * q2, phi, w2);
+        float phi2 = mt * q1;  // 2 phi: mt is the doubled mass sum
+        float w2 = q2 * (phi2 + v2);
+        float ga = q2 * __builtin_fmaf(q2, phi2, w2);
         ga *= r.eta * __builtin_amdgcn_rsqf(w2);
         w2 = __builtin_fmaf(-ga, rv, w2);
         w2 = mask ? w2 : 0.0f;

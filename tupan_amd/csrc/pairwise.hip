@@ -42,6 +42,109 @@ static void launch_post_fallback(const typename Op::Args& a, hipStream_t stream)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Coincidence scan (split AccJerk): does an i share its position with a j
+// of different velocity?  Position means all three FP64 components equal,
+// the reference's r^2 = 0 (smoothing.h: such a pair is dropped), and such a
+// pair, unlike a particle with itself, would add a jerk term if its tile ran
+// unmasked (ops.h: OpSplitAccJerkT::unmask_allowed).  The j positions go
+// into a hash table (open addressing, linear probing, a power of two of at
+// least four times nj entries), every i probes it, and the flag stays 0 when no
+// such pair exists and every j's zero-separation term is finite
+// (Op::self_safe_j; bit 1 otherwise).  A NaN position equals nothing; its
+// pairs give non-finite sums and take the FP64 fallback either way.  Calls
+// below kCoincideMinPairs pairs skip the scan and keep the mask.
+// ---------------------------------------------------------------------------
+constexpr double kCoincideMinPairs = 268435456.0;  // 2^28
+
+template <class Op, class = void>
+struct HasCoincideScan : std::false_type {};
+template <class Op>
+struct HasCoincideScan<Op, std::void_t<decltype(Op::kCoincideScan)>>
+    : std::integral_constant<bool, Op::kCoincideScan> {};
+
+__device__ __forceinline__ unsigned long long coincide_bits(double v)
+{
+    return static_cast<unsigned long long>(__double_as_longlong(v == 0.0 ? 0.0 : v));
+}
+
+__device__ __forceinline__ unsigned coincide_hash(double x, double y, double z, unsigned mask)
+{
+    unsigned long long h = coincide_bits(x) * 0x9E3779B97F4A7C15ull;
+    h ^= (coincide_bits(y) + 0x632BE59BD9B4E019ull) * 0xC2B2AE3D27D4EB4Full;
+    h ^= (coincide_bits(z) + 0x165667B19E3779F9ull) * 0xD6E8FEB86659FD93ull;
+    h ^= h >> 31;
+    h *= 0x94D049BB133111EBull;
+    h ^= h >> 29;
+    return static_cast<unsigned>(h) & mask;
+}
+
+__global__ __launch_bounds__(256) void coincide_clear_kernel(int* table, unsigned size, int* flag)
+{
+    const unsigned k = blockIdx.x * 256u + threadIdx.x;
+    if (k < size) table[k] = -1;
+    if (k == 0) *flag = 0;
+}
+
+// j and k (both j-particles) at one position with different velocities.
+__device__ __forceinline__ bool coincide_pair(const PairArgs<double>& a, int j, int k)
+{
+    return a.jp[1][k] == a.jp[1][j] && a.jp[2][k] == a.jp[2][j] && a.jp[3][k] == a.jp[3][j] &&
+           !(a.jp[5][k] == a.jp[5][j] && a.jp[6][k] == a.jp[6][j] && a.jp[7][k] == a.jp[7][j]);
+}
+
+// With `same` (the i-set is the j-set) the insertion is the whole scan:
+// of two particles at one position, the one inserted second passes the
+// first one's entry on its way to a free one, and compares with it.
+template <class Op>
+__global__ __launch_bounds__(256) void coincide_insert_kernel(PairArgs<double> a, int* table,
+                                                              unsigned mask, int* flag, int same)
+{
+    const int j = static_cast<int>(blockIdx.x * 256u + threadIdx.x);
+    if (j >= a.nj) return;
+    if (!Op::self_safe_j(a, j)) atomicOr(flag, 2);
+    unsigned h = coincide_hash(a.jp[1][j], a.jp[2][j], a.jp[3][j], mask);
+    for (unsigned probe = 0; probe <= mask; ++probe) {
+        const int k = atomicCAS(&table[h], -1, j);
+        if (k == -1) return;
+        if (same && coincide_pair(a, j, k)) {
+            atomicOr(flag, 1);
+            return;
+        }
+        h = (h + 1) & mask;
+    }
+    atomicOr(flag, 1);  // no free entry (not reached at quarter load): keep the mask
+}
+
+__global__ __launch_bounds__(256) void coincide_probe_kernel(PairArgs<double> a,
+                                                             const int* table, unsigned mask,
+                                                             int* flag)
+{
+    const int i = static_cast<int>(blockIdx.x * 256u + threadIdx.x);
+    if (i >= a.ni) return;
+    const double x = a.ip[1][i], y = a.ip[2][i], z = a.ip[3][i];
+    unsigned h = coincide_hash(x, y, z, mask);
+    for (unsigned probe = 0; probe <= mask; ++probe) {
+        const int j = table[h];
+        if (j < 0) return;
+        if (a.jp[1][j] == x && a.jp[2][j] == y && a.jp[3][j] == z &&
+            !(a.jp[5][j] == a.ip[5][i] && a.jp[6][j] == a.ip[6][i] && a.jp[7][j] == a.ip[7][i])) {
+            atomicOr(flag, 1);
+            return;
+        }
+        h = (h + 1) & mask;
+    }
+}
+
+// Entries of the coincidence table for nj j-particles: a quarter full at
+// most, so an insertion or a probe ends within ~1.4 entries on average.
+static unsigned coincide_entries(int nj)
+{
+    unsigned size = 256;
+    while (size < 4u * static_cast<unsigned>(nj)) size <<= 1;
+    return size;
+}
+
 template <class Op, int BLOCK, int WPT, int TILE, int UNROLL>
 static int run_pairwise(typename Op::Args a, hipStream_t stream, int wg_per_cu,
                         const FixedScratch* fixed)
@@ -80,15 +183,45 @@ static int run_pairwise(typename Op::Args a, hipStream_t stream, int wg_per_cu,
     a.nsplit = L.nsplit;
     a.jchunk = L.jchunk;
     a.partial = nullptr;
-    if (L.nsplit > 1) {
-        size_t need = static_cast<size_t>(L.nsplit) * Op::kNacc * a.ni * sizeof(double);
+    a.coincide = nullptr;
+    // the coincidence table and flag follow the partials in the workspace
+    bool scan = false;
+    if constexpr (HasCoincideScan<Op>::value)
+        scan = !fixed && static_cast<double>(a.ni) * static_cast<double>(a.nj) >= kCoincideMinPairs;
+    const size_t part_bytes =
+        L.nsplit > 1 ? static_cast<size_t>(L.nsplit) * Op::kNacc * a.ni * sizeof(double) : 0;
+    const size_t part_round = (part_bytes + 255) / 256 * 256;
+    const unsigned table_n = scan ? coincide_entries(a.nj) : 0u;
+    const size_t scan_bytes = scan ? static_cast<size_t>(table_n) * sizeof(int) + 256 : 0;
+    if (L.nsplit > 1 || scan) {
         if (fixed) {
-            if (need > fixed->doubles * sizeof(double))
+            if (part_bytes > fixed->doubles * sizeof(double))
                 return set_error(TUPAN_ERR_ARG, "j-split partials exceed the fixed scratch");
             a.partial = fixed->p;
         } else {
-            a.partial = static_cast<double*>(workspace(need, stream));
-            if (!a.partial) return set_error(TUPAN_ERR_ALLOC, "workspace allocation failed");
+            char* w = static_cast<char*>(workspace(part_round + scan_bytes, stream));
+            if (!w) return set_error(TUPAN_ERR_ALLOC, "workspace allocation failed");
+            if (L.nsplit > 1) a.partial = reinterpret_cast<double*>(w);
+            if constexpr (HasCoincideScan<Op>::value) {
+                if (scan) {
+                    int* flag = reinterpret_cast<int*>(w + part_round);
+                    int* table = flag + 64;
+                    const unsigned mask = table_n - 1;
+                    hipLaunchKernelGGL(coincide_clear_kernel, dim3((table_n + 255) / 256),
+                                       dim3(256), 0, stream, table, table_n, flag);
+                    bool same = a.ni == a.nj;
+                    for (int f : {1, 2, 3, 5, 6, 7}) same = same && a.ip[f] == a.jp[f];
+                    hipLaunchKernelGGL(coincide_insert_kernel<Op>,
+                                       dim3(static_cast<unsigned>((a.nj + 255) / 256)), dim3(256),
+                                       0, stream, a, table, mask, flag, same ? 1 : 0);
+                    if (!same)
+                        hipLaunchKernelGGL(coincide_probe_kernel,
+                                           dim3(static_cast<unsigned>((a.ni + 255) / 256)),
+                                           dim3(256), 0, stream, a,
+                                           static_cast<const int*>(table), mask, flag);
+                    a.coincide = flag;
+                }
+            }
         }
     }
     const unsigned grid = static_cast<unsigned>(L.nblock_i) * static_cast<unsigned>(L.nsplit);

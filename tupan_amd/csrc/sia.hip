@@ -596,6 +596,7 @@ __device__ void lvl_plan(LvlCtl* c, int op, int ilo, int ni, int jlo, int nj)
         a.jp[f] = lvl_field(c, fr[f]) + jlo;
     }
     for (int k = 0; k < 8; ++k) a.out[k] = nullptr;
+    a.coincide = nullptr;
     if (op == 0) {
         a.out[0] = lvl_field(c, R_TSTEP) + ilo;
         a.out[1] = lvl_field(c, R_TSTEPIJ) + ilo;
