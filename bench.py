@@ -52,8 +52,8 @@ HBM_PEAK = 8.0e12
 # calibrated rocprofv3 counter runs in profiles/accjerk_plummer65536_pmc_
 # traffic.md (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, KiB -> bytes).
 TRAFFIC_65536 = {
-    "split32": (14851.4 * 2.0 + 196608.0) * 1024,
-    "fp64": (14627.0 * 2.0 + 98304.0) * 1024,
+    "split32": (14871.8 * 2.0 + 196608.0) * 1024,
+    "fp64": (14637.1 * 2.0 + 98304.0) * 1024,
 }
 
 
