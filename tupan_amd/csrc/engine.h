@@ -39,13 +39,14 @@ struct HasUniformE2 : std::false_type {};
 template <class Op>
 struct HasUniformE2<Op, std::void_t<decltype(Op::kUniformE2)>>
     : std::integral_constant<bool, Op::kUniformE2> {};
-// Ops that declare unmask_allowed may run a uniform tile without their
-// pair mask when the call allows it, and, for ops that also declare
-// self_safe, every j of the tile passes that (see pair_body).
+// Ops whose kUnmask is true may run a uniform tile without their pair
+// mask when the call allows it (unmask_allowed) and, for ops that also
+// declare self_safe, every j of the tile passes that (see pair_body).
 template <class Op, class = void>
 struct HasUnmask : std::false_type {};
 template <class Op>
-struct HasUnmask<Op, std::void_t<decltype(&Op::unmask_allowed)>> : std::true_type {};
+struct HasUnmask<Op, std::void_t<decltype(Op::kUnmask)>>
+    : std::integral_constant<bool, Op::kUnmask> {};
 template <class Op, class = void>
 struct HasSelfSafe : std::false_type {};
 template <class Op>
@@ -115,6 +116,21 @@ __device__ void fallback64(const typename F::Args& a, int i, int j0, int j1, int
     }
     F::flush(acc);
     for (int k = 0; k < F::kNacc; ++k) tot[k] = F::get(acc, k);
+}
+
+// A value every lane of the wave holds, moved to scalar registers.
+__device__ __forceinline__ float uniform_value(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ double uniform_value(double v)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = static_cast<unsigned>(
+        __builtin_amdgcn_readfirstlane(static_cast<int>(static_cast<unsigned>(b))));
+    const unsigned hi = static_cast<unsigned>(
+        __builtin_amdgcn_readfirstlane(static_cast<int>(static_cast<unsigned>(b >> 32))));
+    return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
 }
 
 // One pair of a tile pass: uniform (kU) or not, masked (kM) or not.
@@ -198,7 +214,7 @@ __device__ __forceinline__ void pair_body(const typename Op::Args& a, int bid)
     bool iuni = true;
     [[maybe_unused]] typename SoftType<Op>::type e2_first{};
     [[maybe_unused]] bool unmask = false;
-    [[maybe_unused]] float y_self = 0.0f;
+    [[maybe_unused]] typename SoftType<Op>::type y_self{};
     if constexpr (kUni) {
         e2_first = Op::e2_of_i(a, ibase);
 #pragma unroll
@@ -209,11 +225,9 @@ __device__ __forceinline__ void pair_body(const typename Op::Args& a, int bid)
     }
     if constexpr (kUnmask) unmask = Op::unmask_allowed(a);
     if constexpr (kSelf) {
-        // the same for every thread: held in a scalar register, so the pair
+        // the same for every thread: held in scalar registers, so the pair
         // loop keeps its vector registers
-        y_self = __builtin_bit_cast(
-            float, __builtin_amdgcn_readfirstlane(
-                       __builtin_bit_cast(int, Op::self_y(e2_first + e2_first))));
+        y_self = uniform_value(Op::self_y(e2_first + e2_first));
     }
 
     const int j0 = g.jsplit * a.jchunk;

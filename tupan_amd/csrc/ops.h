@@ -221,8 +221,25 @@ struct OpAcc {
 #pragma unroll
         for (int f = 0; f < 5; ++f) s.v[f] = a.jp[f][j];
     }
+    // Unmasked uniform tiles (the FP64 fast path): a pair at zero
+    // separation has r2 = 0 and R^2 = e2u exactly, and adds -q3 * 0 = 0
+    // when its q3 = m_j / e2u^1.5 is finite (self_safe, per tile), so the
+    // tile runs without the r2 > 0 test and gives the masked sums.
+    static constexpr bool kUnmask = std::is_same<P, Fast64>::value;
+    __device__ static bool unmask_allowed(const Args&) { return true; }
+    __device__ static T self_y(T e2u)
+    {
+        T q1, q2;
+        P::q12(T(0), e2u, true, q1, q2);
+        return q1;
+    }
+    __device__ static bool self_safe(T y0, const Slot& s)
+    {
+        const T q3 = ((y0 * y0) * y0) * s.v[0];
+        return __builtin_isfinite(q3);
+    }
     __device__ static void zero(Acc& c) { c.ax = c.ay = c.az = 0; }
-    template <bool kU = false>
+    template <bool kU = false, bool kM = true>
     __device__ static __forceinline__ void interact(const IReg& r, const Slot& s, Acc& c,
                                                     T e2u = T(0))
     {
@@ -232,7 +249,7 @@ struct OpAcc {
         T e2 = kU ? e2u : r.e2 + s.v[4];
         T r2 = dot3<P>(dx, dy, dz, dx, dy, dz);
         T q1, q2;
-        P::q12(r2, e2, r2 > T(0), q1, q2);
+        P::q12(r2, e2, (kM || !kU) ? r2 > T(0) : true, q1, q2);
         T q3 = q2 * q1;
         q3 *= s.v[0];
         c.ax = nmad<P>(q3, dx, c.ax);
@@ -285,8 +302,26 @@ struct OpAccJerk {
 #pragma unroll
         for (int f = 0; f < 8; ++f) s.v[f] = a.jp[f][j];
     }
+    // Unmasked uniform tiles (the FP64 fast path), as OpSplitAccJerkT's: the
+    // call's coincidence scan (pairwise.hip) clears the flag only when no two
+    // particles share a position with different velocities and every j's
+    // zero-separation term is finite.
+    static constexpr bool kCoincideScan = std::is_same<P, Fast64>::value;
+    static constexpr bool kUnmask = kCoincideScan;
+    __device__ static bool unmask_allowed(const Args& a)
+    {
+        return a.coincide != nullptr && *a.coincide == 0;
+    }
+    __device__ static bool self_safe_j(const Args& a, int j)
+    {
+        const T e2 = a.jp[4][j];
+        T q1, q2;
+        P::q12(T(0), e2 + e2, true, q1, q2);
+        const T q3 = (q2 * q1) * a.jp[0][j];
+        return __builtin_isfinite(q2) && __builtin_isfinite(q3);
+    }
     __device__ static void zero(Acc& c) { c.ax = c.ay = c.az = c.jx = c.jy = c.jz = 0; }
-    template <bool kU = false>
+    template <bool kU = false, bool kM = true>
     __device__ static __forceinline__ void interact(const IReg& r, const Slot& s, Acc& c,
                                                     T e2u = T(0))
     {
@@ -300,7 +335,7 @@ struct OpAccJerk {
         T r2 = dot3<P>(dx, dy, dz, dx, dy, dz);
         T rv = dot3<P>(dx, dy, dz, ux, uy, uz);
         T q1, q2;
-        P::q12(r2, e2, r2 > T(0), q1, q2);
+        P::q12(r2, e2, (kM || !kU) ? r2 > T(0) : true, q1, q2);
         T q3 = q2 * q1;
         T alpha = T(3) * rv * q2;
         ux = nmad<P>(alpha, dx, ux);
@@ -619,6 +654,7 @@ struct OpSplitAccT {
     // instructions fewer per pair.  (A pair with 0 < r^2 below half an FP32
     // ulp of e2, which the mask also drops, then contributes, as in the
     // reference.)
+    static constexpr bool kUnmask = true;
     __device__ static bool unmask_allowed(const Args&) { return true; }
     __device__ static float self_y(float e2u) { return split_self_rsqrt<kRefine>(e2u); }
     __device__ static bool self_safe(float y0, const Slot& s)
@@ -720,6 +756,7 @@ struct OpSplitAccJerkT {
     // pair exists and every j passes self_safe_j, and then every uniform
     // tile goes unmasked.
     static constexpr bool kCoincideScan = true;
+    static constexpr bool kUnmask = true;
     __device__ static bool unmask_allowed(const Args& a)
     {
         return a.coincide != nullptr && *a.coincide == 0;
