@@ -1,6 +1,7 @@
-"""Unmasked uniform-softening tiles of the split-FP32 Acc and AccJerk
-(ops.h: self_safe, unmask_allowed; pairwise.hip: the coincidence scan), on
-Plummer N = 16384 (eps = 1e-4, 2^28 pairs: the AccJerk call runs the scan).
+"""Unmasked uniform-softening tiles of the Acc and AccJerk, split-FP32 and
+FP64 (ops.h: self_safe, unmask_allowed; pairwise.hip: the coincidence
+scan), on Plummer N = 16384 (eps = 1e-4, 2^28 pairs: the AccJerk call runs
+the scan).
 
   * On a Plummer sphere the unmasked sums equal the masked ones bit for bit:
     tupan_pairwise (scan, unmasked tiles) against tupan_pairwise_noalloc
@@ -46,12 +47,16 @@ def place_pair(ps, dx):
     torch.cuda.synchronize()
 
 
-def split_rows(ps, kernel, rows, iset="same"):
-    """Rows of the split call with the system as i- and j-set ("same": one
-    set, the scan's insertion alone), or a copy as i-set ("copy": the scan
+# tolerances against the oracle's rows: split-FP32 error, FP64 rounding
+TOL = {"split32": (1e-5, 1e-4), "fp64": (1e-12, 1e-11)}
+
+
+def split_rows(ps, kernel, rows, iset="same", mode="split32"):
+    """Rows of the call with the system as i- and j-set ("same": one set,
+    the scan's insertion alone), or a copy as i-set ("copy": the scan
     probes the j table with every i)."""
     from tupan_amd.lib import extensions
-    k = getattr(extensions, kernel)("HIP", "float64", mode="split32")
+    k = getattr(extensions, kernel)("HIP", "float64", mode=mode)
     ips = ps if iset == "same" else ps.copy()
     res = k.calc(ips, ps)
     return [r.detach().cpu().numpy()[rows] for r in res]
@@ -65,7 +70,8 @@ def oracle_rows(ps, kernel, rows):
     return fn({k: v[rows] for k, v in host.items()}, host)
 
 
-def test_unmasked_equals_masked_on_plummer():
+@pytest.mark.parametrize("mode", ["split32", "fp64"])
+def test_unmasked_equals_masked_on_plummer(mode):
     from tupan_amd import _native
     ps = plummer()
     fields = [ps.mass, ps.rx, ps.ry, ps.rz, ps.eps2, ps.vx, ps.vy, ps.vz]
@@ -73,11 +79,11 @@ def test_unmasked_equals_masked_on_plummer():
     lib = _native.lib()
     outs = {}
     for name in ("tupan_pairwise", "tupan_pairwise_noalloc"):
-        print("split AccJerk N = %d through %s" % (N, name), flush=True)
+        print("%s AccJerk N = %d through %s" % (mode, N, name), flush=True)
         out = [torch.full((N,), np.nan, dtype=torch.float64, device="cuda:0")
                for _ in range(6)]
         op = _native.ptr_array([o.data_ptr() for o in out])
-        rc = getattr(lib, name)(2, _native.MODES["split32"], N, ip, N, ip,
+        rc = getattr(lib, name)(2, _native.MODES[mode], N, ip, N, ip,
                                 op, 0.0, None)
         _native.check(rc, name)
         torch.cuda.synchronize()
@@ -87,33 +93,39 @@ def test_unmasked_equals_masked_on_plummer():
         assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("kernel,iset", [("AccJerk", "same"),
-                                         ("AccJerk", "copy"),
-                                         ("Acc", "same")])
-def test_tiny_separation_pair_is_kept(kernel, iset):
+@pytest.mark.parametrize("kernel,iset,mode", [("AccJerk", "same", "split32"),
+                                              ("AccJerk", "copy", "split32"),
+                                              ("Acc", "same", "split32"),
+                                              ("AccJerk", "same", "fp64"),
+                                              ("Acc", "same", "fp64")])
+def test_tiny_separation_pair_is_kept(kernel, iset, mode):
     ps = plummer()
     place_pair(ps, 1e-9)
-    print("split %s, pair 1e-9 apart, N = %d, i-set %s" % (kernel, N, iset),
+    print("%s %s, pair 1e-9 apart, N = %d, i-set %s" % (mode, kernel, N, iset),
           flush=True)
     rows = np.array([0, 1, 2])
-    got = split_rows(ps, kernel, rows, iset)
+    got = split_rows(ps, kernel, rows, iset, mode)
     ref = oracle_rows(ps, kernel, rows)
-    assert vec_rel_err(got[:3], ref[:3]) <= 1e-5
+    tol = TOL[mode][0]
+    assert vec_rel_err(got[:3], ref[:3]) <= tol
     if kernel == "AccJerk":
-        assert vec_rel_err(got[3:], ref[3:]) <= 1e-5
+        assert vec_rel_err(got[3:], ref[3:]) <= tol
         # the pair's m |dv| / eps^3 term (~1.5e7) dominates the jerks
         assert np.hypot(ref[3][0], ref[4][0]) > 1e6
 
 
-@pytest.mark.parametrize("iset", ["same", "copy"])
-def test_coincident_pair_keeps_the_mask(iset):
+@pytest.mark.parametrize("iset,mode", [("same", "split32"),
+                                       ("copy", "split32"),
+                                       ("same", "fp64"),
+                                       ("copy", "fp64")])
+def test_coincident_pair_keeps_the_mask(iset, mode):
     ps = plummer()
     place_pair(ps, 0.0)
-    print("split AccJerk, coincident pair, N = %d, i-set %s" % (N, iset),
+    print("%s AccJerk, coincident pair, N = %d, i-set %s" % (mode, N, iset),
           flush=True)
     rows = np.array([0, 1, 2])
-    got = split_rows(ps, "AccJerk", rows, iset)
+    got = split_rows(ps, "AccJerk", rows, iset, mode)
     ref = oracle_rows(ps, "AccJerk", rows)
     assert np.all(np.abs(np.stack(ref[3:])) < 1e4)
-    assert vec_rel_err(got[:3], ref[:3]) <= 1e-5
-    assert vec_rel_err(got[3:], ref[3:]) <= 1e-4
+    assert vec_rel_err(got[:3], ref[:3]) <= TOL[mode][0]
+    assert vec_rel_err(got[3:], ref[3:]) <= TOL[mode][1]
