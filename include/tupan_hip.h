@@ -24,7 +24,13 @@
  *
  * Output arrays are overwritten, never accumulated into.  Kernels keep no
  * pointer past the call.  The device layer uses a per-device scratch
- * workspace for j-split partial sums, so calls on one device must not
+ * workspace for j-split partial sums (and, for Acc-Jerk calls of at least
+ * 2^28 pairs in the fp64 and split32 modes, the hash table of a scan for
+ * particles that share a position: it decides whether the pair loop may
+ * leave out the test that drops zero-separation pairs; the sums are the
+ * same either way, but for split32 pairs closer than half an FP32 ulp of
+ * their softening, which then count as in the reference), so calls on one
+ * device must not
  * overlap in time on different streams.  A workspace that a larger call
  * outgrows is replaced by one at least twice its size and kept allocated
  * until the process ends, so HIP graphs captured earlier stay valid.
