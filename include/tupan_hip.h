@@ -352,8 +352,11 @@ int tupan_release_workspaces(void);
  * in time on one device (different streams, threads, or replays of graphs
  * that hold this call) race on it and give undefined results.  The array
  * needs no clearing between calls: every partial a call combines is
- * written by that call first.  TUPAN_K_PNACC returns TUPAN_ERR_ARG, as for
- * tupan_pairwise. */
+ * written by that call first.  The coincidence scan of large Acc-Jerk
+ * calls is left out, so their tiles keep the zero-separation test: the same
+ * sums, but for split32 pairs closer than half an FP32 ulp of their
+ * softening, which tupan_pairwise then counts.  TUPAN_K_PNACC returns
+ * TUPAN_ERR_ARG, as for tupan_pairwise. */
 #define TUPAN_NOALLOC_SCRATCH_BYTES (64u * 1024u * 1024u)
 int tupan_pairwise_noalloc(int kernel, int mode, size_t ni,
                            const double *const *ifields, size_t nj,
