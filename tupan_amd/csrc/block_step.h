@@ -32,10 +32,11 @@ __device__ __forceinline__ double min_nan(double a, double b)
     return (a != a || b != b) ? __builtin_nan("") : fmin(a, b);
 }
 
-__global__ __launch_bounds__(kTauThreads) void block_tau_kernel(int n, const double* ts,
-                                                                   int adaptive, double* ctl)
+// The work of block_tau_kernel for a workgroup of kTauThreads threads with
+// `red` its shared array; thread 0 writes tau (and the status) to ctl.
+__device__ __forceinline__ void block_tau_run(int n, const double* ts, int adaptive, double* ctl,
+                                              double* red)
 {
-    __shared__ double red[kTauThreads];
     const int tid = static_cast<int>(threadIdx.x);
     double lo = DBL_MAX;
     if (adaptive)
@@ -93,6 +94,13 @@ __global__ __launch_bounds__(kTauThreads) void block_tau_kernel(int n, const dou
         tau = bts;
     }
     ctl[kCtlTau] = tau;
+}
+
+__global__ __launch_bounds__(kTauThreads) void block_tau_kernel(int n, const double* ts,
+                                                                   int adaptive, double* ctl)
+{
+    __shared__ double red[kTauThreads];
+    block_tau_run(n, ts, adaptive, ctl, red);
 }
 
 }  // namespace

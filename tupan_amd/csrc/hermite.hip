@@ -240,6 +240,84 @@ struct OpHermite4 : Base {
     }
 };
 
+// Tstep and AccJerk of one state in one pass of the small-set kernel (FP64
+// fast path): the two interactions read the same slot (m rx ry rz e2 vx vy
+// vz), and each keeps its own sums, so both results equal those of the two
+// separate calls bit for bit (the same pairs per lane in the same order,
+// the same combine tree).  out[0..5] take a, j and out[6], out[7] the two
+// Tstep outputs.
+struct OpAccJerkTstep {
+    using AJ = OpAccJerk<Fast64>;
+    using TS = OpTstep<Fast64>;
+    using Args = PairArgs<double>;
+    using Frame = NoFrame;
+    using Slot = AJ::Slot;
+    static_assert(std::is_same<Slot, TS::Slot>::value, "one slot serves both interactions");
+    struct IReg { AJ::IReg a; TS::IReg t; };
+    struct Acc { AJ::Acc a; TS::Acc t; };
+    static constexpr int kNacc = AJ::kNacc + TS::kNacc;
+    static constexpr int kFlush = 1 << 30;
+    __device__ static Frame frame(const Args&, int) { return {}; }
+    __device__ static void load_i(const Args& a, const Frame& f, int i, IReg& r)
+    {
+        AJ::load_i(a, f, i, r.a);
+        TS::load_i(a, f, i, r.t);
+    }
+    __device__ static void load_j(const Args& a, const Frame& f, int j, Slot& s)
+    {
+        AJ::load_j(a, f, j, s);
+    }
+    __device__ static void zero(Acc& c)
+    {
+        AJ::zero(c.a);
+        TS::zero(c.t);
+    }
+    __device__ static __forceinline__ void interact(const IReg& r, const Slot& s, Acc& c)
+    {
+        AJ::interact(r.a, s, c.a);
+        TS::interact(r.t, s, c.t);
+    }
+    __device__ static void flush(Acc&) {}
+    __device__ static double get(const Acc& c, int k)
+    {
+        return k < AJ::kNacc ? AJ::get(c.a, k) : TS::get(c.t, k - AJ::kNacc);
+    }
+    __device__ static void combine(double* t, const double* p)
+    {
+        AJ::combine(t, p);
+        TS::combine(t + AJ::kNacc, p + AJ::kNacc);
+    }
+    __device__ static void store(const Args& a, int i, const double* t)
+    {
+        AJ::store(a, i, t);
+        Args b = a;
+        b.out[0] = a.out[6];
+        b.out[1] = a.out[7];
+        TS::store(b, i, t + AJ::kNacc);
+    }
+};
+
+// The step size (block_tau_kernel's work) and then the order-4 predictor
+// of every particle into the r0, v0 scratch from the start forces a0, j0,
+// in one workgroup: the predictor that stage 0 runs in its epilogue, for a
+// step whose start forces came with Tstep in one pass.
+__global__ __launch_bounds__(kTauThreads) void tau_predict4_kernel(int n, HermiteArrays h,
+                                                                   int adaptive, double* ctl)
+{
+    __shared__ double red[kTauThreads];
+    __shared__ double tau_sh;
+    block_tau_run(n, h.ts, adaptive, ctl, red);
+    if (threadIdx.x == 0) tau_sh = ctl[kCtlTau];
+    __syncthreads();
+    const double tau = tau_sh;
+    if (tau == 0) return;
+    for (int i = static_cast<int>(threadIdx.x); i < n; i += kTauThreads) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+            predict_dim<4>(h, i, d, tau, h.a0[d][i], h.j0[d][i], nullptr, nullptr, h.r0, h.v0);
+    }
+}
+
 // Whether an order-4 step of n particles in `mode` takes the fused launches:
 // the small-set kernel's range (as pairwise_call decides it for AccJerk).
 bool fused4(int mode, int n)
@@ -289,6 +367,42 @@ int fused_step4(int n, const HermiteArrays& h, double* ctl, int adaptive, hipStr
     return rc;
 }
 
+// Adaptive FP64 order-4 step of a small set in four launches: Tstep and the
+// start forces in one pass, the step size and the predictor, then stages 1
+// and 2 (the same operations as the five-launch form, the same results).
+int fused_adaptive_step4(int n, const HermiteArrays& h, double eta, double* ctl, hipStream_t s)
+{
+    using Op = OpAccJerkTstep;
+    Op::Args a{};
+    const double* in[8] = {h.m, h.r[0], h.r[1], h.r[2], h.e2, h.v[0], h.v[1], h.v[2]};
+    for (int f = 0; f < 8; ++f) {
+        a.ip[f] = in[f];
+        a.jp[f] = in[f];
+    }
+    for (int k = 0; k < 3; ++k) {
+        a.out[k] = h.a0[k];
+        a.out[3 + k] = h.j0[k];
+    }
+    a.out[6] = h.ts;
+    a.out[7] = h.tsij;
+    a.ni = n;
+    a.nj = n;
+    a.nsplit = 1;
+    a.jchunk = n;
+    a.eta = eta;
+    constexpr int ipb = kSmallBlock / kSmallGroups;
+    const unsigned grid = static_cast<unsigned>((n + ipb - 1) / ipb);
+    hipLaunchKernelGGL((pair_small_kernel<Op, kSmallBlock, kSmallGroups, small_tile<Op>()>),
+                       dim3(grid), dim3(kSmallBlock), 0, s, a);
+    int rc = check_launch("hermite start forces and Tstep");
+    if (rc != TUPAN_OK) return rc;
+    hipLaunchKernelGGL(tau_predict4_kernel, dim3(1), dim3(kTauThreads), 0, s, n, h, 1, ctl);
+    rc = check_launch("hermite tau and predictor");
+    if (rc == TUPAN_OK) rc = force_stage4<OpAccJerk<Fast64>, 1>(n, h, ctl, 1, s);
+    if (rc == TUPAN_OK) rc = force_stage4<OpAccJerk<Fast64>, 2>(n, h, ctl, 1, s);
+    return rc;
+}
+
 int forces(int order, int mode, size_t n, const HermiteArrays& h, bool start, hipStream_t s)
 {
     // AccJerk: i and j fields m rx ry rz e2 vx vy vz
@@ -316,6 +430,10 @@ int step_order(int adaptive, int mode, int n, const HermiteArrays& h, double eta
                hipStream_t s)
 {
     const unsigned grid = static_cast<unsigned>((n + 255) / 256);
+    if constexpr (Order == 4) {
+        if (adaptive && mode == TUPAN_MODE_FP64 && fused4(mode, n))
+            return fused_adaptive_step4(n, h, eta, ctl, s);
+    }
     if (adaptive) {
         const double* in[8] = {h.m, h.r[0], h.r[1], h.r[2], h.e2, h.v[0], h.v[1], h.v[2]};
         double* out[2] = {h.ts, h.tsij};
