@@ -3,9 +3,10 @@ against tupan_pairwise through the C ABI.
 
 Two calls in a row on each (kernel, mode), the second with a smaller input:
   - the first (N = 40000) needs more j-split partial sums than the fixed
-    64 MiB array holds for AccJerk (the tuned plan: 50 splits x 6 fields x
-    40000 doubles, 96 MB), so the no-alloc plan runs 34 j-splits: FP64
-    results within 1e-13 per particle (vector norm) of tupan_pairwise's;
+    64 MiB array holds for AccJerk (the tuned plans run 50 to 64 splits x 6
+    fields x 40000 doubles, 96 to 123 MB), so the no-alloc plan runs 34
+    j-splits: FP64 results within 1e-13 per particle (vector norm) of
+    tupan_pairwise's;
   - the second (N = 3000) fits: bit-identical in every mode.
 A 1024-particle call takes the one-launch small-set kernel, which uses no
 partial sums: bit-identical.  Arguments are checked before any device work.
