@@ -27,8 +27,7 @@ import torch
 
 from . import h5lib
 from ..lib.utils import ctype
-from ..particles.system import (ATTRS, MEMBER_EXTRAS, ParticleSystem,
-                                default_device)
+from ..particles.system import ATTRS, ParticleSystem, default_device
 
 __all__ = ["HDF5IO", "class_string", "parse_class_string"]
 
