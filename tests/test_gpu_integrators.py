@@ -239,6 +239,62 @@ def test_device_hermite4_fused_launches_match_host_driver(method, eta, steps,
                                    err_msg=k)
 
 
+@pytest.mark.parametrize("mode", ["fp64", "split32"])
+def test_device_hermite4_start_pass_equals_separate_calls(mode):
+    """The adaptive order-4 step of a small set computes Tstep and the start
+    forces in one pass (OpAccJerkTstep): its a0, j0 and Tstep outputs equal
+    the separate AccJerk and Tstep calls bit for bit, also where a lane's
+    sums of one of the two overflow FP32 (split32: unsoftened pairs 1e-15
+    apart, one approaching, one receding; only that interaction's sums are
+    evaluated again in FP64, as in its own kernel).  N = 600; a step with
+    t = t_end runs the pass and changes no particle."""
+    from tupan_amd.integrator import Integrator
+    from tupan_amd.lib import extensions
+    from tupan_amd.particles import ParticleSystem
+    rng = np.random.default_rng(21)
+    n = 600
+    d = {k: rng.random(n) for k in ("rx", "ry", "rz", "vx", "vy", "vz")}
+    d["mass"] = np.full(n, 1.0 / n) * (1 + rng.random(n))
+    d["eps2"] = np.zeros(n)
+    x0 = (d["rx"][0], d["ry"][0], d["rz"][0])
+    d["rx"][1] = x0[0] + 1e-15
+    d["ry"][1], d["rz"][1] = x0[1], x0[2]
+    d["rx"][2], d["ry"][2], d["rz"][2] = x0[0], x0[1] + 1e-6, x0[2]
+    d["rx"][3], d["ry"][3], d["rz"][3] = x0[0] + 1e-15, x0[1] + 1e-6, x0[2]
+    d["vx"][0], d["vx"][1] = 1.0, -1.0
+    d["vx"][2], d["vx"][3] = -1.0, 1.0
+    eta = 0.01
+    print("%s start pass, N = 600 with close pairs" % mode, flush=True)
+    install_hip("float64", mode=mode)
+    try:
+        ps = ParticleSystem.from_numpy(d, device="cuda:0")
+        type(ps).t_curr = 0.0
+        aj = [r.detach().cpu().numpy().copy() for r in
+              extensions.AccJerk("HIP", "float64", mode=mode).calc(ps, ps)]
+        ts = [r.detach().cpu().numpy().copy() for r in
+              extensions.Tstep("HIP", "float64", mode=mode).calc(ps, ps, eta)]
+        it = Integrator(eta, 0.0, ps, method="ahermite4", device_steps=True)
+        dev = it.integrator._device()
+        dev.scratch.zero_()
+        dev.set_control(0.0)
+        dev.launch()
+        torch.cuda.synchronize()
+        sc = dev.scratch.cpu().numpy()
+        q = it.particle_system.to_numpy()
+    finally:
+        install_hip("float64")
+    a0 = [sc[(6 + k) * n:(7 + k) * n] for k in range(3)]
+    j0 = [sc[(9 + k) * n:(10 + k) * n] for k in range(3)]
+    for k in range(3):
+        np.testing.assert_array_equal(a0[k], aj[k])
+        np.testing.assert_array_equal(j0[k], aj[3 + k])
+    np.testing.assert_array_equal(sc[18 * n:19 * n], ts[0])
+    np.testing.assert_array_equal(sc[19 * n:20 * n], ts[1])
+    assert np.all(np.isfinite(np.stack(aj))) and np.all(np.isfinite(ts[0]))
+    for k in ("rx", "vx", "time"):
+        np.testing.assert_array_equal(q[k], d[k] if k in d else 0 * q[k])
+
+
 @pytest.mark.parametrize("n", [256, 1024])
 def test_device_hermite_graph_replays_equal_eager_steps(n):
     """evolve() replaying a graph of 16 device steps equals the same steps

@@ -82,6 +82,13 @@ template <class Op, class = void>
 struct HasPostFallback : std::false_type {};
 template <class Op>
 struct HasPostFallback<Op, std::void_t<typename Op::PostFallback64>> : std::true_type {};
+// Ops that combine two interactions check each one's sums and evaluate
+// only a non-finite part again (fallback_parts(a, i, j0, j1, jstep, tot)),
+// as the two separate calls would.
+template <class Op, class = void>
+struct HasPartFallback : std::false_type {};
+template <class Op>
+struct HasPartFallback<Op, std::void_t<decltype(&Op::fallback_parts)>> : std::true_type {};
 // Ops whose per-particle store is spread over the lanes of the small-set
 // kernel's group (store_group(a, i, totals, lane)) instead of one lane.
 template <class Op, class = void>
@@ -328,7 +335,10 @@ __device__ __forceinline__ void pair_small_body(const typename Op::Args& a, int 
     double tot[Op::kNacc];
 #pragma unroll
     for (int k = 0; k < Op::kNacc; ++k) tot[k] = Op::get(acc, k);
-    if constexpr (HasFallback<Op>::value) {
+    if constexpr (HasPartFallback<Op>::value) {
+        // lane g took the j with j % G == g (TILE is a multiple of G)
+        if (i < a.ni) Op::fallback_parts(a, i, g, a.nj, G, tot);
+    } else if constexpr (HasFallback<Op>::value) {
         // lane g took the j with j % G == g (TILE is a multiple of G)
         if (i < a.ni && !all_finite(tot))
             fallback64<typename Op::Fallback64>(a, i, g, a.nj, G, tot);

@@ -240,24 +240,27 @@ struct OpHermite4 : Base {
     }
 };
 
-// Tstep and AccJerk of one state in one pass of the small-set kernel (FP64
-// fast path): the two interactions read the same slot (m rx ry rz e2 vx vy
-// vz), and each keeps its own sums, so both results equal those of the two
-// separate calls bit for bit (the same pairs per lane in the same order,
-// the same combine tree).  out[0..5] take a, j and out[6], out[7] the two
-// Tstep outputs.
+// Tstep and AccJerk of one state in one pass of the small-set kernel: the
+// two interactions read the same slot (m rx ry rz e2 vx vy vz; the split
+// Tstep, which holds doubled masses, gets a copy with 2 m, exact), and each
+// keeps its own sums, flushed at the same points, so both results equal
+// those of the two separate calls bit for bit (the same pairs per lane in
+// the same order, the same combine tree, and, in split32, the FP64
+// evaluation of a lane's sums only for the interaction whose sums are not
+// finite).  out[0..5] take a, j and out[6], out[7] the two Tstep outputs.
+template <class AJ, class TS>
 struct OpAccJerkTstep {
-    using AJ = OpAccJerk<Fast64>;
-    using TS = OpTstep<Fast64>;
     using Args = PairArgs<double>;
-    using Frame = NoFrame;
-    using Slot = AJ::Slot;
-    static_assert(std::is_same<Slot, TS::Slot>::value, "one slot serves both interactions");
-    struct IReg { AJ::IReg a; TS::IReg t; };
-    struct Acc { AJ::Acc a; TS::Acc t; };
+    using Frame = typename AJ::Frame;
+    using Slot = typename AJ::Slot;
+    static constexpr bool kSplit = std::is_same<AJ, OpSplitAccJerk>::value;
+    static_assert(std::is_same<Slot, typename TS::Slot>::value,
+                  "one slot layout serves both interactions");
+    struct IReg { typename AJ::IReg a; typename TS::IReg t; };
+    struct Acc { typename AJ::Acc a; typename TS::Acc t; };
     static constexpr int kNacc = AJ::kNacc + TS::kNacc;
-    static constexpr int kFlush = 1 << 30;
-    __device__ static Frame frame(const Args&, int) { return {}; }
+    static constexpr int kFlush = AJ::kFlush < TS::kFlush ? AJ::kFlush : TS::kFlush;
+    __device__ static Frame frame(const Args& a, int ibase) { return AJ::frame(a, ibase); }
     __device__ static void load_i(const Args& a, const Frame& f, int i, IReg& r)
     {
         AJ::load_i(a, f, i, r.a);
@@ -275,9 +278,33 @@ struct OpAccJerkTstep {
     __device__ static __forceinline__ void interact(const IReg& r, const Slot& s, Acc& c)
     {
         AJ::interact(r.a, s, c.a);
-        TS::interact(r.t, s, c.t);
+        if constexpr (kSplit) {
+            Slot s2 = s;
+            s2.v[3] *= 2.0f;
+            TS::interact(r.t, s2, c.t);
+        } else {
+            TS::interact(r.t, s, c.t);
+        }
     }
-    __device__ static void flush(Acc&) {}
+    __device__ static void flush(Acc& c)
+    {
+        AJ::flush(c.a);
+        TS::flush(c.t);
+    }
+    // split32: each interaction's lane sums evaluated again in FP64 only if
+    // they are not finite, as in its own kernel
+    __device__ static void fallback_parts(const Args& a, int i, int j0, int j1, int jstep,
+                                          double* tot)
+    {
+        if constexpr (kSplit) {
+            double t1[AJ::kNacc], t2[TS::kNacc];
+            for (int k = 0; k < AJ::kNacc; ++k) t1[k] = tot[k];
+            for (int k = 0; k < TS::kNacc; ++k) t2[k] = tot[AJ::kNacc + k];
+            if (!all_finite(t1)) fallback64<typename AJ::Fallback64>(a, i, j0, j1, jstep, tot);
+            if (!all_finite(t2))
+                fallback64<typename TS::Fallback64>(a, i, j0, j1, jstep, tot + AJ::kNacc);
+        }
+    }
     __device__ static double get(const Acc& c, int k)
     {
         return k < AJ::kNacc ? AJ::get(c.a, k) : TS::get(c.t, k - AJ::kNacc);
@@ -367,13 +394,14 @@ int fused_step4(int n, const HermiteArrays& h, double* ctl, int adaptive, hipStr
     return rc;
 }
 
-// Adaptive FP64 order-4 step of a small set in four launches: Tstep and the
+// Adaptive order-4 step of a small set in four launches: Tstep and the
 // start forces in one pass, the step size and the predictor, then stages 1
 // and 2 (the same operations as the five-launch form, the same results).
+template <class AJ, class TS>
 int fused_adaptive_step4(int n, const HermiteArrays& h, double eta, double* ctl, hipStream_t s)
 {
-    using Op = OpAccJerkTstep;
-    Op::Args a{};
+    using Op = OpAccJerkTstep<AJ, TS>;
+    typename Op::Args a{};
     const double* in[8] = {h.m, h.r[0], h.r[1], h.r[2], h.e2, h.v[0], h.v[1], h.v[2]};
     for (int f = 0; f < 8; ++f) {
         a.ip[f] = in[f];
@@ -398,8 +426,8 @@ int fused_adaptive_step4(int n, const HermiteArrays& h, double eta, double* ctl,
     if (rc != TUPAN_OK) return rc;
     hipLaunchKernelGGL(tau_predict4_kernel, dim3(1), dim3(kTauThreads), 0, s, n, h, 1, ctl);
     rc = check_launch("hermite tau and predictor");
-    if (rc == TUPAN_OK) rc = force_stage4<OpAccJerk<Fast64>, 1>(n, h, ctl, 1, s);
-    if (rc == TUPAN_OK) rc = force_stage4<OpAccJerk<Fast64>, 2>(n, h, ctl, 1, s);
+    if (rc == TUPAN_OK) rc = force_stage4<AJ, 1>(n, h, ctl, 1, s);
+    if (rc == TUPAN_OK) rc = force_stage4<AJ, 2>(n, h, ctl, 1, s);
     return rc;
 }
 
@@ -431,8 +459,12 @@ int step_order(int adaptive, int mode, int n, const HermiteArrays& h, double eta
 {
     const unsigned grid = static_cast<unsigned>((n + 255) / 256);
     if constexpr (Order == 4) {
-        if (adaptive && mode == TUPAN_MODE_FP64 && fused4(mode, n))
-            return fused_adaptive_step4(n, h, eta, ctl, s);
+        if (adaptive && fused4(mode, n)) {
+            return mode == TUPAN_MODE_SPLIT32
+                       ? fused_adaptive_step4<OpSplitAccJerk, OpSplitTstep>(n, h, eta, ctl, s)
+                       : fused_adaptive_step4<OpAccJerk<Fast64>, OpTstep<Fast64>>(n, h, eta,
+                                                                                  ctl, s);
+        }
     }
     if (adaptive) {
         const double* in[8] = {h.m, h.r[0], h.r[1], h.r[2], h.e2, h.v[0], h.v[1], h.v[2]};
