@@ -112,7 +112,7 @@ def parse():
                     help="force modes to time (the first is the headline)")
     ap.add_argument("--skip", default="",
                     help="comma-separated parts to leave out: sakura_steps, "
-                         "ensemble_graph, force_error")
+                         "ensemble_graph, ensemble_learned, force_error")
     return ap.parse_args()
 
 
@@ -424,13 +424,16 @@ def sakura_forms(n, net_path, reps=3):
     return out
 
 
-def ensemble_rate(nsys, lanes, graph=True):
+def ensemble_rate(nsys, lanes, graph=True, net_path=None):
     """The ensemble workload: nsys Plummer systems with sizes U[3,64] and
     log-uniform masses in [0.1, 1], Sakura with dt = 1/64 to t = 1, every
     pair-drift stage one launch over all systems.  Timed twice from the same
     initial state: eager launches (kernel time from HIP events, counters)
     and replays of one captured 8-step HIP graph (wall time; the final
-    states are compared)."""
+    states are compared).  With `net_path`, the eager run again from the
+    same state with the learned Kepler guess (kernel time, Laguerre steps,
+    fallbacks, and the final states' largest difference from the plain
+    run's, per particle relative to its vector)."""
     import torch
     from tupan_amd import _native
     from tupan_amd.ics.ensemble import make_ensemble
@@ -443,6 +446,7 @@ def ensemble_rate(nsys, lanes, graph=True):
                             device="cuda:0", min_lanes=lanes)
     t_ic = time.perf_counter() - t0
     ps_init = ps.copy()
+    ps_learned = ps.copy() if net_path and os.path.exists(net_path) else None
     drv = SakuraEnsemble(ps, ens)
     e0 = drv.energies()
     # eager launches: kernel time (HIP events) and work counters
@@ -472,6 +476,36 @@ def ensemble_rate(nsys, lanes, graph=True):
         ens.check_status()
         same = all(torch.equal(getattr(ps, k), getattr(ps_init, k))
                    for k in ("rx", "ry", "rz", "vx", "vy", "vz"))
+    learned = None
+    if ps_learned is not None:
+        from tupan_amd.learned import KeplerNet
+        ens.set_guess(KeplerNet.load(net_path, device="cuda:0"))
+        ldrv = SakuraEnsemble(ps_learned, ens)
+        ens.counters = torch.zeros(4, dtype=torch.int64, device="cuda:0")
+        torch.cuda.synchronize()
+        _native.timing(True)
+        ldrv.evolve(1.0, 1.0 / 64)
+        torch.cuda.synchronize()
+        lkms, _ = _native.timing_read()
+        _native.timing(False)
+        lc = [int(x) for x in ens.counters.cpu().tolist()]
+        ens.counters = None
+        fb = int(ens.fallbacks.item())
+        ens.set_guess(None)
+        diff = 0.0
+        for group in (("rx", "ry", "rz"), ("vx", "vy", "vz")):
+            a = torch.stack([getattr(ps_learned, k) for k in group])
+            b = torch.stack([getattr(ps, k) for k in group])
+            rel = (a - b).norm(dim=0) / b.norm(dim=0).clamp_min(1e-300)
+            diff = max(diff, float(rel.max()))
+        learned = {"kernel_s": lkms / 1e3,
+                   "kepler_solves": lc[0], "laguerre_iterations": lc[1],
+                   "subdivisions": lc[2], "leapfrog_pairs": lc[3],
+                   "iterations_per_solve": lc[1] / max(lc[0], 1),
+                   "guess_fallbacks": fb,
+                   "speedup_vs_plain": kms / lkms,
+                   "final_state_max_rel_diff": diff,
+                   "network": os.path.basename(net_path)}
     solves, iters, retries, leap = [int(x) for x in counts]
     sizes = np.diff(ens.offsets)
     pairs = int(np.sum(sizes.astype(np.int64)**2))
@@ -501,6 +535,7 @@ def ensemble_rate(nsys, lanes, graph=True):
             "iterations_per_solve": iters / max(solves, 1),
             "iterations_per_s": iters / (kms / 1e3),
             "roofline": roofline,
+            "learned_guess": learned,
             "dE_E_quantiles": energy_quantiles(drv, e0)}
 
 
@@ -1006,7 +1041,8 @@ def main():
         log("bench: Sakura ensemble of %d systems" % args.ensemble_systems)
         out["sakura_ensemble"] = ensemble_rate(
             args.ensemble_systems, args.ensemble_lanes,
-            graph="ensemble_graph" not in skip)
+            graph="ensemble_graph" not in skip,
+            net_path=None if "ensemble_learned" in skip else args.learned_net)
 
     if args.sia_n > 0:
         log("bench: block steps N=%d" % args.sia_n)

@@ -605,6 +605,17 @@ int tupan_ensemble_create(int nsys, const int *offsets, int min_lanes,
 int tupan_ensemble_destroy(tupan_ensemble *plan);
 int tupan_ensemble_info(const tupan_ensemble *plan, int *nsys, int *ntot,
                         int *nwaves);
+/* Kepler-guess network for the plan's Sakura calls (no counterpart in the
+ * reference; the learned guess of tupan_sakura_ex applied to ensembles).
+ * net: packed network (tupan_mlp_pack; device, 16-byte aligned) or NULL for
+ * the plain guess.  With a network every j step of a wave evaluates it on
+ * the wave's 64 pairs and each pair's first solve starts from its guess
+ * (falling back to the plain guess after 8 Laguerre steps); converged
+ * states equal the plain solver's to its tolerance.  fallbacks: NULL or one
+ * device counter the calls add their guess fallbacks to.  Both pointers are
+ * kept by the plan and must outlive its calls. */
+int tupan_ensemble_set_guess(tupan_ensemble *plan, const void *net,
+                             unsigned long long *fallbacks);
 /* Sakura pair drift inside every system (flag and counters/status as for
  * tupan_sakura). */
 int tupan_ensemble_sakura(const tupan_ensemble *plan, const double *m,

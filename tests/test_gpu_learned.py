@@ -195,6 +195,72 @@ def test_learned_guess_gain_and_identical_results():
 
 
 @pytest.mark.parametrize("flag", [-1, 2])
+def test_ensemble_learned_guess_same_states(flag):
+    """The learned form of the ensemble kernel (tupan_ensemble_set_guess,
+    the shipped network) on the ensemble workload (4096 systems of
+    U{3..64} particles, log-uniform masses in [0.1, 1], eps = 0, seed 1),
+    one pair-drift stage at dt = 1/64.
+      * The same leapfrog pairs (every stage's far/near test agrees).  A
+        guess can spare a pair the subdivision its plain-guess solve needs
+        (at flag 2: 4 subdivisions plain, 2 learned), which changes the
+        solve count; with equal subdivisions the solves are equal.
+      * States after the drift within 1e-12 per particle.  Each solve stops
+        after a correction of at most 2^-42 (2.3e-13) of s, so solves from
+        different guesses converge that far apart, and in a few-body system
+        one pair dominates a particle's drift (measured: 3e-14 in positions,
+        1.3e-13 in velocities).  The increments dr = mu (r1 - r0) are
+        printed, not bounded: relative to themselves they carry the states'
+        difference amplified by |r| / |dr|, unbounded where a particle's
+        increments nearly cancel (2.4e-7 at flag 2).
+    Laguerre steps and fallbacks are printed."""
+    import os
+    from tupan_amd.ics.ensemble import make_ensemble
+    from tupan_amd.learned import KeplerNet
+    path = os.path.join(os.path.dirname(__file__), os.pardir, "tupan_amd",
+                        "learned", "kepler_net.npz")
+    net = KeplerNet.load(path, device="cuda:0")
+    ps, ens = make_ensemble(nsys=4096, nmin=3, nmax=64, seed=1,
+                            device="cuda:0")
+    print("ensemble 4096 systems, flag %d, plain then learned" % flag,
+          flush=True)
+    res = {}
+    for name, guess in (("plain", None), ("learned", net)):
+        ens.set_guess(guess)
+        ens.counters = torch.zeros(4, dtype=torch.int64, device="cuda:0")
+        out = [t.cpu().numpy().copy() for t in ens.sakura(ps, 1.0 / 64, flag)]
+        fb = int(ens.fallbacks.item()) if guess is not None else 0
+        res[name] = (out, ens.counters.cpu().tolist(), fb)
+    ens.set_guess(None)
+    ens.counters = None
+    (plain, cp, _), (learned, cl, fb) = res["plain"], res["learned"]
+    print("solves %d; Laguerre steps per solve: plain %.4f learned %.4f; "
+          "fallbacks %d" % (cp[0], cp[1] / cp[0], cl[1] / cl[0], fb))
+    print("counters plain", cp, "learned", cl)
+    r0 = [getattr(ps, k).cpu().numpy() for k in ("rx", "ry", "rz")]
+    v0 = [getattr(ps, k).cpu().numpy() for k in ("vx", "vy", "vz")]
+    errs = []
+    for base, k0 in ((r0, 0), (v0, 3)):
+        a = [b + d for b, d in zip(base, learned[k0:k0 + 3])]
+        b = [b + d for b, d in zip(base, plain[k0:k0 + 3])]
+        errs.append(vec_rel_err(a, b))
+    errs += [vec_rel_err(learned[:3], plain[:3]),
+             vec_rel_err(learned[3:], plain[3:])]
+    print("states %.2e %.2e increments %.2e %.2e" % tuple(errs))
+    for name, (g, r) in (("dr", (learned[:3], plain[:3])),
+                         ("dv", (learned[3:], plain[3:]))):
+        g, r = np.stack(g), np.stack(r)
+        e = np.sqrt(((g - r) ** 2).sum(0)) / np.maximum(
+            np.sqrt((r * r).sum(0)), 1e-300)
+        top = np.argsort(e)[-5:]
+        print(name, "largest", e[top].tolist(), "at", top.tolist(),
+              "n > 1e-9:", int((e > 1e-9).sum()))
+    assert cp[3] == cl[3]
+    if cp[2] == cl[2]:
+        assert cp[0] == cl[0]
+    assert max(errs[:2]) <= 1e-12
+
+
+@pytest.mark.parametrize("flag", [-1, 2])
 def test_graph_safe_two_pass_abi(plummer4k, flag):
     """tupan_sakura_graph (the two-pass form without its host read of the
     near-pair count) through the C ABI: with room for the list it equals

@@ -113,6 +113,8 @@ def _declare(lib):
     lib.tupan_ensemble_destroy.argtypes = [_P]
     lib.tupan_ensemble_info.restype = ctypes.c_int
     lib.tupan_ensemble_info.argtypes = [_P] + [ctypes.POINTER(ctypes.c_int)] * 3
+    lib.tupan_ensemble_set_guess.restype = ctypes.c_int
+    lib.tupan_ensemble_set_guess.argtypes = [_P, _P, _P]
     lib.tupan_ensemble_sakura.restype = ctypes.c_int
     lib.tupan_ensemble_sakura.argtypes = ([_P] + [_P] * 8
                                           + [ctypes.c_double, ctypes.c_long]
@@ -249,7 +251,8 @@ DEVICE_ABI = ("tupan_last_error", "tupan_version", "tupan_device_info",
               "tupan_sakura", "tupan_kepler_pairs", "tupan_host_pairwise_f32",
               "tupan_host_sakura_f32", "tupan_host_kepler_f32",
               "tupan_ensemble_create", "tupan_ensemble_destroy",
-              "tupan_ensemble_info", "tupan_ensemble_sakura",
+              "tupan_ensemble_info", "tupan_ensemble_set_guess",
+              "tupan_ensemble_sakura",
               "tupan_ensemble_phi", "tupan_sakura_ex", "tupan_mlp_sizes",
               "tupan_mlp_grad", "tupan_mlp_forward", "tupan_mlp_adam",
               "tupan_mlp_pack", "tupan_mlp_infer", "tupan_sia_subtree",

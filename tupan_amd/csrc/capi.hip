@@ -897,6 +897,11 @@ int tupan_ensemble_info(const tupan_ensemble* plan, int* nsys, int* ntot, int* n
     return ensemble_plan_info(plan, nsys, ntot, nwaves);
 }
 
+int tupan_ensemble_set_guess(tupan_ensemble* plan, const void* net, unsigned long long* fallbacks)
+{
+    return ensemble_plan_set_guess(plan, net, fallbacks);
+}
+
 int tupan_ensemble_sakura(const tupan_ensemble* plan, const double* m, const double* rx,
                           const double* ry, const double* rz, const double* e2,
                           const double* vx, const double* vy, const double* vz, double dt,

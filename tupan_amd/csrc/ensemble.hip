@@ -7,11 +7,19 @@
 // systems are sorted by size and each size bucket gets L lanes per system
 // (L in {8, 16, 32, 64}, at least the system size), so a wave holds 64/L
 // systems of similar size.  Lane l of a wave serves particle l % L of
-// system l / L and loops over that system's j particles, so the cost of
-// Kepler solves of uneven iteration counts is summed per lane over j rather
-// than paid at the slowest pair of a wave.  Every lane stays in the wave to
-// the end (inactive lanes compute nothing), so the wave reductions of the
-// work counters see all 64 lanes.
+// system l / L and loops over that system's j particles.  The lanes of a
+// wave step through their j loops together, so each step lasts as long as
+// the slowest Kepler solve among the wave's pairs at that step; grouping
+// systems of similar size keeps the lanes' loop lengths close.  Every lane
+// stays in the wave to the end (inactive lanes compute nothing), so the
+// wave reductions of the work counters see all 64 lanes.
+//
+// With a Kepler-guess network set on the plan (ensemble_plan_set_guess)
+// the launch takes the learned form: each j step first evaluates the
+// network on the wave's 64 pairs (mlp_wave.h, the same inference as the
+// Sakura near pass) and every pair's first solve starts from its guess.
+// The network needs all 64 lanes, so each lane runs the wave's longest j
+// loop, idle past its own range.
 //
 // The Sakura drift runs each plan wave as one workgroup of kEnsParts waves:
 // wave p takes the p-th of kEnsParts consecutive j ranges of the same lanes,
@@ -21,9 +29,11 @@
 //
 // Compiled with -ffp-contract=off (reference operation order).
 #include <algorithm>
+#include <cstdint>
 #include <vector>
 
 #include "kepler.h"
+#include "mlp_wave.h"
 #include "tupan_internal.h"
 
 namespace tupan {
@@ -44,6 +54,8 @@ struct EnsPlanImpl {
     int* d_order = nullptr;
     EnsWave* d_waves = nullptr;
     int device = -1;
+    const NetPacked* net = nullptr;           // Kepler-guess network or null
+    unsigned long long* fallbacks = nullptr;  // guess fallback counter or null
 };
 
 struct EnsArgs {
@@ -57,6 +69,8 @@ struct EnsArgs {
     long flag;
     unsigned long long* counters;
     int* status;
+    const NetPacked* net;
+    unsigned long long* fallbacks;
 };
 
 __device__ __forceinline__ unsigned int ens_wave_sum(unsigned int v)
@@ -93,7 +107,54 @@ __device__ __forceinline__ EnsLane ens_lane(const EnsArgs& a, int wave, int lane
 
 constexpr int kEnsParts = 4;
 
-__global__ __launch_bounds__(kWave * kEnsParts) void ensemble_sakura_kernel(EnsArgs a)
+// The i particle of a lane.
+struct EnsI {
+    double m, x, y, z, e2, vx, vy, vz;
+};
+
+// One pair drift of particle i against j (sakura_kernel_common.h:194-243),
+// its increments weighted by m_j / (m_i + m_j) added to d.
+__device__ __forceinline__ void ens_pair_drift(const EnsArgs& a, const EnsI& I, int j,
+                                               const kep::Guess* guess, kep::Counters& cnt,
+                                               double d[6])
+{
+    const double jm = a.p[0][j];
+    const double p0[3] = {I.x - a.p[1][j], I.y - a.p[2][j], I.z - a.p[3][j]};
+    const double e2 = I.e2 + a.p[4][j];
+    const double v0[3] = {I.vx - a.p[5][j], I.vy - a.p[6][j], I.vz - a.p[7][j]};
+    const double m = I.m + jm;
+    double p[3] = {p0[0], p0[1], p0[2]};
+    double v[3] = {v0[0], v0[1], v0[2]};
+    kep::drift_twobody(a.dt, a.flag, m, e2, p, v, cnt, guess);
+    const double mu = jm / m;
+    d[0] += mu * (p[0] - p0[0]);
+    d[1] += mu * (p[1] - p0[1]);
+    d[2] += mu * (p[2] - p0[2]);
+    d[3] += mu * (v[0] - v0[0]);
+    d[4] += mu * (v[1] - v0[1]);
+    d[5] += mu * (v[2] - v0[2]);
+}
+
+// The network inputs of pair (i, j): the first twobody stage's features
+// (kepler.h: SolveFeatures, NaN when that stage has no solve setup),
+// prepared as mlp_wave.h expects.
+__device__ __forceinline__ void ens_guess_inputs(const EnsArgs& a, const EnsI& I, int j,
+                                                 const NetPacked* net, float xs[4])
+{
+    double q[3] = {I.x - a.p[1][j], I.y - a.p[2][j], I.z - a.p[3][j]};
+    const double e2 = I.e2 + a.p[4][j];
+    const double v0[3] = {I.vx - a.p[5][j], I.vy - a.p[6][j], I.vz - a.p[7][j]};
+    const double m = I.m + a.p[0][j];
+    const double dts = kep::first_stage(a.dt, a.flag, q, v0);
+    double k[5], s0;
+    kep::SolveFeatures f;
+    if (!kep::solve_setup(dts, m, e2, q, v0, k, s0, &f))
+        f.tau = f.sigma = f.kappa = f.eps = __builtin_nanf("");
+    prepare_features(f.tau, f.sigma, f.kappa, f.eps, net->mean, net->inv_std, xs);
+}
+
+template <bool kLearned>
+__device__ __forceinline__ void ensemble_sakura_body(const EnsArgs& a)
 {
     __shared__ double part_sum[kEnsParts - 1][6][kWave];
     __shared__ unsigned int part_cnt[kEnsParts][4];
@@ -104,30 +165,54 @@ __global__ __launch_bounds__(kWave * kEnsParts) void ensemble_sakura_kernel(EnsA
     const EnsLane L = ens_lane(a, wave, lane);
     kep::Counters cnt;
     double d[6] = {0, 0, 0, 0, 0, 0};
+    int ja = 0, jb = 0;
+    EnsI I{};
     if (L.active) {
         const int i = L.i;
         const int nj = L.j1 - L.j0;
-        const int ja = L.j0 + (nj * part) / kEnsParts;
-        const int jb = L.j0 + (nj * (part + 1)) / kEnsParts;
-        const double im = a.p[0][i], ix = a.p[1][i], iy = a.p[2][i], iz = a.p[3][i];
-        const double ie2 = a.p[4][i], ivx = a.p[5][i], ivy = a.p[6][i], ivz = a.p[7][i];
-        for (int j = ja; j < jb; ++j) {
-            const double jm = a.p[0][j];
-            const double p0[3] = {ix - a.p[1][j], iy - a.p[2][j], iz - a.p[3][j]};
-            const double e2 = ie2 + a.p[4][j];
-            const double v0[3] = {ivx - a.p[5][j], ivy - a.p[6][j], ivz - a.p[7][j]};
-            const double m = im + jm;
-            double p[3] = {p0[0], p0[1], p0[2]};
-            double v[3] = {v0[0], v0[1], v0[2]};
-            kep::drift_twobody(a.dt, a.flag, m, e2, p, v, cnt);
-            const double mu = jm / m;
-            d[0] += mu * (p[0] - p0[0]);
-            d[1] += mu * (p[1] - p0[1]);
-            d[2] += mu * (p[2] - p0[2]);
-            d[3] += mu * (v[0] - v0[0]);
-            d[4] += mu * (v[1] - v0[1]);
-            d[5] += mu * (v[2] - v0[2]);
+        ja = L.j0 + (nj * part) / kEnsParts;
+        jb = L.j0 + (nj * (part + 1)) / kEnsParts;
+        I = EnsI{a.p[0][i], a.p[1][i], a.p[2][i], a.p[3][i],
+                 a.p[4][i], a.p[5][i], a.p[6][i], a.p[7][i]};
+    }
+    if constexpr (kLearned) {
+        __shared__ __align__(16) NetPacked lnet;
+        mlp_stage(a.net, &lnet);
+        int steps = jb - ja;
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1)
+            steps = max(steps, __shfl_xor(steps, off, kWave));
+        for (int k = 0; k < steps; ++k) {
+            // the weights are read from LDS as the network uses them, not
+            // hoisted out of the j loop and kept live across the solves
+            __asm__ volatile("" ::: "memory");
+            const bool has = ja + k < jb;
+            float xs[4] = {0.f, 0.f, 0.f, 0.f};
+            if (has) ens_guess_inputs(a, I, ja + k, &lnet, xs);
+            // lanes l and l + 32 supply the B operands of items l mod 32
+            // and 32 + l mod 32
+            float x[2][4];
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    x[c][q] = __shfl(xs[q], 32 * c + (lane & 31), kWave);
+            }
+            float y[2];
+            mlp_wave_eval(&lnet, x, lane, y);
+            const float dl = mlp_delta(&lnet, lane < 32 ? y[0] : y[1]);
+            if (has) {
+                kep::Guess gs;
+                gs.use = dl == dl && dl > -0.9f && dl < 8.0f;
+                gs.delta = static_cast<double>(dl);
+                ens_pair_drift(a, I, ja + k, &gs, cnt, d);
+            }
         }
+        const unsigned int fb = ens_wave_sum(L.active ? cnt.fallbacks : 0u);
+        if (lane == 0 && fb && a.fallbacks)
+            atomicAdd(a.fallbacks, static_cast<unsigned long long>(fb));
+    } else {
+        for (int j = ja; j < jb; ++j) ens_pair_drift(a, I, j, nullptr, cnt, d);
     }
     if (part > 0) {
 #pragma unroll
@@ -172,6 +257,18 @@ __global__ __launch_bounds__(kWave * kEnsParts) void ensemble_sakura_kernel(EnsA
         }
         if (a.status && fail) atomicOr(a.status, 1);
     }
+}
+
+__global__ __launch_bounds__(kWave * kEnsParts) void ensemble_sakura_kernel(EnsArgs a)
+{
+    ensemble_sakura_body<false>(a);
+}
+
+// The learned form held to two waves per SIMD (the network's registers
+// would otherwise take it to one).
+__global__ __launch_bounds__(kWave * kEnsParts, 2) void ensemble_sakura_learned_kernel(EnsArgs a)
+{
+    ensemble_sakura_body<true>(a);
 }
 
 // Potential of each particle due to its own system (phi_kernel_common.h:7-32),
@@ -289,6 +386,17 @@ int ensemble_plan_info(const void* plan, int* nsys, int* ntot, int* nwaves)
     return TUPAN_OK;
 }
 
+int ensemble_plan_set_guess(void* plan, const void* net, unsigned long long* fallbacks)
+{
+    EnsPlanImpl* p = static_cast<EnsPlanImpl*>(plan);
+    if (!p) return set_error(TUPAN_ERR_ARG, "null plan");
+    if (reinterpret_cast<uintptr_t>(net) % 16 != 0)
+        return set_error(TUPAN_ERR_ARG, "network block must be 16-byte aligned");
+    p->net = static_cast<const NetPacked*>(net);
+    p->fallbacks = net ? fallbacks : nullptr;
+    return TUPAN_OK;
+}
+
 static int ens_args(const void* plan, const double* const* in8, EnsArgs& a)
 {
     const EnsPlanImpl* p = static_cast<const EnsPlanImpl*>(plan);
@@ -298,6 +406,8 @@ static int ens_args(const void* plan, const double* const* in8, EnsArgs& a)
     a.order = p->d_order;
     a.waves = p->d_waves;
     a.nwaves = p->nwaves;
+    a.net = p->net;
+    a.fallbacks = p->fallbacks;
     for (int f = 0; f < 8; ++f) {
         if (p->ntot > 0 && !in8[f]) return set_error(TUPAN_ERR_ARG, "null input array");
         a.p[f] = in8[f];
@@ -324,8 +434,12 @@ int ensemble_sakura_call(const void* plan, const double* const* in8, double dt, 
     if (a.nwaves == 0) return TUPAN_OK;
     hipStream_t s = static_cast<hipStream_t>(stream);
     timing_begin(s);
-    hipLaunchKernelGGL(ensemble_sakura_kernel, dim3(a.nwaves), dim3(kWave * kEnsParts), 0, s,
-                       a);
+    if (a.net)
+        hipLaunchKernelGGL(ensemble_sakura_learned_kernel, dim3(a.nwaves), dim3(kWave * kEnsParts),
+                           0, s, a);
+    else
+        hipLaunchKernelGGL(ensemble_sakura_kernel, dim3(a.nwaves), dim3(kWave * kEnsParts),
+                           0, s, a);
     timing_end(s);
     return check_launch("ensemble_sakura");
 }

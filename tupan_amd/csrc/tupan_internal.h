@@ -176,6 +176,7 @@ int mlp_infer(int n, const float* feat, const void* net, float* delta, hipStream
 int ensemble_plan_create(int nsys, const int* h_offsets, int min_lanes, void** plan_out);
 int ensemble_plan_destroy(void* plan);
 int ensemble_plan_info(const void* plan, int* nsys, int* ntot, int* nwaves);
+int ensemble_plan_set_guess(void* plan, const void* net, unsigned long long* fallbacks);
 int ensemble_sakura_call(const void* plan, const double* const* in8, double dt, long flag,
                          double* const* out6, unsigned long long* counters, int* status,
                          void* stream);

@@ -36,6 +36,8 @@ class Ensemble(object):
         self._plan = plan
         self._status = torch.zeros(1, dtype=torch.int32, device=hip_device())
         self.counters = None
+        self._net = None
+        self.fallbacks = None
         ns = ctypes.c_int()
         nt = ctypes.c_int()
         nw = ctypes.c_int()
@@ -45,6 +47,22 @@ class Ensemble(object):
         self.nwaves = nw.value
         self.system_of = torch.as_tensor(
             np.repeat(np.arange(self.nsys), sizes), device=hip_device())
+
+    def set_guess(self, net=None):
+        """Use the Kepler-guess network ``net`` (a KeplerNet, or None for
+        the plain guess) in this plan's Sakura calls; its fallbacks are
+        counted in ``self.fallbacks`` (a device counter)."""
+        self._net = net
+        if net is None:
+            self.fallbacks = None
+            rc = self.lib.tupan_ensemble_set_guess(self._plan, None, None)
+        else:
+            self.fallbacks = torch.zeros(1, dtype=torch.int64,
+                                         device=hip_device())
+            rc = self.lib.tupan_ensemble_set_guess(
+                self._plan, net.packed.data_ptr(),
+                self.fallbacks.data_ptr())
+        _native.check(rc, "tupan_ensemble_set_guess")
 
     def close(self):
         if getattr(self, "_plan", None) is not None and self._plan.value:
