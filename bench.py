@@ -68,7 +68,7 @@ SAKURA_FAR_HW = {"valu_per_pair": 126.5, "fp64_flops_per_pair": 127.0}
 # Ensemble Sakura kernel (4096 systems, one pair-drift stage): hardware
 # counts per pair drift from the same profile (nearly every pair is a
 # Kepler solve, whose iterations the reference does not count).
-ENSEMBLE_HW = {"valu_per_pair": 2307.0, "fp64_flops_per_pair": 2090.0}
+ENSEMBLE_HW = {"valu_per_pair": 2378.8, "fp64_flops_per_pair": 2097.2}
 CLOCK_PEAK = 2.4e9           # MI355X peak engine clock
 SIMDS = 1024                 # 256 CUs x 4 SIMDs
 FP64_CYCLES_PER_WAVE_INSTR = 4
